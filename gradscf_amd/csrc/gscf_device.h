@@ -1,0 +1,153 @@
+// Device-side building blocks of the MI355X direct-J/K engine (gfx950, wave64).
+//
+// Arithmetic follows the Rys-quadrature form of the reference's ERI kernels
+// (libcint, src/gradscf/integrals/_native/vendor/libcint/src/g2e.c:4425-4569 for
+// the root/weight and B00/B10/B01/C00/C0p set-up, g2e.c:272-690 for the 2D
+// recurrences and transfers, cint2e.c:660-758,961 for assembly/contraction),
+// restructured for the GPU: one angular-momentum class per template
+// instantiation, everything unrolled into registers, and the density
+// digestion of csrc/direct_jk.cc:45-76 fused into the same kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "_generated/rys_table.inc"
+#include "gscf_args.h"
+
+namespace gscf {
+
+constexpr int kMaxL = 2;  // shells up to d in the J/K kernels
+
+__device__ static const double rys_tab[] = {
+#include "_generated/rys_table_body.inc"
+};
+
+// ---------------------------------------------------------------------------
+// Rys roots (s = t^2) and weights for weight exp(-x t^2) on [0,1]
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void rys_roots(double x, double* __restrict__ s, double* __restrict__ w) {
+  if (x < RYS_XMAX) {
+    const int k = (int)(x * (1.0 / RYS_H));
+    const double y = x - (RYS_H * k + 0.5 * RYS_H);
+    const double* __restrict__ c = rys_tab + rys_table_offset[N - 1] + k * ((RYS_DEG + 1) * 2 * N);
+    double b1[2 * N], b2[2 * N];
+#pragma unroll
+    for (int f = 0; f < 2 * N; ++f) { b1[f] = 0.0; b2[f] = 0.0; }
+    const double y2 = 2.0 * y;
+#pragma unroll
+    for (int j = RYS_DEG; j >= 1; --j) {
+#pragma unroll
+      for (int f = 0; f < 2 * N; ++f) {
+        const double b0 = fma(y2, b1[f], c[j * 2 * N + f] - b2[f]);
+        b2[f] = b1[f];
+        b1[f] = b0;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < N; ++f) {
+      s[f] = fma(y, b1[f], c[f] - b2[f]);
+      w[f] = fma(y, b1[N + f], c[N + f] - b2[N + f]);
+    }
+  } else {
+    const double ix = 1.0 / x;
+    const double sq = sqrt(ix);
+    constexpr int base = N * (N - 1) / 2;
+#pragma unroll
+    for (int f = 0; f < N; ++f) {
+      s[f] = rys_asym_root_c[base + f] * ix;
+      w[f] = rys_asym_weight_c[base + f] * sq;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cartesian components in libcint order (descending x, then descending y)
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+__host__ __device__ constexpr int cart_comp(int l, int i, int axis) {
+  int n = 0;
+  for (int lx = l; lx >= 0; --lx)
+    for (int ly = l - lx; ly >= 0; --ly) {
+      if (n == i) return axis == 0 ? lx : (axis == 1 ? ly : l - lx - ly);
+      ++n;
+    }
+  return 0;
+}
+
+__host__ __device__ constexpr int binom(int n, int k) {
+  int r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Vertical recurrence for one axis and one root:
+//   G(n+1,0) = C00 G(n,0) + n B10 G(n-1,0)
+//   G(n,m+1) = C0p G(n,m) + m B01 G(n,m-1) + n B00 G(n-1,m)
+// ---------------------------------------------------------------------------
+template <int NM, int MM>
+__device__ __forceinline__ void vrr(double c00, double c0p, double b10, double b01, double b00,
+                                    double base, double (&G)[NM + 1][MM + 1]) {
+  G[0][0] = base;
+#pragma unroll
+  for (int n = 0; n < NM; ++n) G[n + 1][0] = c00 * G[n][0] + (n ? n * b10 * G[n - 1][0] : 0.0);
+#pragma unroll
+  for (int m = 0; m < MM; ++m)
+#pragma unroll
+    for (int n = 0; n <= NM; ++n)
+      G[n][m + 1] = c0p * G[n][m] + (m ? m * b01 * G[n][m - 1] : 0.0) + (n ? n * b00 * G[n - 1][m] : 0.0);
+}
+
+// Horizontal transfer (x - B) = (x - A) + (A - B) on the bra index of a 1D vector.
+template <int LA, int LB>
+__device__ __forceinline__ void hrr_bra1(const double (&g)[LA + LB + 1], double ab,
+                                        double (&h)[LA + 1][LB + 1]) {
+  double t[LB + 1][LA + LB + 1];
+#pragma unroll
+  for (int n = 0; n <= LA + LB; ++n) t[0][n] = g[n];
+#pragma unroll
+  for (int b = 1; b <= LB; ++b)
+#pragma unroll
+    for (int a = 0; a <= LA + LB - b; ++a) t[b][a] = t[b - 1][a + 1] + ab * t[b - 1][a];
+#pragma unroll
+  for (int a = 0; a <= LA; ++a)
+#pragma unroll
+    for (int b = 0; b <= LB; ++b) h[a][b] = t[b][a];
+}
+
+// Full 4-index transfer of one axis: H[a][b][c][d] from G[n][m].
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void hrr_full(const double (&G)[LA + LB + 1][LC + LD + 1], double ab, double cd,
+                                         double (&H)[LA + 1][LB + 1][LC + 1][LD + 1]) {
+  double X[LA + 1][LB + 1][LC + LD + 1];
+#pragma unroll
+  for (int m = 0; m <= LC + LD; ++m) {
+    double g[LA + LB + 1];
+#pragma unroll
+    for (int n = 0; n <= LA + LB; ++n) g[n] = G[n][m];
+    double h[LA + 1][LB + 1];
+    hrr_bra1<LA, LB>(g, ab, h);
+#pragma unroll
+    for (int a = 0; a <= LA; ++a)
+#pragma unroll
+      for (int b = 0; b <= LB; ++b) X[a][b][m] = h[a][b];
+  }
+#pragma unroll
+  for (int a = 0; a <= LA; ++a)
+#pragma unroll
+    for (int b = 0; b <= LB; ++b) {
+      double g[LC + LD + 1];
+#pragma unroll
+      for (int m = 0; m <= LC + LD; ++m) g[m] = X[a][b][m];
+      double h[LC + 1][LD + 1];
+      hrr_bra1<LC, LD>(g, cd, h);
+#pragma unroll
+      for (int c = 0; c <= LC; ++c)
+#pragma unroll
+        for (int d = 0; d <= LD; ++d) H[a][b][c][d] = h[c][d];
+    }
+}
+
+}  // namespace gscf

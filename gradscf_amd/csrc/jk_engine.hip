@@ -1,0 +1,1031 @@
+// Host side of the MI355X direct-J/K engine: table validation, device plan,
+// Schwarz bounds, class task lists, density / J / K transforms, C ABI.
+//
+// Reference semantics followed:
+//   table validation and AO offsets      src/gradscf/integrals/_native/csrc/tables.h:31-76
+//   Schwarz bounds Q_ij                   csrc/coulomb_shells.h:60-77
+//   pair prune / quartet skip tests       csrc/coulomb_shells.h:98-110
+//   operator selection, shape, finite-density checks, output layout
+//                                         csrc/direct_jk.cc:15-28,51,64,97-102
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gradscf_jk.h"
+#include "gscf_args.h"
+
+namespace gscf {
+
+// ----------------------------------------------------------------------------
+// errors
+// ----------------------------------------------------------------------------
+static thread_local std::string g_error;
+
+struct InvalidArgument : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct Internal : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      throw Internal(std::string("HIP error: ") + hipGetErrorString(_e) + " at " #expr); \
+  } while (0)
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return GSCF_OK;
+  } catch (const InvalidArgument& e) {
+    g_error = e.what();
+    return GSCF_INVALID_ARGUMENT;
+  } catch (const std::invalid_argument& e) {
+    g_error = e.what();
+    return GSCF_INVALID_ARGUMENT;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return GSCF_INTERNAL;
+  }
+}
+
+// libcint table slots
+constexpr int ATM_SLOTS = 6, BAS_SLOTS = 8;
+constexpr int CHARGE_OF = 0, PTR_COORD = 1, NUC_MOD_OF = 2;
+constexpr int ATOM_OF = 0, ANG_OF = 1, NPRIM_OF = 2, NCTR_OF = 3, KAPPA_OF = 4, PTR_EXP = 5, PTR_COEFF = 6;
+constexpr int PTR_ENV_START = 20, PTR_RANGE_OMEGA = 8;
+constexpr int kMaxL = 2;
+
+static int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
+
+// Cartesian -> real spherical coefficients for l <= 2 (rows: spherical m in
+// PySCF order; columns: Cartesian components in libcint order).
+static void c2s_rows(int l, std::vector<std::vector<double>>& m) {
+  const double pi = 3.14159265358979323846;
+  if (l <= 1) {
+    m.assign(ncart_h(l), std::vector<double>(ncart_h(l), 0.0));
+    for (int i = 0; i < ncart_h(l); ++i) m[i][i] = 1.0;
+    return;
+  }
+  const double a = std::sqrt(15.0 / (4.0 * pi)), b = std::sqrt(5.0 / (16.0 * pi)), c = std::sqrt(15.0 / (16.0 * pi));
+  m.assign(5, std::vector<double>(6, 0.0));
+  m[0][1] = a;                                   // xy
+  m[1][4] = a;                                   // yz
+  m[2][0] = -b; m[2][3] = -b; m[2][5] = 2.0 * b;  // z^2
+  m[3][2] = a;                                   // xz
+  m[4][0] = c; m[4][3] = -c;                      // x^2 - y^2
+}
+
+template <class T>
+static T* dev_upload(const std::vector<T>& v, hipStream_t s) {
+  if (v.empty()) return nullptr;
+  T* p = nullptr;
+  HIP_OK(hipMalloc(&p, sizeof(T) * v.size()));
+  HIP_OK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+  return p;
+}
+
+struct Seg {
+  int parent, ctr, l, np;
+  double r[3];
+  int ao_cart;
+  std::vector<double> a, c;  // exponents, coefficients * s/p factor
+};
+
+struct QuartetClass {
+  int la, lb, lc, ld;
+  int bra0, ket0, nbra, same_class;
+  int64_t nq;
+  int64_t* d_qoff;
+};
+
+struct PlanData {
+  int cart = 1, n = 0, n_cart = 0, nbas = 0, natm = 0;
+  double cutoff = 0.0, plan_ms = 0.0;
+  std::vector<int> atm, bas, ao, ao_cart;
+  std::vector<double> env;
+  std::vector<Seg> seg;
+  std::vector<PrimPair> prims;
+  // all segmented pairs (unscreened, unsorted) for block queries
+  std::vector<Pair> all_pairs;
+  std::vector<int> all_class;
+  std::map<std::pair<int, int>, int> pair_index;  // (S,T) S>=T -> all_pairs index
+  // screened, class-sorted pairs
+  std::vector<Pair> pairs;
+  std::vector<double> pair_q;
+  int class_begin[kNumPairClasses + 1] = {0};
+  std::vector<QuartetClass> classes;
+  int64_t nquartets = 0;
+  // device
+  ShellXYZ* d_shells = nullptr;
+  PrimPair* d_prims = nullptr;
+  Pair* d_pairs = nullptr;
+  Pair* d_all_pairs = nullptr;
+  int* d_c2s_idx = nullptr;     // [n_cart][2] spherical AO feeding a Cartesian AO
+  double* d_c2s_coef = nullptr;
+  int* d_s2c_idx = nullptr;     // [n][3] Cartesian AOs feeding a spherical AO
+  double* d_s2c_coef = nullptr;
+  // workspace
+  double* d_dc = nullptr;
+  double* d_jp = nullptr;
+  double* d_kp = nullptr;
+  size_t ws_nbd = 0;
+  int* d_flag = nullptr;
+  int* h_flag = nullptr;
+  // one-electron: all segmented pairs sorted by class, nuclei (x, y, z, Z)
+  Pair* d_pairs1e = nullptr;
+  int p1e_begin[kNumPairClasses + 1] = {0};
+  double* d_nuclei = nullptr;
+  int nnuc = 0;
+  // parent-shell Schwarz bounds (nbas x nbas, lower triangle; empty when cutoff == 0)
+  std::vector<double> parent_q;
+  // optional per-class kernel timing with HIP events on the launch stream
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> prof_recs;  // (class index, event pair offset)
+};
+
+// Per-class launchers, one translation unit each (jk_class.hip).
+#define GSCF_CLASSES(X)                                                               \
+  X(0, 0, 0, 0) X(1, 0, 0, 0) X(1, 0, 1, 0) X(1, 1, 0, 0) X(1, 1, 1, 0) X(1, 1, 1, 1) \
+  X(2, 0, 0, 0) X(2, 0, 1, 0) X(2, 0, 1, 1) X(2, 0, 2, 0) X(2, 1, 0, 0) X(2, 1, 1, 0) \
+  X(2, 1, 1, 1) X(2, 1, 2, 0) X(2, 1, 2, 1) X(2, 2, 0, 0) X(2, 2, 1, 0) X(2, 2, 1, 1) \
+  X(2, 2, 2, 0) X(2, 2, 2, 1) X(2, 2, 2, 2)
+#define X(A, B, C, D)                                                       \
+  void launch_jk_c##A##B##C##D(const JKArgs& a, hipStream_t s);            \
+  void launch_blocks_c##A##B##C##D(const BlockArgs& a, hipStream_t s);
+GSCF_CLASSES(X)
+#undef X
+
+void launch_int1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
+                        const ShellXYZ* shells, const double* nuclei, int nnuc, int nc, double* s, double* t,
+                        double* v, hipStream_t st);
+
+void launch_jk_class(int la, int lb, int lc, int ld, const JKArgs& a, hipStream_t s) {
+#define X(A, B, C, D)                                      \
+  if (la == A && lb == B && lc == C && ld == D) {          \
+    launch_jk_c##A##B##C##D(a, s);                         \
+    return;                                                \
+  }
+  GSCF_CLASSES(X)
+#undef X
+  throw Internal("no kernel for this angular-momentum class");
+}
+
+void launch_blocks_class(int la, int lb, int lc, int ld, const BlockArgs& a, hipStream_t s) {
+#define X(A, B, C, D)                                      \
+  if (la == A && lb == B && lc == C && ld == D) {          \
+    launch_blocks_c##A##B##C##D(a, s);                     \
+    return;                                                \
+  }
+  GSCF_CLASSES(X)
+#undef X
+  throw Internal("no kernel for this angular-momentum class");
+}
+
+}  // namespace gscf
+
+struct gscf_plan {
+  gscf::PlanData d;
+};
+
+namespace gscf {
+
+// ----------------------------------------------------------------------------
+// small device kernels
+// ----------------------------------------------------------------------------
+__global__ void fill_kernel(double* p, size_t count, double v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+__global__ void finite_kernel(const double* p, size_t count, int* flag) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count && !isfinite(p[i])) atomicOr(flag, 1);
+}
+
+// Dc[b][P][Q] = sum T[p,P] T[q,Q] D[src b][p][q] (or D^T when transpose)
+__global__ void dens_to_cart_kernel(const double* __restrict__ D, int n, int nb, double* __restrict__ Dc, int nc,
+                                    const int* __restrict__ idx, const double* __restrict__ coef, int nbd) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)nbd * nc * nc;
+  if (i >= total) return;
+  const int bt = (int)(i / ((size_t)nc * nc));
+  const int rem = (int)(i - (size_t)bt * nc * nc);
+  const int P = rem / nc, Q = rem - (rem / nc) * nc;
+  const bool tr = bt >= nb;
+  const double* Db = D + (size_t)(tr ? bt - nb : bt) * n * n;
+  double s = 0.0;
+  for (int e1 = 0; e1 < 2; ++e1) {
+    const double c1 = coef[P * 2 + e1];
+    if (c1 == 0.0) continue;
+    const int p = idx[P * 2 + e1];
+    for (int e2 = 0; e2 < 2; ++e2) {
+      const double c2 = coef[Q * 2 + e2];
+      if (c2 == 0.0) continue;
+      const int q = idx[Q * 2 + e2];
+      s += c1 * c2 * (tr ? Db[(size_t)q * n + p] : Db[(size_t)p * n + q]);
+    }
+  }
+  Dc[i] = s;
+}
+
+// out[op][b][p][q]: J = T (Jp + Jp^T) T^T, K = T (Kp_b + Kp_b'^T) T^T
+__global__ void finalize_kernel(const double* __restrict__ jp, const double* __restrict__ kp, int nc, int n, int nb,
+                                int sym, int with_j, int with_k, const int* __restrict__ idx,
+                                const double* __restrict__ coef, double* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nops = with_j + with_k;
+  const size_t total = (size_t)nops * nb * n * n;
+  if (i >= total) return;
+  const int opb = (int)(i / ((size_t)n * n));
+  const int rem = (int)(i - (size_t)opb * n * n);
+  const int p = rem / n, q = rem - (rem / n) * n;
+  const int op = opb / nb, b = opb - (opb / nb) * nb;
+  const bool isj = with_j && op == 0;
+  const double* A = isj ? jp + (size_t)b * nc * nc : kp + (size_t)b * nc * nc;
+  const double* B = isj ? A : kp + (size_t)(sym ? b : nb + b) * nc * nc;
+  double s = 0.0;
+  for (int e1 = 0; e1 < 3; ++e1) {
+    const double c1 = coef[p * 3 + e1];
+    if (c1 == 0.0) continue;
+    const int P = idx[p * 3 + e1];
+    for (int e2 = 0; e2 < 3; ++e2) {
+      const double c2 = coef[q * 3 + e2];
+      if (c2 == 0.0) continue;
+      const int Q = idx[q * 3 + e2];
+      s += c1 * c2 * (A[(size_t)P * nc + Q] + B[(size_t)Q * nc + P]);
+    }
+  }
+  out[i] = s;
+}
+
+// out[p][q] = sum T[p,P] T[q,Q] M[P][Q] (Cartesian -> requested AO basis)
+__global__ void sandwich_kernel(const double* __restrict__ M, int nc, int n, const int* __restrict__ idx,
+                                const double* __restrict__ coef, double* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * n) return;
+  const int p = (int)(i / n), q = (int)(i - (size_t)(i / n) * n);
+  double s = 0.0;
+  for (int e1 = 0; e1 < 3; ++e1) {
+    const double c1 = coef[p * 3 + e1];
+    if (c1 == 0.0) continue;
+    const int P = idx[p * 3 + e1];
+    for (int e2 = 0; e2 < 3; ++e2) {
+      const double c2 = coef[q * 3 + e2];
+      if (c2 == 0.0) continue;
+      s += c1 * c2 * M[(size_t)P * nc + idx[q * 3 + e2]];
+    }
+  }
+  out[i] = s;
+}
+
+static unsigned nblocks(size_t count) { return (unsigned)((count + 255) / 256); }
+
+// ----------------------------------------------------------------------------
+// plan construction
+// ----------------------------------------------------------------------------
+static void validate_tables(PlanData& P, const int32_t* atm, int natm, const int32_t* bas, int nbas,
+                            const double* env, int nenv, int cart) {
+  if (!atm || !bas || !env || natm < 0 || nbas < 0 || nenv < 20 || (cart != 0 && cart != 1))
+    throw InvalidArgument("Invalid libcint table dimensions or operator");
+  P.natm = natm;
+  P.nbas = nbas;
+  P.cart = cart;
+  P.atm.assign(atm, atm + (size_t)natm * ATM_SLOTS);
+  P.bas.assign(bas, bas + (size_t)nbas * BAS_SLOTS);
+  P.env.assign(env, env + nenv);
+  for (double v : P.env)
+    if (!std::isfinite(v)) throw InvalidArgument("Native integral env must be finite");
+  if (P.env[PTR_RANGE_OMEGA] > std::sqrt(std::numeric_limits<double>::max()) / 2.)
+    throw InvalidArgument("omega exceeds the safe float64 squared range");
+  if (P.env[PTR_RANGE_OMEGA] != 0.)
+    throw InvalidArgument("Range-separated integrals require the full ERI path and nonnegative omega");
+  auto inside = [&](int ptr, int64_t count) {
+    return ptr >= PTR_ENV_START && count > 0 && int64_t(ptr) + count <= int64_t(nenv);
+  };
+  for (int a = 0; a < natm; ++a) {
+    const int* row = P.atm.data() + a * ATM_SLOTS;
+    if (!inside(row[PTR_COORD], 3) || row[NUC_MOD_OF] != 1)
+      throw InvalidArgument("Native backend requires valid point nuclei");
+  }
+  P.ao.assign(nbas + 1, 0);
+  P.ao_cart.assign(nbas + 1, 0);
+  for (int s = 0; s < nbas; ++s) {
+    const int* row = P.bas.data() + s * BAS_SLOTS;
+    const int l = row[ANG_OF], np = row[NPRIM_OF], nc = row[NCTR_OF];
+    if (row[ATOM_OF] < 0 || row[ATOM_OF] >= natm || l < 0 || l > 12 || np < 1 || np > 64 || nc < 1 ||
+        nc > 64 || row[KAPPA_OF] != 0 || !inside(row[PTR_EXP], np) || !inside(row[PTR_COEFF], int64_t(np) * nc))
+      throw InvalidArgument("Invalid or unsupported libcint basis table");
+    for (int p = 0; p < np; ++p)
+      if (P.env[row[PTR_EXP] + p] <= 0.) throw InvalidArgument("Gaussian exponents must be positive");
+    if (l > kMaxL)
+      throw InvalidArgument("The MI355X J/K engine supports shells up to d (l <= 2)");
+    P.ao[s + 1] = P.ao[s] + nc * (cart ? ncart_h(l) : 2 * l + 1);
+    P.ao_cart[s + 1] = P.ao_cart[s] + nc * ncart_h(l);
+  }
+  P.n = P.ao.back();
+  P.n_cart = P.ao_cart.back();
+  if (P.n < 1) throw InvalidArgument("Invalid AO count");
+}
+
+static void build_segments(PlanData& P) {
+  const double pi = 3.14159265358979323846;
+  for (int s = 0; s < P.nbas; ++s) {
+    const int* row = P.bas.data() + s * BAS_SLOTS;
+    const int l = row[ANG_OF], np = row[NPRIM_OF], nc = row[NCTR_OF];
+    const double fsp = l == 0 ? 0.5 / std::sqrt(pi) : (l == 1 ? std::sqrt(3.0 / (4.0 * pi)) : 1.0);
+    const double* r = P.env.data() + P.atm[row[ATOM_OF] * ATM_SLOTS + PTR_COORD];
+    for (int c = 0; c < nc; ++c) {
+      Seg g;
+      g.parent = s;
+      g.ctr = c;
+      g.l = l;
+      g.np = np;
+      g.r[0] = r[0]; g.r[1] = r[1]; g.r[2] = r[2];
+      g.ao_cart = P.ao_cart[s] + c * ncart_h(l);
+      for (int p = 0; p < np; ++p) {
+        g.a.push_back(P.env[row[PTR_EXP] + p]);
+        g.c.push_back(P.env[row[PTR_COEFF] + c * np + p] * fsp);
+      }
+      P.seg.push_back(std::move(g));
+    }
+  }
+}
+
+// Primitive pairs of segmented shells (A, B) with l(A) >= l(B).  Pairs whose
+// Gaussian-product factor exp(-mu R^2) < e^{-60} are dropped (the primitive
+// screening threshold libcint applies by default, g2e.c:57-62).
+static void make_pair(PlanData& P, int sa, int sb, Pair& out) {
+  const Seg& A = P.seg[sa];
+  const Seg& B = P.seg[sb];
+  const double pi = 3.14159265358979323846;
+  const double s2p = std::sqrt(2.0 * std::pow(pi, 2.5));
+  double ab2 = 0;
+  for (int x = 0; x < 3; ++x) ab2 += (A.r[x] - B.r[x]) * (A.r[x] - B.r[x]);
+  out.sa = sa;
+  out.sb = sb;
+  out.poff = (int)P.prims.size();
+  out.np = 0;
+  for (int i = 0; i < A.np; ++i)
+    for (int j = 0; j < B.np; ++j) {
+      const double ai = A.a[i], aj = B.a[j], p = ai + aj;
+      const double mu = ai * aj / p * ab2;
+      if (mu > 60.0) continue;
+      PrimPair pp;
+      pp.p = p;
+      pp.px = (ai * A.r[0] + aj * B.r[0]) / p;
+      pp.py = (ai * A.r[1] + aj * B.r[1]) / p;
+      pp.pz = (ai * A.r[2] + aj * B.r[2]) / p;
+      pp.e = A.c[i] * B.c[j] * std::exp(-mu) * s2p / p;
+      pp.aa = ai;
+      pp.ab = aj;
+      pp.pad = 0;
+      P.prims.push_back(pp);
+      ++out.np;
+    }
+}
+
+// Runs the block kernel over tasks grouped by class; tasks must be class-sorted.
+static void run_blocks(PlanData& P, const Pair* d_pairs, const std::vector<BlockTask>& tasks,
+                       const std::vector<std::array<int, 4>>& task_class, double* d_out, hipStream_t s) {
+  if (tasks.empty()) return;
+  BlockTask* d_tasks = dev_upload(tasks, s);
+  size_t i = 0;
+  while (i < tasks.size()) {
+    size_t j = i;
+    while (j < tasks.size() && task_class[j] == task_class[i]) ++j;
+    BlockArgs a;
+    a.pairs = d_pairs;
+    a.prims = P.d_prims;
+    a.shells = P.d_shells;
+    a.tasks = d_tasks + i;
+    a.ntasks = (int64_t)(j - i);
+    a.out = d_out;
+    launch_blocks_class(task_class[i][0], task_class[i][1], task_class[i][2], task_class[i][3], a, s);
+    i = j;
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(s));
+  HIP_OK(hipFree(d_tasks));
+}
+
+static void compute_schwarz(PlanData& P, std::vector<double>& seg_q, hipStream_t s) {
+  const int npair = (int)P.all_pairs.size();
+  // tasks sorted by pair class
+  std::vector<int> order(npair);
+  for (int i = 0; i < npair; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return P.all_class[x] < P.all_class[y]; });
+  std::vector<BlockTask> tasks;
+  std::vector<std::array<int, 4>> tcls;
+  std::vector<int64_t> base(npair);
+  int64_t total = 0;
+  for (int i : order) {
+    const Pair& pr = P.all_pairs[i];
+    const int la = P.seg[pr.sa].l, lb = P.seg[pr.sb].l;
+    const int na = ncart_h(la), nb = ncart_h(lb);
+    BlockTask t;
+    t.bra = t.ket = i;
+    t.base = total;
+    base[i] = total;
+    t.stride[0] = 1;
+    t.stride[1] = na;
+    t.stride[2] = na * nb;
+    t.stride[3] = na * nb * na;
+    for (int k = 0; k < 4; ++k) t.coff[k] = 0;
+    tasks.push_back(t);
+    tcls.push_back({la, lb, la, lb});
+    total += (int64_t)na * nb * na * nb;
+  }
+  double* d_out = nullptr;
+  HIP_OK(hipMalloc(&d_out, sizeof(double) * std::max<int64_t>(total, 1)));
+  run_blocks(P, P.d_all_pairs, tasks, tcls, d_out, s);
+  std::vector<double> h(total);
+  HIP_OK(hipMemcpy(h.data(), d_out, sizeof(double) * total, hipMemcpyDeviceToHost));
+  HIP_OK(hipFree(d_out));
+  seg_q.assign(npair, 0.0);
+  for (int i = 0; i < npair; ++i) {
+    const Pair& pr = P.all_pairs[i];
+    const int la = P.seg[pr.sa].l, lb = P.seg[pr.sb].l;
+    const int na = ncart_h(la), nb = ncart_h(lb);
+    const double* B = h.data() + base[i];
+    auto at = [&](int a, int b, int c, int d) { return B[((d * na + c) * nb + b) * na + a]; };
+    double largest = 0.0;
+    if (P.cart) {
+      for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b) largest = std::max(largest, std::fabs(at(a, b, a, b)));
+    } else {
+      std::vector<std::vector<double>> ta, tb;
+      c2s_rows(la, ta);
+      c2s_rows(lb, tb);
+      for (size_t sa = 0; sa < ta.size(); ++sa)
+        for (size_t sb = 0; sb < tb.size(); ++sb) {
+          double v = 0.0;
+          for (int a1 = 0; a1 < na; ++a1) {
+            if (ta[sa][a1] == 0) continue;
+            for (int a2 = 0; a2 < na; ++a2) {
+              if (ta[sa][a2] == 0) continue;
+              for (int b1 = 0; b1 < nb; ++b1) {
+                if (tb[sb][b1] == 0) continue;
+                for (int b2 = 0; b2 < nb; ++b2) {
+                  if (tb[sb][b2] == 0) continue;
+                  v += ta[sa][a1] * ta[sa][a2] * tb[sb][b1] * tb[sb][b2] * at(a1, b1, a2, b2);
+                }
+              }
+            }
+          }
+          largest = std::max(largest, std::fabs(v));
+        }
+    }
+    seg_q[i] = largest;
+  }
+}
+
+static void build_c2s_maps(PlanData& P, hipStream_t s) {
+  std::vector<int> cidx((size_t)P.n_cart * 2, 0), sidx((size_t)P.n * 3, 0);
+  std::vector<double> ccoef((size_t)P.n_cart * 2, 0.0), scoef((size_t)P.n * 3, 0.0);
+  for (int sh = 0; sh < P.nbas; ++sh) {
+    const int l = P.bas[sh * BAS_SLOTS + ANG_OF], nc = P.bas[sh * BAS_SLOTS + NCTR_OF];
+    const int nf = ncart_h(l);
+    const int ns = P.cart ? nf : 2 * l + 1;
+    std::vector<std::vector<double>> T;
+    if (P.cart) {
+      T.assign(nf, std::vector<double>(nf, 0.0));
+      for (int i = 0; i < nf; ++i) T[i][i] = 1.0;
+    } else {
+      c2s_rows(l, T);
+    }
+    for (int c = 0; c < nc; ++c) {
+      const int cb = P.ao_cart[sh] + c * nf, sbase = P.ao[sh] + c * ns;
+      for (int m = 0; m < ns; ++m) {
+        int k = 0;
+        for (int f = 0; f < nf; ++f)
+          if (T[m][f] != 0.0) {
+            if (k >= 3) throw Internal("c2s map overflow");
+            sidx[(size_t)(sbase + m) * 3 + k] = cb + f;
+            scoef[(size_t)(sbase + m) * 3 + k] = T[m][f];
+            ++k;
+          }
+      }
+      for (int f = 0; f < nf; ++f) {
+        int k = 0;
+        for (int m = 0; m < ns; ++m)
+          if (T[m][f] != 0.0) {
+            if (k >= 2) throw Internal("c2s map overflow");
+            cidx[(size_t)(cb + f) * 2 + k] = sbase + m;
+            ccoef[(size_t)(cb + f) * 2 + k] = T[m][f];
+            ++k;
+          }
+      }
+    }
+  }
+  P.d_c2s_idx = dev_upload(cidx, s);
+  P.d_c2s_coef = dev_upload(ccoef, s);
+  P.d_s2c_idx = dev_upload(sidx, s);
+  P.d_s2c_coef = dev_upload(scoef, s);
+}
+
+static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
+  if (!std::isfinite(cutoff) || cutoff < 0) throw InvalidArgument("Invalid Schwarz cutoff");
+  P.cutoff = cutoff;
+  build_segments(P);
+  const int nseg = (int)P.seg.size();
+  // shells on device
+  std::vector<ShellXYZ> sh(nseg);
+  for (int i = 0; i < nseg; ++i) {
+    sh[i].x = P.seg[i].r[0];
+    sh[i].y = P.seg[i].r[1];
+    sh[i].z = P.seg[i].r[2];
+    sh[i].ao = P.seg[i].ao_cart;
+    sh[i].l = P.seg[i].l;
+  }
+  P.d_shells = dev_upload(sh, s);
+  // all segmented pairs S >= T
+  for (int S = 0; S < nseg; ++S)
+    for (int T = 0; T <= S; ++T) {
+      int a = S, b = T;
+      if (P.seg[b].l > P.seg[a].l) std::swap(a, b);
+      Pair pr;
+      make_pair(P, a, b, pr);
+      P.pair_index[{S, T}] = (int)P.all_pairs.size();
+      P.all_pairs.push_back(pr);
+      P.all_class.push_back(pair_class(P.seg[a].l, P.seg[b].l));
+    }
+  if (P.prims.empty()) P.prims.push_back(PrimPair{});
+  P.d_prims = dev_upload(P.prims, s);
+  P.d_all_pairs = dev_upload(P.all_pairs, s);
+  // Schwarz bounds (parent-shell maxima, as the reference defines them per shell pair)
+  const int npair = (int)P.all_pairs.size();
+  std::vector<double> q(npair, 1.0);
+  if (cutoff > 0) {
+    std::vector<double> seg_q;
+    compute_schwarz(P, seg_q, s);
+    std::map<std::pair<int, int>, double> parent_max;
+    for (int S = 0; S < nseg; ++S)
+      for (int T = 0; T <= S; ++T) {
+        const int i = P.pair_index[{S, T}];
+        const int ps = P.seg[S].parent, pt = P.seg[T].parent;
+        auto key = std::make_pair(std::max(ps, pt), std::min(ps, pt));
+        double& m = parent_max[key];
+        m = std::max(m, seg_q[i]);
+      }
+    for (int S = 0; S < nseg; ++S)
+      for (int T = 0; T <= S; ++T) {
+        const int ps = P.seg[S].parent, pt = P.seg[T].parent;
+        q[P.pair_index[{S, T}]] = std::sqrt(parent_max[{std::max(ps, pt), std::min(ps, pt)}]);
+      }
+    P.parent_q.assign((size_t)P.nbas * P.nbas, 0.0);
+    for (auto& kv : parent_max) P.parent_q[(size_t)kv.first.first * P.nbas + kv.first.second] = std::sqrt(kv.second);
+  }
+  double qmax = 0.0;
+  for (double v : q) qmax = std::max(qmax, v);
+  // retained pairs, grouped by class, Schwarz-descending inside a class
+  std::vector<int> keep;
+  for (int i = 0; i < npair; ++i) {
+    if (P.all_pairs[i].np == 0) continue;
+    if (cutoff > 0 && q[i] * qmax < cutoff) continue;
+    keep.push_back(i);
+  }
+  std::stable_sort(keep.begin(), keep.end(), [&](int x, int y) {
+    if (P.all_class[x] != P.all_class[y]) return P.all_class[x] < P.all_class[y];
+    return q[x] > q[y];
+  });
+  for (int c = 0; c <= kNumPairClasses; ++c) P.class_begin[c] = 0;
+  for (int i : keep) {
+    P.pairs.push_back(P.all_pairs[i]);
+    P.pair_q.push_back(q[i]);
+  }
+  {
+    int c = 0, k = 0;
+    for (c = 0; c < kNumPairClasses; ++c) {
+      P.class_begin[c] = k;
+      while (k < (int)keep.size() && P.all_class[keep[k]] == c) ++k;
+    }
+    P.class_begin[kNumPairClasses] = k;
+  }
+  P.d_pairs = dev_upload(P.pairs, s);
+  // quartet classes
+  static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+  P.nquartets = 0;
+  for (int cb = 0; cb < kNumPairClasses; ++cb)
+    for (int ck = 0; ck <= cb; ++ck) {
+      const int b0 = P.class_begin[cb], nbra = P.class_begin[cb + 1] - b0;
+      const int k0 = P.class_begin[ck], nket = P.class_begin[ck + 1] - k0;
+      if (nbra == 0 || nket == 0) continue;
+      std::vector<int64_t> qoff(nbra + 1, 0);
+      for (int b = 0; b < nbra; ++b) {
+        int cnt = nket;
+        if (cutoff > 0) {
+          // kets are Schwarz-descending: count those with Q_b Q_k >= cutoff
+          const double qb = P.pair_q[b0 + b];
+          int lo = 0, hi = nket;
+          while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            if (qb * P.pair_q[k0 + mid] < cutoff) hi = mid; else lo = mid + 1;
+          }
+          cnt = lo;
+        }
+        if (cb == ck) cnt = std::min(cnt, b + 1);
+        qoff[b + 1] = qoff[b] + cnt;
+      }
+      if (qoff[nbra] == 0) continue;
+      QuartetClass qc;
+      qc.la = pl[cb][0];
+      qc.lb = pl[cb][1];
+      qc.lc = pl[ck][0];
+      qc.ld = pl[ck][1];
+      qc.bra0 = b0;
+      qc.ket0 = k0;
+      qc.nbra = nbra;
+      qc.same_class = cb == ck;
+      qc.nq = qoff[nbra];
+      qc.d_qoff = dev_upload(qoff, s);
+      P.nquartets += qc.nq;
+      P.classes.push_back(qc);
+    }
+  build_c2s_maps(P, s);
+  {
+    std::vector<int> order(npair);
+    for (int i = 0; i < npair; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return P.all_class[x] < P.all_class[y]; });
+    std::vector<Pair> p1e;
+    int k = 0;
+    for (int c = 0; c < kNumPairClasses; ++c) {
+      P.p1e_begin[c] = k;
+      while (k < npair && P.all_class[order[k]] == c) p1e.push_back(P.all_pairs[order[k++]]);
+    }
+    P.p1e_begin[kNumPairClasses] = k;
+    P.d_pairs1e = dev_upload(p1e, s);
+    std::vector<double> nuc;
+    for (int a = 0; a < P.natm; ++a) {
+      const int z = P.atm[a * ATM_SLOTS + CHARGE_OF];
+      if (z == 0) continue;
+      const double* r = P.env.data() + P.atm[a * ATM_SLOTS + PTR_COORD];
+      nuc.insert(nuc.end(), {r[0], r[1], r[2], (double)z});
+    }
+    P.nnuc = (int)(nuc.size() / 4);
+    if (nuc.empty()) nuc.assign(4, 0.0);
+    P.d_nuclei = dev_upload(nuc, s);
+  }
+  HIP_OK(hipMalloc(&P.d_flag, sizeof(int)));
+  HIP_OK(hipHostMalloc(&P.h_flag, sizeof(int)));
+  HIP_OK(hipStreamSynchronize(s));
+}
+
+static void free_plan(PlanData& P) {
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(P.d_shells); f(P.d_prims); f(P.d_pairs); f(P.d_all_pairs);
+  f(P.d_c2s_idx); f(P.d_c2s_coef); f(P.d_s2c_idx); f(P.d_s2c_coef);
+  f(P.d_dc); f(P.d_jp); f(P.d_kp); f(P.d_flag);
+  f(P.d_pairs1e); f(P.d_nuclei);
+  for (auto& c : P.classes) f(c.d_qoff);
+  for (hipEvent_t e : P.ev_pool) (void)hipEventDestroy(e);
+  if (P.h_flag) (void)hipHostFree(P.h_flag);
+}
+
+static void ensure_workspace(PlanData& P, size_t nbd) {
+  if (P.ws_nbd >= nbd) return;
+  auto f = [](double*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  };
+  f(P.d_dc); f(P.d_jp); f(P.d_kp);
+  const size_t nn = (size_t)P.n_cart * P.n_cart;
+  HIP_OK(hipMalloc(&P.d_dc, sizeof(double) * nn * nbd));
+  HIP_OK(hipMalloc(&P.d_jp, sizeof(double) * nn * nbd));
+  HIP_OK(hipMalloc(&P.d_kp, sizeof(double) * nn * nbd));
+  P.ws_nbd = nbd;
+}
+
+static void jk_build(PlanData& P, const double* density, int nb, double* out, int with_j, int with_k,
+                     uint32_t flags, hipStream_t s) {
+  if ((with_j != 0 && with_j != 1) || (with_k != 0 && with_k != 1) || !(with_j || with_k))
+    throw InvalidArgument("Direct J/K requires at least one selected operator");
+  if (nb < 1) throw InvalidArgument("Direct J/K shape mismatch");
+  if (!density || !out) throw InvalidArgument("Direct J/K requires device density and output buffers");
+  const int n = P.n, nc = P.n_cart;
+  const size_t nn = (size_t)n * n, ncc = (size_t)nc * nc;
+  if (!(flags & 4u)) {
+    HIP_OK(hipMemsetAsync(P.d_flag, 0, sizeof(int), s));
+    hipLaunchKernelGGL(finite_kernel, dim3(nblocks(nn * nb)), dim3(256), 0, s, density, nn * nb, P.d_flag);
+    HIP_OK(hipMemcpyAsync(P.h_flag, P.d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (*P.h_flag) throw InvalidArgument("Nonfinite density");
+  }
+  const bool sym = (flags & GSCF_FLAG_SYMMETRIC_DENSITY) != 0;
+  const int nbd = (with_k && !sym) ? 2 * nb : nb;
+  ensure_workspace(P, nbd);
+  hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, density, n, nb, P.d_dc, nc,
+                     P.d_c2s_idx, P.d_c2s_coef, nbd);
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_jp, ncc * nbd, 0.0);
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_kp, ncc * nbd, 0.0);
+  for (size_t ci = 0; ci < P.classes.size(); ++ci) {
+    const QuartetClass& qc = P.classes[ci];
+    if (P.prof) {
+      if (P.ev_used + 2 > P.ev_pool.size()) {
+        for (int e = 0; e < 64; ++e) {
+          hipEvent_t ev;
+          HIP_OK(hipEventCreate(&ev));
+          P.ev_pool.push_back(ev);
+        }
+      }
+      P.prof_recs.push_back({(int)ci, P.ev_used});
+      HIP_OK(hipEventRecord(P.ev_pool[P.ev_used], s));
+    }
+    JKArgs a;
+    a.pairs = P.d_pairs;
+    a.prims = P.d_prims;
+    a.shells = P.d_shells;
+    a.qoff = qc.d_qoff;
+    a.bra0 = qc.bra0;
+    a.ket0 = qc.ket0;
+    a.nbra = qc.nbra;
+    a.same_class = qc.same_class;
+    a.nq = qc.nq;
+    a.dens = P.d_dc;
+    a.jp = P.d_jp;
+    a.kp = P.d_kp;
+    a.n = nc;
+    a.nbd = with_k ? nbd : nb;
+    a.nbj = nb;
+    a.with_j = with_j;
+    a.with_k = with_k;
+    launch_jk_class(qc.la, qc.lb, qc.lc, qc.ld, a, s);
+    if (P.prof) {
+      HIP_OK(hipEventRecord(P.ev_pool[P.ev_used + 1], s));
+      P.ev_used += 2;
+    }
+  }
+  hipLaunchKernelGGL(finalize_kernel, dim3(nblocks((size_t)(with_j + with_k) * nb * nn)), dim3(256), 0, s,
+                     P.d_jp, P.d_kp, nc, n, nb, sym ? 1 : 0, with_j, with_k, P.d_s2c_idx, P.d_s2c_coef, out);
+  HIP_OK(hipGetLastError());
+}
+
+static void int1e(PlanData& P, double* s_out, double* t_out, double* v_out, hipStream_t st) {
+  if (!s_out || !t_out || !v_out) throw InvalidArgument("int1e requires three device output buffers");
+  const size_t ncc = (size_t)P.n_cart * P.n_cart;
+  double* tmp = nullptr;
+  HIP_OK(hipMalloc(&tmp, sizeof(double) * 3 * ncc));
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(3 * ncc)), dim3(256), 0, st, tmp, 3 * ncc, 0.0);
+  static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+  for (int c = 0; c < kNumPairClasses; ++c) {
+    const int b = P.p1e_begin[c], e = P.p1e_begin[c + 1];
+    launch_int1e_class(pl[c][0], pl[c][1], P.d_pairs1e + b, e - b, P.d_prims, P.d_shells, P.d_nuclei, P.nnuc,
+                       P.n_cart, tmp, tmp + ncc, tmp + 2 * ncc, st);
+  }
+  double* outs[3] = {s_out, t_out, v_out};
+  for (int k = 0; k < 3; ++k)
+    hipLaunchKernelGGL(sandwich_kernel, dim3(nblocks((size_t)P.n * P.n)), dim3(256), 0, st, tmp + k * ncc,
+                       P.n_cart, P.n, P.d_s2c_idx, P.d_s2c_coef, outs[k]);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(st));
+  HIP_OK(hipFree(tmp));
+}
+
+static void eri_blocks(PlanData& P, const int32_t* shls, int64_t nq, const int64_t* offsets, double* out,
+                       hipStream_t s) {
+  if (nq < 0 || (nq > 0 && (!shls || !offsets || !out))) throw InvalidArgument("Invalid ERI block request");
+  struct Item {
+    std::array<int, 4> cls;
+    BlockTask t;
+  };
+  std::vector<Item> items;
+  std::vector<int> seg_first(P.nbas + 1, 0);
+  for (size_t i = 0; i < P.seg.size(); ++i) seg_first[P.seg[i].parent + 1] = (int)i + 1;
+  for (int i = 0; i < P.nbas; ++i) seg_first[i + 1] = std::max(seg_first[i + 1], seg_first[i]);
+  std::vector<int64_t> offs(offsets, offsets + nq);
+  for (int64_t q = 0; q < nq; ++q) {
+    int sh[4], dims[4];
+    for (int k = 0; k < 4; ++k) {
+      sh[k] = shls[q * 4 + k];
+      if (sh[k] < 0 || sh[k] >= P.nbas) throw InvalidArgument("ERI block shell index out of range");
+      const int l = P.bas[sh[k] * BAS_SLOTS + ANG_OF], nc = P.bas[sh[k] * BAS_SLOTS + NCTR_OF];
+      dims[k] = nc * ncart_h(l);
+    }
+    const int stride_req[4] = {1, dims[0], dims[0] * dims[1], dims[0] * dims[1] * dims[2]};
+    const int nc0 = P.bas[sh[0] * BAS_SLOTS + NCTR_OF], nc1 = P.bas[sh[1] * BAS_SLOTS + NCTR_OF];
+    const int nc2 = P.bas[sh[2] * BAS_SLOTS + NCTR_OF], nc3 = P.bas[sh[3] * BAS_SLOTS + NCTR_OF];
+    for (int c0 = 0; c0 < nc0; ++c0)
+      for (int c1 = 0; c1 < nc1; ++c1)
+        for (int c2 = 0; c2 < nc2; ++c2)
+          for (int c3 = 0; c3 < nc3; ++c3) {
+            const int sg[4] = {seg_first[sh[0]] + c0, seg_first[sh[1]] + c1, seg_first[sh[2]] + c2,
+                               seg_first[sh[3]] + c3};
+            const int cf[4] = {c0, c1, c2, c3};
+            // positions of each requested slot's segment inside its pair
+            auto pair_of = [&](int x, int y, int& idx, int& first_pos, int& second_pos) {
+              const int S = std::max(sg[x], sg[y]), T = std::min(sg[x], sg[y]);
+              idx = P.pair_index.at({S, T});
+              const Pair& pr = P.all_pairs[idx];
+              if (pr.sa == sg[x] && (pr.sb == sg[y])) {
+                first_pos = x;
+                second_pos = y;
+              } else {
+                first_pos = y;
+                second_pos = x;
+              }
+            };
+            int i01, a01, b01, i23, a23, b23;
+            pair_of(0, 1, i01, a01, b01);
+            pair_of(2, 3, i23, a23, b23);
+            int bra = i01, ket = i23, pos[4] = {a01, b01, a23, b23};
+            if (P.all_class[i23] > P.all_class[i01]) {
+              bra = i23;
+              ket = i01;
+              pos[0] = a23; pos[1] = b23; pos[2] = a01; pos[3] = b01;
+            }
+            Item it;
+            it.t.bra = bra;
+            it.t.ket = ket;
+            it.t.base = offs[q];
+            for (int k = 0; k < 4; ++k) {
+              const int l = P.seg[sg[pos[k]]].l;
+              it.t.stride[k] = stride_req[pos[k]];
+              it.t.coff[k] = cf[pos[k]] * ncart_h(l);
+            }
+            const Pair& bp = P.all_pairs[bra];
+            const Pair& kp = P.all_pairs[ket];
+            it.cls = {P.seg[bp.sa].l, P.seg[bp.sb].l, P.seg[kp.sa].l, P.seg[kp.sb].l};
+            items.push_back(it);
+          }
+  }
+  std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cls < y.cls; });
+  std::vector<BlockTask> tasks;
+  std::vector<std::array<int, 4>> cls;
+  for (auto& it : items) {
+    tasks.push_back(it.t);
+    cls.push_back(it.cls);
+  }
+  run_blocks(P, P.d_all_pairs, tasks, cls, out, s);
+}
+
+}  // namespace gscf
+
+// ----------------------------------------------------------------------------
+// C ABI
+// ----------------------------------------------------------------------------
+using namespace gscf;
+
+extern "C" {
+
+const char* gscf_last_error(void) { return g_error.c_str(); }
+
+int gscf_plan_create(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                     int32_t nenv, int32_t cart, double cutoff, void* stream, gscf_plan** plan) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("plan output pointer is NULL");
+    *plan = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    gscf_plan* p = new gscf_plan();
+    try {
+      validate_tables(p->d, atm, natm, bas, nbas, env, nenv, cart);
+      build_plan(p->d, cutoff, (hipStream_t)stream);
+    } catch (...) {
+      free_plan(p->d);
+      delete p;
+      throw;
+    }
+    p->d.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *plan = p;
+  });
+}
+
+int gscf_plan_destroy(gscf_plan* plan) {
+  return guarded([&] {
+    if (!plan) return;
+    free_plan(plan->d);
+    delete plan;
+  });
+}
+
+int gscf_plan_get_info(const gscf_plan* plan, gscf_plan_info* info) {
+  return guarded([&] {
+    if (!plan || !info) throw InvalidArgument("NULL plan or info");
+    const PlanData& P = plan->d;
+    info->n = P.n;
+    info->n_cart = P.n_cart;
+    info->nbas = P.nbas;
+    info->nseg = (int32_t)P.seg.size();
+    info->npairs = (int64_t)P.pairs.size();
+    info->nquartets = P.nquartets;
+    info->cutoff = P.cutoff;
+    info->plan_ms = P.plan_ms;
+  });
+}
+
+int gscf_jk_build(gscf_plan* plan, const double* density, int32_t nb, double* out, int32_t with_j, int32_t with_k,
+                  uint32_t flags, double mixed_threshold, void* stream) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("NULL plan");
+    (void)mixed_threshold;
+    jk_build(plan->d, density, nb, out, with_j, with_k, flags, (hipStream_t)stream);
+  });
+}
+
+int gscf_direct_jk(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                   int32_t nenv, const double* density, int32_t nb, int32_t n, double* out, int32_t cart,
+                   double cutoff, int32_t workers, int32_t with_j, int32_t with_k, void* stream) {
+  return guarded([&] {
+    if (workers < 1) throw InvalidArgument("Direct J/K workers must be positive");
+    if ((with_j != 0 && with_j != 1) || (with_k != 0 && with_k != 1) || !(with_j || with_k))
+      throw InvalidArgument("Direct J/K requires at least one selected operator");
+    gscf_plan p;
+    try {
+      validate_tables(p.d, atm, natm, bas, nbas, env, nenv, cart);
+      if (n != p.d.n || nb < 1) throw InvalidArgument("Direct J/K shape mismatch");
+      build_plan(p.d, cutoff, (hipStream_t)stream);
+      jk_build(p.d, density, nb, out, with_j, with_k, 0u, (hipStream_t)stream);
+      HIP_OK(hipStreamSynchronize((hipStream_t)stream));
+    } catch (...) {
+      free_plan(p.d);
+      throw;
+    }
+    free_plan(p.d);
+  });
+}
+
+int gscf_plan_get_schwarz(const gscf_plan* plan, double* q) {
+  return guarded([&] {
+    if (!plan || !q) throw InvalidArgument("NULL plan or output");
+    const PlanData& P = plan->d;
+    if (P.parent_q.empty()) throw InvalidArgument("Schwarz bounds are computed only for a positive cutoff");
+    std::copy(P.parent_q.begin(), P.parent_q.end(), q);
+  });
+}
+
+int gscf_plan_profile(gscf_plan* plan, int32_t enable) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("NULL plan");
+    PlanData& P = plan->d;
+    P.prof = enable != 0;
+    P.prof_recs.clear();
+    P.ev_used = 0;
+  });
+}
+
+int gscf_plan_class_times(gscf_plan* plan, int32_t max_classes, int32_t* classes, double* ms,
+                          int64_t* launches, int32_t* count) {
+  return guarded([&] {
+    if (!plan || !count) throw InvalidArgument("NULL plan or output");
+    PlanData& P = plan->d;
+    const int nc = (int)P.classes.size();
+    *count = nc;
+    if (max_classes < nc || !classes || !ms || !launches) throw InvalidArgument("class arrays too small");
+    for (int i = 0; i < nc; ++i) {
+      classes[4 * i + 0] = P.classes[i].la;
+      classes[4 * i + 1] = P.classes[i].lb;
+      classes[4 * i + 2] = P.classes[i].lc;
+      classes[4 * i + 3] = P.classes[i].ld;
+      ms[i] = 0.0;
+      launches[i] = 0;
+    }
+    for (const auto& r : P.prof_recs) {
+      HIP_OK(hipEventSynchronize(P.ev_pool[r.second + 1]));
+      float t = 0.f;
+      HIP_OK(hipEventElapsedTime(&t, P.ev_pool[r.second], P.ev_pool[r.second + 1]));
+      ms[r.first] += t;
+      launches[r.first] += 1;
+    }
+  });
+}
+
+int gscf_eri_blocks(gscf_plan* plan, const int32_t* shls, int64_t nq, const int64_t* offsets, double* out,
+                    void* stream) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("NULL plan");
+    eri_blocks(plan->d, shls, nq, offsets, out, (hipStream_t)stream);
+  });
+}
+
+int gscf_int1e(gscf_plan* plan, double* s, double* t, double* v, void* stream) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("NULL plan");
+    int1e(plan->d, s, t, v, (hipStream_t)stream);
+  });
+}
+
+int gscf_jk_geometry_vjp(gscf_plan* plan, const double* density, const double* cot_j, const double* cot_k,
+                         double* grad, void* stream) {
+  return guarded([&] {
+    (void)plan; (void)density; (void)cot_j; (void)cot_k; (void)grad; (void)stream;
+    throw Internal("gscf_jk_geometry_vjp: not built yet");
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,510 @@
+// Rys-quadrature shell-quartet kernels with fused J/K digestion (FP64), as templates;
+// instantiated one class per translation unit by jk_class.hip.
+//
+// One template instantiation per canonical angular-momentum class
+// (LA>=LB, LC>=LD, pair class AB >= CD), 21 classes for s/p/d.  Two layouts:
+//   * full:  one lane = one shell quartet; all Cartesian components of the
+//            block live in registers (small classes);
+//   * split: one lane = (shell quartet, one Cartesian ket component pair);
+//            the ket transfer is applied as a weight vector on the vertical
+//            table, the bra transfer and the block slice stay in registers
+//            (large classes).
+// Digestion follows the 8-fold permutational sum of csrc/direct_jk.cc:45-76,
+// folded into partial matrices: Jp[a,b] += f v (D_cd + D_dc),
+// Jp[c,d] += f v (D_ab + D_ba), Kp[a,c] += f v D_bd, Kp[a,d] += f v D_bc,
+// Kp[b,c] += f v D_ad, Kp[b,d] += f v D_ac, with f = 1/2 per coincident
+// shell / pair; J = Jp + Jp^T and K = Kp(D) + Kp(D^T)^T are formed afterwards.
+#pragma once
+#include "gscf_device.h"
+#include "gscf_args.h"
+
+namespace gscf {
+
+__device__ __forceinline__ int find_bra(const int64_t* __restrict__ qoff, int nbra, int64_t g) {
+  int lo = 0, hi = nbra;  // qoff[lo] <= g < qoff[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (qoff[mid] <= g) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int LA, int LB, int LC, int LD>
+struct Class {
+  static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+  static constexpr int NM = LA + LB, MM = LC + LD;
+  static constexpr int NR = (LA + LB + LC + LD) / 2 + 1;
+  static constexpr int NBRA = NA * NB, NKET = NC * ND;
+  static constexpr bool kSplit = NBRA * NKET > 36;
+  static constexpr int SPLIT = kSplit ? NKET : 1;
+};
+
+struct QuartetGeom {
+  double A[3], B[3], C[3], D[3];
+};
+
+// Per primitive quartet: roots and the per-root recurrence coefficients.
+template <int NR>
+struct PrimQ {
+  double s[NR], w[NR];
+  double pref, ip, iq, ipq, qf, pf;
+  double PA[3], QC[3], PQ[3];
+};
+
+template <int NR>
+__device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q, const QuartetGeom& g,
+                                           PrimQ<NR>& o) {
+  const double pq = P.p + Q.p;
+  o.ipq = 1.0 / pq;
+  o.ip = 1.0 / P.p;
+  o.iq = 1.0 / Q.p;
+  o.PQ[0] = P.px - Q.px;
+  o.PQ[1] = P.py - Q.py;
+  o.PQ[2] = P.pz - Q.pz;
+  o.PA[0] = P.px - g.A[0];
+  o.PA[1] = P.py - g.A[1];
+  o.PA[2] = P.pz - g.A[2];
+  o.QC[0] = Q.px - g.C[0];
+  o.QC[1] = Q.py - g.C[1];
+  o.QC[2] = Q.pz - g.C[2];
+  o.qf = Q.p * o.ipq;  // rho / p
+  o.pf = P.p * o.ipq;  // rho / q
+  const double rho = P.p * Q.p * o.ipq;
+  const double T = rho * (o.PQ[0] * o.PQ[0] + o.PQ[1] * o.PQ[1] + o.PQ[2] * o.PQ[2]);
+  o.pref = P.e * Q.e * sqrt(o.ipq);
+  rys_roots<NR>(T, o.s, o.w);
+}
+
+// Full block of one primitive quartet accumulated into acc[NA][NB][NC][ND].
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void accumulate_full(const PrimQ<Class<LA, LB, LC, LD>::NR>& q, const double AB[3],
+                                                const double CD[3],
+                                                double (&acc)[ncart(LA)][ncart(LB)][ncart(LC)][ncart(LD)]) {
+  using K = Class<LA, LB, LC, LD>;
+#pragma unroll
+  for (int r = 0; r < K::NR; ++r) {
+    const double u = q.s[r];
+    const double b00 = 0.5 * u * q.ipq;
+    const double b10 = 0.5 * q.ip * (1.0 - q.qf * u);
+    const double b01 = 0.5 * q.iq * (1.0 - q.pf * u);
+    double H[3][LA + 1][LB + 1][LC + 1][LD + 1];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double G[K::NM + 1][K::MM + 1];
+      const double c00 = q.PA[x] - q.qf * q.PQ[x] * u;
+      const double c0p = q.QC[x] + q.pf * q.PQ[x] * u;
+      vrr<K::NM, K::MM>(c00, c0p, b10, b01, b00, x == 2 ? q.w[r] * q.pref : 1.0, G);
+      hrr_full<LA, LB, LC, LD>(G, AB[x], CD[x], H[x]);
+    }
+#pragma unroll
+    for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+          for (int id = 0; id < K::ND; ++id) {
+            const double vx = H[0][cart_comp(LA, ia, 0)][cart_comp(LB, ib, 0)][cart_comp(LC, ic, 0)][cart_comp(LD, id, 0)];
+            const double vy = H[1][cart_comp(LA, ia, 1)][cart_comp(LB, ib, 1)][cart_comp(LC, ic, 1)][cart_comp(LD, id, 1)];
+            const double vz = H[2][cart_comp(LA, ia, 2)][cart_comp(LB, ib, 2)][cart_comp(LC, ic, 2)][cart_comp(LD, id, 2)];
+            acc[ia][ib][ic][id] += vx * vy * vz;
+          }
+  }
+}
+
+// Runtime Cartesian exponents of component i of shell l.
+template <int L>
+__device__ __forceinline__ void cart_xyz(int i, int (&e)[3]) {
+  e[0] = e[1] = e[2] = 0;
+#pragma unroll
+  for (int k = 0; k < ncart(L); ++k)
+    if (k == i) {
+      e[0] = cart_comp(L, k, 0);
+      e[1] = cart_comp(L, k, 1);
+      e[2] = cart_comp(L, k, 2);
+    }
+}
+
+// Ket transfer weights for a fixed ket component pair (c, d) on one axis:
+// I(n; c, d) = sum_m wk[m] G(n, m), wk[m] = C(d, m-c) CD^{d-(m-c)}.
+template <int MM>
+__device__ __forceinline__ void ket_weights(int c, int d, double cd, double (&wk)[MM + 1]) {
+#pragma unroll
+  for (int m = 0; m <= MM; ++m) {
+    const int k = m - c;
+    double v = 0.0;
+    if (k >= 0 && k <= d) {
+      double pw = 1.0;
+      for (int t = 0; t < d - k; ++t) pw *= cd;
+      v = (double)binom(d, k) * pw;
+    }
+    wk[m] = v;
+  }
+}
+
+// Block slice for one ket component pair accumulated into acc[NA][NB].
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void accumulate_split(const PrimQ<Class<LA, LB, LC, LD>::NR>& q, const double AB[3],
+                                                 const double (&wk)[3][LC + LD + 1],
+                                                 double (&acc)[ncart(LA)][ncart(LB)]) {
+  using K = Class<LA, LB, LC, LD>;
+#pragma unroll
+  for (int r = 0; r < K::NR; ++r) {
+    const double u = q.s[r];
+    const double b00 = 0.5 * u * q.ipq;
+    const double b10 = 0.5 * q.ip * (1.0 - q.qf * u);
+    const double b01 = 0.5 * q.iq * (1.0 - q.pf * u);
+    double h[3][LA + 1][LB + 1];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      double G[K::NM + 1][K::MM + 1];
+      const double c00 = q.PA[x] - q.qf * q.PQ[x] * u;
+      const double c0p = q.QC[x] + q.pf * q.PQ[x] * u;
+      vrr<K::NM, K::MM>(c00, c0p, b10, b01, b00, x == 2 ? q.w[r] * q.pref : 1.0, G);
+      double g1[K::NM + 1];
+#pragma unroll
+      for (int n = 0; n <= K::NM; ++n) {
+        double s = 0.0;
+#pragma unroll
+        for (int m = 0; m <= K::MM; ++m) s += wk[x][m] * G[n][m];
+        g1[n] = s;
+      }
+      hrr_bra1<LA, LB>(g1, AB[x], h[x]);
+    }
+#pragma unroll
+    for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < K::NB; ++ib)
+        acc[ia][ib] += h[0][cart_comp(LA, ia, 0)][cart_comp(LB, ib, 0)] *
+                       h[1][cart_comp(LA, ia, 1)][cart_comp(LB, ib, 1)] *
+                       h[2][cart_comp(LA, ia, 2)][cart_comp(LB, ib, 2)];
+  }
+}
+
+__device__ __forceinline__ void load_geom(const ShellXYZ& a, const ShellXYZ& b, const ShellXYZ& c,
+                                          const ShellXYZ& d, QuartetGeom& g, double AB[3], double CD[3]) {
+  g.A[0] = a.x; g.A[1] = a.y; g.A[2] = a.z;
+  g.B[0] = b.x; g.B[1] = b.y; g.B[2] = b.z;
+  g.C[0] = c.x; g.C[1] = c.y; g.C[2] = c.z;
+  g.D[0] = d.x; g.D[1] = d.y; g.D[2] = d.z;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    AB[x] = g.A[x] - g.B[x];
+    CD[x] = g.C[x] - g.D[x];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// J/K kernels
+// ----------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
+  using K = Class<LA, LB, LC, LD>;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.nq) return;
+  const int b = find_bra(a.qoff, a.nbra, g);
+  const int k = (int)(g - a.qoff[b]);
+  const Pair bp = a.pairs[a.bra0 + b];
+  const Pair kp = a.pairs[a.ket0 + k];
+  const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
+  QuartetGeom geo;
+  double AB[3], CD[3];
+  load_geom(sA, sB, sC, sD, geo, AB, CD);
+  double acc[K::NA][K::NB][K::NC][K::ND];
+#pragma unroll
+  for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+      for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+        for (int id = 0; id < K::ND; ++id) acc[ia][ib][ic][id] = 0.0;
+  for (int ip = 0; ip < bp.np; ++ip) {
+    const PrimPair P = a.prims[bp.poff + ip];
+    for (int kq = 0; kq < kp.np; ++kq) {
+      const PrimPair Q = a.prims[kp.poff + kq];
+      PrimQ<K::NR> q;
+      prim_setup<K::NR>(P, Q, geo, q);
+      accumulate_full<LA, LB, LC, LD>(q, AB, CD, acc);
+    }
+  }
+  double f = 1.0;
+  if (bp.sa == bp.sb) f *= 0.5;
+  if (kp.sa == kp.sb) f *= 0.5;
+  if (a.same_class && b == k) f *= 0.5;
+  const int n = a.n;
+  const int oa = sA.ao, ob = sB.ao, oc = sC.ao, od = sD.ao;
+  for (int bt = 0; bt < a.nbd; ++bt) {
+    const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+    if (a.with_j && bt < a.nbj) {
+      double* J = a.jp + (size_t)bt * n * n;
+      double dcd[K::NC][K::ND], dab[K::NA][K::NB];
+#pragma unroll
+      for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+        for (int id = 0; id < K::ND; ++id)
+          dcd[ic][id] = Dm[(size_t)(oc + ic) * n + od + id] + Dm[(size_t)(od + id) * n + oc + ic];
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib)
+          dab[ia][ib] = Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia];
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib) {
+          double s = 0.0;
+#pragma unroll
+          for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id) s += acc[ia][ib][ic][id] * dcd[ic][id];
+          atomicAdd(J + (size_t)(oa + ia) * n + ob + ib, f * s);
+        }
+#pragma unroll
+      for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+        for (int id = 0; id < K::ND; ++id) {
+          double s = 0.0;
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < K::NB; ++ib) s += acc[ia][ib][ic][id] * dab[ia][ib];
+          atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * s);
+        }
+    }
+    if (a.with_k) {
+      double* Kp = a.kp + (size_t)bt * n * n;
+      double dbd[K::NB][K::ND], dbc[K::NB][K::NC], dad[K::NA][K::ND], dac[K::NA][K::NC];
+#pragma unroll
+      for (int ib = 0; ib < K::NB; ++ib) {
+#pragma unroll
+        for (int id = 0; id < K::ND; ++id) dbd[ib][id] = Dm[(size_t)(ob + ib) * n + od + id];
+#pragma unroll
+        for (int ic = 0; ic < K::NC; ++ic) dbc[ib][ic] = Dm[(size_t)(ob + ib) * n + oc + ic];
+      }
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia) {
+#pragma unroll
+        for (int id = 0; id < K::ND; ++id) dad[ia][id] = Dm[(size_t)(oa + ia) * n + od + id];
+#pragma unroll
+        for (int ic = 0; ic < K::NC; ++ic) dac[ia][ic] = Dm[(size_t)(oa + ia) * n + oc + ic];
+      }
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+        for (int ic = 0; ic < K::NC; ++ic) {
+          double s = 0.0;
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id) s += acc[ia][ib][ic][id] * dbd[ib][id];
+          atomicAdd(Kp + (size_t)(oa + ia) * n + oc + ic, f * s);
+        }
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+        for (int id = 0; id < K::ND; ++id) {
+          double s = 0.0;
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+            for (int ic = 0; ic < K::NC; ++ic) s += acc[ia][ib][ic][id] * dbc[ib][ic];
+          atomicAdd(Kp + (size_t)(oa + ia) * n + od + id, f * s);
+        }
+#pragma unroll
+      for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < K::NC; ++ic) {
+          double s = 0.0;
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id) s += acc[ia][ib][ic][id] * dad[ia][id];
+          atomicAdd(Kp + (size_t)(ob + ib) * n + oc + ic, f * s);
+        }
+#pragma unroll
+      for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+        for (int id = 0; id < K::ND; ++id) {
+          double s = 0.0;
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+            for (int ic = 0; ic < K::NC; ++ic) s += acc[ia][ib][ic][id] * dac[ia][ic];
+          atomicAdd(Kp + (size_t)(ob + ib) * n + od + id, f * s);
+        }
+    }
+  }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
+  using K = Class<LA, LB, LC, LD>;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t g = t / K::SPLIT;
+  if (g >= a.nq) return;
+  const int kc = (int)(t - g * K::SPLIT);
+  const int ic = kc / K::ND, id = kc - (kc / K::ND) * K::ND;
+  const int b = find_bra(a.qoff, a.nbra, g);
+  const int k = (int)(g - a.qoff[b]);
+  const Pair bp = a.pairs[a.bra0 + b];
+  const Pair kp = a.pairs[a.ket0 + k];
+  const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
+  QuartetGeom geo;
+  double AB[3], CD[3];
+  load_geom(sA, sB, sC, sD, geo, AB, CD);
+  int ec[3], ed[3];
+  cart_xyz<LC>(ic, ec);
+  cart_xyz<LD>(id, ed);
+  double wk[3][K::MM + 1];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) ket_weights<K::MM>(ec[x], ed[x], CD[x], wk[x]);
+  double acc[K::NA][K::NB];
+#pragma unroll
+  for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < K::NB; ++ib) acc[ia][ib] = 0.0;
+  for (int ip = 0; ip < bp.np; ++ip) {
+    const PrimPair P = a.prims[bp.poff + ip];
+    for (int kq = 0; kq < kp.np; ++kq) {
+      const PrimPair Q = a.prims[kp.poff + kq];
+      PrimQ<K::NR> q;
+      prim_setup<K::NR>(P, Q, geo, q);
+      accumulate_split<LA, LB, LC, LD>(q, AB, wk, acc);
+    }
+  }
+  double f = 1.0;
+  if (bp.sa == bp.sb) f *= 0.5;
+  if (kp.sa == kp.sb) f *= 0.5;
+  if (a.same_class && b == k) f *= 0.5;
+  const int n = a.n;
+  const int oa = sA.ao, ob = sB.ao;
+  const int c = sC.ao + ic, d = sD.ao + id;
+  for (int bt = 0; bt < a.nbd; ++bt) {
+    const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+    if (a.with_j && bt < a.nbj) {
+      double* J = a.jp + (size_t)bt * n * n;
+      const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
+      double s2 = 0.0;
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib) {
+          atomicAdd(J + (size_t)(oa + ia) * n + ob + ib, f * acc[ia][ib] * dcd);
+          s2 += acc[ia][ib] * (Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia]);
+        }
+      atomicAdd(J + (size_t)c * n + d, f * s2);
+    }
+    if (a.with_k) {
+      double* Kp = a.kp + (size_t)bt * n * n;
+      double db_d[K::NB], db_c[K::NB];
+#pragma unroll
+      for (int ib = 0; ib < K::NB; ++ib) {
+        db_d[ib] = Dm[(size_t)(ob + ib) * n + d];
+        db_c[ib] = Dm[(size_t)(ob + ib) * n + c];
+      }
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia) {
+        double sc = 0.0, sd = 0.0;
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib) {
+          sc += acc[ia][ib] * db_d[ib];
+          sd += acc[ia][ib] * db_c[ib];
+        }
+        atomicAdd(Kp + (size_t)(oa + ia) * n + c, f * sc);
+        atomicAdd(Kp + (size_t)(oa + ia) * n + d, f * sd);
+      }
+      double da_d[K::NA], da_c[K::NA];
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia) {
+        da_d[ia] = Dm[(size_t)(oa + ia) * n + d];
+        da_c[ia] = Dm[(size_t)(oa + ia) * n + c];
+      }
+#pragma unroll
+      for (int ib = 0; ib < K::NB; ++ib) {
+        double sc = 0.0, sd = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < K::NA; ++ia) {
+          sc += acc[ia][ib] * da_d[ia];
+          sd += acc[ia][ib] * da_c[ia];
+        }
+        atomicAdd(Kp + (size_t)(ob + ib) * n + c, f * sc);
+        atomicAdd(Kp + (size_t)(ob + ib) * n + d, f * sd);
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Block kernel (explicit quartet list; one lane per (task, ket component pair))
+// ----------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void blocks_kernel(BlockArgs a) {
+  using K = Class<LA, LB, LC, LD>;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t g = t / K::NKET;
+  if (g >= a.ntasks) return;
+  const int kc = (int)(t - g * K::NKET);
+  const int ic = kc / K::ND, id = kc - (kc / K::ND) * K::ND;
+  const BlockTask task = a.tasks[g];
+  const Pair bp = a.pairs[task.bra];
+  const Pair kp = a.pairs[task.ket];
+  const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
+  QuartetGeom geo;
+  double AB[3], CD[3];
+  load_geom(sA, sB, sC, sD, geo, AB, CD);
+  int ec[3], ed[3];
+  cart_xyz<LC>(ic, ec);
+  cart_xyz<LD>(id, ed);
+  double wk[3][K::MM + 1];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) ket_weights<K::MM>(ec[x], ed[x], CD[x], wk[x]);
+  double acc[K::NA][K::NB];
+#pragma unroll
+  for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < K::NB; ++ib) acc[ia][ib] = 0.0;
+  for (int ip = 0; ip < bp.np; ++ip) {
+    const PrimPair P = a.prims[bp.poff + ip];
+    for (int kq = 0; kq < kp.np; ++kq) {
+      const PrimPair Q = a.prims[kp.poff + kq];
+      PrimQ<K::NR> q;
+      prim_setup<K::NR>(P, Q, geo, q);
+      accumulate_split<LA, LB, LC, LD>(q, AB, wk, acc);
+    }
+  }
+  const int64_t o = task.base + (int64_t)(task.coff[2] + ic) * task.stride[2] +
+                    (int64_t)(task.coff[3] + id) * task.stride[3];
+#pragma unroll
+  for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < K::NB; ++ib)
+      a.out[o + (int64_t)(task.coff[0] + ia) * task.stride[0] + (int64_t)(task.coff[1] + ib) * task.stride[1]] =
+          acc[ia][ib];
+}
+
+// ----------------------------------------------------------------------------
+// Launchers (instantiated per class in jk_class.hip)
+// ----------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+void launch_jk_t(const JKArgs& a, hipStream_t s) {
+  using K = Class<LA, LB, LC, LD>;
+  if (a.nq <= 0) return;
+  const int64_t threads = a.nq * K::SPLIT;
+  const int64_t blocks = (threads + 255) / 256;
+  if constexpr (K::kSplit)
+    hipLaunchKernelGGL((jk_split_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((jk_full_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+}
+
+template <int LA, int LB, int LC, int LD>
+void launch_blocks_t(const BlockArgs& a, hipStream_t s) {
+  using K = Class<LA, LB, LC, LD>;
+  if (a.ntasks <= 0) return;
+  const int64_t threads = a.ntasks * K::NKET;
+  const int64_t blocks = (threads + 255) / 256;
+  hipLaunchKernelGGL((blocks_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+}
+
+}  // namespace gscf

@@ -1,0 +1,249 @@
+"""Direct J/K on the MI355X engine, behind the reference's Python call surface.
+
+Mirrors:
+  * direct_jk(atm, bas, env, density, *, cart, cutoff, workers, with_j, with_k,
+    coordinates) -- src/gradscf/integrals/backends/native/jk.py:399-446
+    (argument checks, complex densities split into real/imaginary parts,
+    (J, K) returned with the density's shape, zeros for an unselected operator);
+  * IntegralPlan.get_jk / make_plan -- src/gradscf/integrals/plan.py:71-92,184-190;
+  * NativeDirectBasis.get_jk -- src/gradscf/integrals/backends/native/jk.py:98-107.
+
+Densities and results are torch tensors on the GPU (float64).  Device plans
+(pairs, primitive-pair data, Schwarz bounds) are cached per (tables, cutoff)
+and reused across SCF iterations; the reference rebuilds them on every call.
+"""
+from __future__ import annotations
+
+import ctypes
+from collections import OrderedDict
+from dataclasses import dataclass
+from numbers import Integral
+
+import numpy as np
+import torch
+
+from . import _native
+from .basis import BasisParameters, BasisTopology
+from .packing import PTR_COORD, pack
+
+_PLAN_CACHE: "OrderedDict[tuple, DirectJKPlan]" = OrderedDict()
+_PLAN_CACHE_SIZE = 8
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("The MI355X J/K engine needs a GPU (torch.cuda is not available).")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream_handle() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DirectJKPlan:
+    """Owns one device plan (gscf_plan) for fixed tables and Schwarz cutoff."""
+
+    def __init__(self, atm, bas, env, *, cart: bool, cutoff: float = 0.0):
+        L = _native.lib()
+        self.atm = np.ascontiguousarray(atm, dtype=np.int32)
+        self.bas = np.ascontiguousarray(bas, dtype=np.int32)
+        self.env = np.ascontiguousarray(env, dtype=np.float64)
+        if self.atm.ndim != 2 or self.atm.shape[1] != 6 or self.bas.ndim != 2 or self.bas.shape[1] != 8:
+            raise ValueError("Invalid libcint table dimensions or operator")
+        self.cart = bool(cart)
+        self.cutoff = float(cutoff)
+        _device()
+        handle = ctypes.c_void_p()
+        _native.check(L.gscf_plan_create(
+            self.atm.ctypes.data, len(self.atm), self.bas.ctypes.data, len(self.bas),
+            self.env.ctypes.data, len(self.env), int(self.cart), ctypes.c_double(self.cutoff),
+            _stream_handle(), ctypes.byref(handle)))
+        self._handle = handle
+        info = _native.PlanInfo()
+        _native.check(L.gscf_plan_get_info(handle, ctypes.byref(info)))
+        self.info = {k: getattr(info, k) for k, _ in _native.PlanInfo._fields_}
+        self.nao = int(info.n)
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _native.lib().gscf_plan_destroy(h)
+            except Exception:
+                pass
+            self._handle = None
+
+    def jk(self, density: torch.Tensor, *, with_j=True, with_k=True, symmetric=None,
+           validate=True, out: torch.Tensor | None = None) -> torch.Tensor:
+        """density (nb, n, n) float64 on the GPU -> out (nops, nb, n, n)."""
+        d = density
+        if d.dim() == 2:
+            d = d[None]
+        n = self.nao
+        if d.dim() != 3 or d.shape[1:] != (n, n) or d.dtype != torch.float64 or not d.is_cuda:
+            raise ValueError("Direct J/K shape mismatch")
+        d = d.contiguous()
+        nb = d.shape[0]
+        nops = int(bool(with_j)) + int(bool(with_k))
+        if out is None:
+            out = torch.empty((nops, nb, n, n), dtype=torch.float64, device=d.device)
+        if symmetric is None:
+            symmetric = bool(torch.equal(d, d.transpose(1, 2)))
+        flags = (_native.FLAG_SYMMETRIC_DENSITY if symmetric else 0) | (0 if validate else _native.FLAG_NO_VALIDATE)
+        _native.check(_native.lib().gscf_jk_build(
+            self._handle, d.data_ptr(), nb, out.data_ptr(), int(bool(with_j)), int(bool(with_k)),
+            flags, ctypes.c_double(0.0), _stream_handle()))
+        return out
+
+    def schwarz(self) -> np.ndarray:
+        """Parent-shell Schwarz bounds (nbas x nbas, lower triangle) of a screened plan."""
+        q = np.zeros((len(self.bas), len(self.bas)))
+        _native.check(_native.lib().gscf_plan_get_schwarz(self._handle, q.ctypes.data))
+        return q
+
+    def profile(self, enable: bool) -> None:
+        """Start (and reset) or stop per-class HIP-event timing of jk() launches."""
+        _native.check(_native.lib().gscf_plan_profile(self._handle, int(bool(enable))))
+
+    def class_times(self) -> list:
+        """[(la, lb, lc, ld), summed_ms, launches] per quartet class since profile(True)."""
+        cap = 64
+        cls = np.zeros((cap, 4), dtype=np.int32)
+        ms = np.zeros(cap)
+        la = np.zeros(cap, dtype=np.int64)
+        cnt = ctypes.c_int32()
+        _native.check(_native.lib().gscf_plan_class_times(self._handle, cap, cls.ctypes.data, ms.ctypes.data,
+                                                          la.ctypes.data, ctypes.byref(cnt)))
+        return [(tuple(int(v) for v in cls[i]), float(ms[i]), int(la[i])) for i in range(cnt.value)]
+
+    def int1e(self):
+        """(S, T, V) as (n, n) float64 GPU tensors in the plan's AO basis."""
+        dev = _device()
+        s, t, v = (torch.empty((self.nao, self.nao), dtype=torch.float64, device=dev) for _ in range(3))
+        _native.check(_native.lib().gscf_int1e(self._handle, s.data_ptr(), t.data_ptr(), v.data_ptr(),
+                                               _stream_handle()))
+        return s, t, v
+
+    def eri_blocks(self, quartets) -> list:
+        """Cartesian contracted blocks for parent-shell quartets, each as an array
+        [d0, d1, d2, d3] (libcint layout transposed to C order)."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1, 4))
+        ao = [0]
+        for row in self.bas:
+            ao.append(ao[-1] + int(row[3]) * (int(row[1]) + 1) * (int(row[1]) + 2) // 2)
+        dims = [[ao[s + 1] - ao[s] for s in quart] for quart in q]
+        sizes = [int(np.prod(d)) for d in dims]
+        offs = np.zeros(len(q), dtype=np.int64)
+        if len(q):
+            offs[1:] = np.cumsum(sizes)[:-1]
+        out = torch.zeros(max(1, int(sum(sizes))), dtype=torch.float64, device=_device())
+        _native.check(_native.lib().gscf_eri_blocks(self._handle, q.ctypes.data, len(q), offs.ctypes.data,
+                                                    out.data_ptr(), _stream_handle()))
+        h = out.cpu().numpy()
+        return [h[o:o + s].reshape(d[3], d[2], d[1], d[0]).transpose(3, 2, 1, 0)
+                for o, s, d in zip(offs, sizes, dims)]
+
+
+def get_plan(atm, bas, env, *, cart: bool, cutoff: float) -> DirectJKPlan:
+    atm = np.ascontiguousarray(atm, dtype=np.int32)
+    bas = np.ascontiguousarray(bas, dtype=np.int32)
+    env = np.ascontiguousarray(env, dtype=np.float64)
+    key = (atm.tobytes(), bas.tobytes(), env.tobytes(), bool(cart), float(cutoff))
+    plan = _PLAN_CACHE.get(key)
+    if plan is None:
+        plan = DirectJKPlan(atm, bas, env, cart=cart, cutoff=cutoff)
+        _PLAN_CACHE[key] = plan
+        while len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
+            _PLAN_CACHE.popitem(last=False)
+    else:
+        _PLAN_CACHE.move_to_end(key)
+    return plan
+
+
+def _validate_selection(with_j, with_k):
+    if not all(isinstance(flag, (bool, np.bool_)) for flag in (with_j, with_k)):
+        raise ValueError("with_j and with_k must be static booleans.")
+
+
+def direct_jk(atm, bas, env, density, *, cart, cutoff=0.0, workers=1, with_j=True, with_k=True,
+              coordinates=None):
+    """Contract J/K for density (..., n, n); returns (J, K) shaped like density."""
+    _validate_selection(with_j, with_k)
+    if isinstance(workers, bool) or not isinstance(workers, Integral) or workers < 1:
+        raise ValueError("workers must be a positive integer.")
+    d = density if isinstance(density, torch.Tensor) else torch.as_tensor(np.asarray(density))
+    if d.dim() < 2:
+        raise ValueError("Direct J/K requires (...,n,n) float64/complex128 density.")
+    n = d.shape[-1]
+    if tuple(d.shape[-2:]) != (n, n) or d.dtype not in (torch.float64, torch.complex128):
+        raise ValueError("Direct J/K requires (...,n,n) float64/complex128 density.")
+    env = np.asarray(env)
+    if env.dtype != np.float64:
+        raise ValueError("Direct J/K requires float64 environment.")
+    if not (with_j or with_k):
+        return torch.zeros_like(d), torch.zeros_like(d)
+    if coordinates is not None:
+        coordinates = np.asarray(coordinates, dtype=np.float64)
+        if coordinates.shape != (len(atm), 3):
+            raise ValueError("Direct J/K coordinates must contain every nuclear and basis-center atom.")
+        env = env.copy()
+        for i, row in enumerate(np.asarray(atm)):
+            env[int(row[PTR_COORD]):int(row[PTR_COORD]) + 3] = coordinates[i]
+    dev = _device()
+    flat = d.reshape(-1, n, n).to(dev)
+    count = flat.shape[0]
+    real = torch.cat((flat.real, flat.imag), 0) if flat.is_complex() else flat
+    plan = get_plan(atm, bas, env, cart=cart, cutoff=cutoff)
+    if plan.nao != n:
+        raise ValueError("Direct J/K shape mismatch")
+    out = plan.jk(real.contiguous(), with_j=with_j, with_k=with_k)
+    if flat.is_complex():
+        out = out[:, :count] + 1j * out[:, count:]
+    out = out.reshape(int(with_j) + int(with_k), *d.shape)
+    return (out[0] if with_j else torch.zeros_like(out[0]),
+            out[-1] if with_k else torch.zeros_like(out[0]))
+
+
+@dataclass(frozen=True)
+class IntegralPlan:
+    """Plan over a basis topology (the native backend only)."""
+
+    topology: BasisTopology
+    backend: str = "native"
+
+    def pack(self, parameters: BasisParameters, *, origin=None):
+        return pack(self.topology, parameters, origin=origin)
+
+    def get_jk(self, parameters: BasisParameters, density, *, screening_threshold=0.0, workers=1,
+               with_j=True, with_k=True, omega=0.0):
+        if omega:
+            raise NotImplementedError("Range-separated (omega != 0) J/K is not supported by the MI355X engine.")
+        if self.backend != "native":
+            raise NotImplementedError("get_jk requires the native backend.")
+        atm, bas, env = self.pack(parameters)
+        coordinates = np.concatenate([np.asarray(parameters.nuclear_coords).reshape(-1, 3),
+                                      np.asarray(parameters.centers).reshape(-1, 3)])
+        if tuple(density.shape[-2:]) != (self.topology.nao(),) * 2:
+            raise ValueError("Density AO dimensions do not match the plan.")
+        return direct_jk(atm, bas, env, density, cart=self.topology.cart, cutoff=screening_threshold,
+                         workers=workers, with_j=with_j, with_k=with_k, coordinates=coordinates)
+
+
+def make_plan(topology: BasisTopology, backend: str = "native") -> IntegralPlan:
+    if backend != "native":
+        raise ValueError("Only the native (MI355X) backend is available.")
+    return IntegralPlan(topology, backend)
+
+
+@dataclass
+class NativeDirectBasis:
+    plan: IntegralPlan
+    parameters: BasisParameters
+
+    @property
+    def nao(self) -> int:
+        return self.plan.topology.nao()
+
+    def get_jk(self, density, *, screening_threshold=0.0, workers=1, with_j=True, with_k=True, omega=0.0):
+        return self.plan.get_jk(self.parameters, density, screening_threshold=screening_threshold,
+                                workers=workers, with_j=with_j, with_k=with_k, omega=omega)

@@ -1,0 +1,113 @@
+/* gradscf_jk.h -- C ABI of the MI355X (gfx950) direct-SCF J/K engine.
+ *
+ * Drop-in boundary for the reference's native direct-J/K FFI target
+ *   "gradscf_direct_jk_cpu_v1" -> GradSCFDirectJK
+ *   (src/gradscf/integrals/_native/csrc/direct_jk.cc:7-102, binding :97-102:
+ *    args atm S32[natm,6], bas S32[nbas,8], env F64[nenv], density F64[nb,n,n];
+ *    result F64[nops,nb,n,n] (J block first, then K); attrs cart, cutoff,
+ *    workers, with_j, with_k)
+ * and for its geometry VJP target
+ *   "gradscf_direct_jk_geometry_vjp_cpu_v2" -> GradSCFDirectJKGeometryVJPParallel
+ *   (src/gradscf/integrals/_native/csrc/direct_jk_geometry.cc:122-201).
+ *
+ * Tables (atm/bas/env) are host arrays in libcint layout.  Densities, results
+ * and gradients are DEVICE pointers (hipMalloc'd, row-major float64).  Streams
+ * are hipStream_t passed as void* (NULL = default stream).  Every call
+ * validates its arguments on the host before any launch and returns
+ *   GSCF_OK, GSCF_INVALID_ARGUMENT (-> ffi::Error::InvalidArgument) or
+ *   GSCF_INTERNAL (-> ffi::Error::Internal), with the message in
+ *   gscf_last_error() (thread-local), mirroring direct_jk.cc:78-89.
+ */
+#ifndef GRADSCF_JK_H
+#define GRADSCF_JK_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSCF_OK 0
+#define GSCF_INVALID_ARGUMENT 1
+#define GSCF_INTERNAL 2
+
+/* jk flags */
+#define GSCF_FLAG_SYMMETRIC_DENSITY 1u  /* caller promises D == D^T (halves K digestion) */
+#define GSCF_FLAG_MIXED_PRECISION 2u    /* FP32 quartets where Q_ij Q_kl max|D| < mixed_threshold */
+#define GSCF_FLAG_NO_VALIDATE 4u        /* skip the device finite-density check (no host sync) */
+
+typedef struct gscf_plan gscf_plan;
+
+typedef struct {
+  int32_t n;            /* AO count (spherical or Cartesian, as requested) */
+  int32_t n_cart;       /* Cartesian AO count used internally */
+  int32_t nbas;         /* shells in the tables */
+  int32_t nseg;         /* segmented (single-contraction) shells */
+  int64_t npairs;       /* retained segmented shell pairs */
+  int64_t nquartets;    /* retained canonical segmented shell quartets */
+  double cutoff;        /* Schwarz cutoff */
+  double plan_ms;       /* wall time of gscf_plan_create */
+} gscf_plan_info;
+
+/* Last error message of the calling thread. */
+const char* gscf_last_error(void);
+
+/* Build a device plan: validates tables exactly like IntegralTables
+ * (csrc/tables.h:31-76), segments generally contracted shells, computes the
+ * Schwarz bounds Q_ij = sqrt(max |(pq|pq)|) on the GPU when cutoff > 0
+ * (csrc/coulomb_shells.h:60-77) and builds Schwarz-sorted pair lists per
+ * angular-momentum class.  Replaces the per-call table copy, libcint optimizer
+ * and bounds of direct_jk.cc:18-33; the plan may be reused for any density. */
+int gscf_plan_create(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas,
+                     const double* env, int32_t nenv, int32_t cart, double cutoff,
+                     void* stream, gscf_plan** plan);
+int gscf_plan_destroy(gscf_plan* plan);
+int gscf_plan_get_info(const gscf_plan* plan, gscf_plan_info* info);
+
+/* Parent-shell Schwarz bounds q[i*nbas + j] (i >= j) used by the plan
+ * (csrc/coulomb_shells.h:60-77); only for plans built with cutoff > 0. */
+int gscf_plan_get_schwarz(const gscf_plan* plan, double* q);
+
+/* Measurement hooks (no counterpart in the reference): while enabled, each
+ * gscf_jk_build records a HIP event pair around every class kernel on the
+ * launch stream; gscf_plan_class_times returns the summed elapsed ms and
+ * launch counts per class (classes[i*4..] = la, lb, lc, ld). */
+int gscf_plan_profile(gscf_plan* plan, int32_t enable);
+int gscf_plan_class_times(gscf_plan* plan, int32_t max_classes, int32_t* classes, double* ms,
+                          int64_t* launches, int32_t* count);
+
+/* J/K for nb densities: density F64[nb,n,n] -> out F64[nops,nb,n,n],
+ * nops = with_j + with_k (J block first), on `stream`.  Asynchronous. */
+int gscf_jk_build(gscf_plan* plan, const double* density, int32_t nb, double* out,
+                  int32_t with_j, int32_t with_k, uint32_t flags, double mixed_threshold,
+                  void* stream);
+
+/* One-shot form with the argument list of GradSCFDirectJK (direct_jk.cc:97-102):
+ * builds a plan, runs gscf_jk_build, synchronises `stream`, frees the plan.
+ * `workers` is validated (>= 1) and otherwise unused on the GPU. */
+int gscf_direct_jk(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas,
+                   const double* env, int32_t nenv, const double* density, int32_t nb, int32_t n,
+                   double* out, int32_t cart, double cutoff, int32_t workers, int32_t with_j,
+                   int32_t with_k, void* stream);
+
+/* Contracted ERI blocks for an explicit list of shell quartets (shls int32[nq,4],
+ * host), written in libcint layout (first index fastest) at out + offsets[q]
+ * (device).  Cartesian or spherical per the plan.  The reference's
+ * int2e_cart/int2e_sph shell-block call (csrc/coulomb_shells.h:32-58). */
+int gscf_eri_blocks(gscf_plan* plan, const int32_t* shls, int64_t nq, const int64_t* offsets,
+                    double* out, void* stream);
+
+/* One-electron S, T, V (each F64[n,n], device) in the plan's AO basis: the
+ * reference's int1e_ovlp / int1e_kin / int1e_nuc evaluation used for the SCF
+ * core Hamiltonian (csrc/ffi.cc:11-86). */
+int gscf_int1e(gscf_plan* plan, double* s, double* t, double* v, void* stream);
+
+/* Coordinate VJP of J/K (direct_jk_geometry.cc:122-182, cutoff must be 0 in the
+ * reference): grad[natm,3] (device) += d/dR sum(cot_j*J(D)) + sum(cot_k*K(D))
+ * for the atm-row coordinates shells sit on.  cot_j / cot_k may be NULL. */
+int gscf_jk_geometry_vjp(gscf_plan* plan, const double* density, const double* cot_j,
+                         const double* cot_k, double* grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,174 @@
+"""Host-side checks that need no GPU: basis parsing and packing conventions, the
+workload geometry generators, the Rys interpolation table, and the C ABI
+library (loads, exports every symbol include/gradscf_jk.h declares)."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gradscf_amd import molecule as M
+from gradscf_amd.basis import normalized_shell_coefficients, prepare_basis
+from gradscf_amd.packing import ao_offsets, pack
+from tests import systems
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.mark.parametrize("spec,basis,cart,nshell,nao", [
+    (M.water_monomer(), "6-31g", True, 9, 13),
+    (M.water_cluster(16, seed=16, box=12.0), "cc-pvdz", False, 176, 384),
+    (M.water_cluster(16, seed=16, box=12.0), "cc-pvdz", True, 176, 400),
+    (M.linear_alkane(20), "6-31g*", True, 204, 384),
+    (M.water_cluster(32, seed=32, box=15.0), "6-31g*", True, 320, 608),
+])
+def test_workload_sizes(spec, basis, cart, nshell, nao):
+    """Shell and AO counts of the benchmark systems (basis compositions of the
+    reference's PySCF snapshot: 6-31G O/C S6 S3 S1 P3 P1, H S3 S1; 6-31G* adds a
+    Cartesian d on O/C; cc-pVDZ O S[8,2c] S1 P3 P1 D1, H S3 S1 P1)."""
+    topo, _ = prepare_basis(spec, basis, cart=cart)
+    assert topo.nshell == nshell
+    assert topo.nao() == nao
+
+
+def test_shell_order_within_atom():
+    topo, par = prepare_basis(M.water_monomer(), "cc-pvdz")
+    assert topo.angular_momenta[:5] == (0, 0, 1, 1, 2)
+    assert topo.primitive_counts[:5] == (8, 1, 3, 1, 1)
+    assert topo.contraction_counts[:5] == (2, 1, 1, 1, 1)
+    assert topo.angular_momenta[5:] == (0, 0, 1, 0, 0, 1)
+
+
+def test_normalized_coefficients_unit_norm():
+    """Contracted radial functions r^l sum c exp(-a r^2) have unit norm."""
+    import math
+    topo, par = prepare_basis(M.water_monomer(), "cc-pvdz")
+    for l, a, c in zip(topo.angular_momenta, par.exponents, par.coefficients):
+        cn = normalized_shell_coefficients(l, a, c)
+        for col in range(cn.shape[1]):
+            s = 0.0
+            for p in range(len(a)):
+                for q in range(len(a)):
+                    s += cn[p, col] * cn[q, col] * math.gamma(l + 1.5) / (2 * (a[p] + a[q]) ** (l + 1.5))
+            assert abs(s - 1.0) < 1e-13
+
+
+def test_pack_layout():
+    topo, par = prepare_basis(M.water_monomer(), "6-31g")
+    atm, bas, env = pack(topo, par)
+    assert atm.shape == (3 + 9, 6) and bas.shape == (9, 8)
+    assert list(atm[:3, 0]) == [8, 1, 1] and np.all(atm[3:, 0] == 0) and np.all(atm[:, 2] == 1)
+    assert list(atm[:, 1]) == [20 + 3 * i for i in range(12)]
+    assert bas[0, 0] == 3 and bas[0, 5] == 20 + 36 and bas[0, 6] == bas[0, 5] + 6
+    np.testing.assert_allclose(env[atm[4, 1]:atm[4, 1] + 3], par.coords_bohr[0] if hasattr(par, "coords_bohr")
+                               else par.centers[1])
+    assert int(ao_offsets(bas, True)[-1]) == 13
+
+
+def test_geometries_match_fixtures():
+    for name, spec in [("water", M.water_monomer()),
+                       ("water16", M.water_cluster(16, seed=16, box=12.0)),
+                       ("water32", M.water_cluster(32, seed=32, box=15.0)),
+                       ("c20h42", M.linear_alkane(20))]:
+        ref = M.from_xyz((GOLDEN / f"{name}.xyz").read_text())
+        assert ref.symbols == spec.symbols
+        np.testing.assert_allclose(spec.coords_bohr, ref.coords_bohr, atol=1e-9)
+
+
+def test_water_cluster_rules():
+    spec = M.water_cluster(16, seed=16, box=12.0)
+    r = spec.coords_bohr / M.ANGSTROM_TO_BOHR
+    o = r[0::3]
+    assert np.all((o >= 0) & (o <= 12.0))
+    d = np.linalg.norm(o[:, None] - o[None], axis=-1) + np.eye(16) * 100
+    assert d.min() >= 2.6
+    for w in range(16):
+        oh1, oh2 = r[3 * w + 1] - r[3 * w], r[3 * w + 2] - r[3 * w]
+        assert abs(np.linalg.norm(oh1) - 0.96) < 1e-12 and abs(np.linalg.norm(oh2) - 0.96) < 1e-12
+        ang = np.degrees(np.arccos(oh1 @ oh2 / 0.96 ** 2))
+        assert abs(ang - 104.5) < 1e-9
+
+
+def test_alkane_geometry():
+    spec = M.linear_alkane(20)
+    r = spec.coords_bohr / M.ANGSTROM_TO_BOHR
+    c, h = r[:20], r[20:]
+    assert len(h) == 42
+    assert np.allclose(np.linalg.norm(np.diff(c, axis=0), axis=1), 1.54)
+    for hh in h:
+        assert abs(np.min(np.linalg.norm(c - hh, axis=1)) - 1.09) < 1e-12
+    v1, v2 = c[0] - c[1], c[2] - c[1]
+    assert abs(np.degrees(np.arccos(v1 @ v2 / 1.54 ** 2)) - 109.4712206) < 1e-6
+
+
+def test_rys_table_against_mpmath():
+    """The generated interpolation table reproduces 40-digit roots/weights."""
+    mpmath = pytest.importorskip("mpmath")
+    import importlib.util
+    path = ROOT / "gradscf_amd" / "csrc" / "gen_rys_tables.py"
+    spec = importlib.util.spec_from_file_location("gen", path)
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    head = (ROOT / "gradscf_amd/csrc/_generated/rys_table.inc")
+    body = (ROOT / "gradscf_amd/csrc/_generated/rys_table_body.inc")
+    if not head.is_file():
+        pytest.skip("engine not built")
+    txt = head.read_text()
+    off = [int(v) for v in re.search(r"rys_table_offset\[8\] = \{([^}]*)\}", txt).group(1).split(",")]
+    tab = np.array([float.fromhex(v.strip()) for v in body.read_text().replace("\n", "").split(",") if v.strip()])
+    rng = np.random.default_rng(1)
+    for n in range(1, 8):
+        for x in list(rng.uniform(0, 80, 6)) + [0.0, 79.999]:
+            k = int(x / 2)
+            y = x - (2 * k + 1)
+            c = tab[off[n - 1] + k * 28 * n: off[n - 1] + (k + 1) * 28 * n].reshape(14, 2 * n)
+            b1 = np.zeros(2 * n)
+            b2 = np.zeros(2 * n)
+            for jj in range(13, 0, -1):
+                b1, b2 = c[jj] + 2 * y * b1 - b2, b1
+            f = c[0] + y * b1 - b2
+            r, w = gen._rys_mp(n, float(x))
+            assert np.max(np.abs(f[:n] - r) / np.array(r)) < 5e-15
+            assert np.max(np.abs(f[n:] - w) / np.array(w)) < 5e-15
+
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "gradscf_jk.h").read_text()
+    declared = set(re.findall(r"\b(gscf_[a-z0-9_]+)\s*\(", header))
+    assert declared
+    lib_path = ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so"
+    if not lib_path.is_file():
+        pytest.skip("engine not built")
+    lib = ctypes.CDLL(str(lib_path), mode=ctypes.RTLD_LOCAL)
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    from gradscf_amd import _native
+    assert set(_native.EXPORTS) == declared
+
+
+def test_c_abi_rejects_bad_tables_without_gpu():
+    """Host validation runs before any device call, so it works on a CPU-only host."""
+    from gradscf_amd import _native
+    lib_path = ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so"
+    if not lib_path.is_file():
+        pytest.skip("engine not built")
+    L = _native.lib()
+    _, _, atm, bas, env = systems.tables(M.water_monomer(), "6-31g")
+    bad = bas.copy()
+    bad[0, 1] = 3  # f shell: outside the engine's s/p/d support
+    env_f = np.ascontiguousarray(env)
+    h = ctypes.c_void_p()
+    rc = L.gscf_plan_create(atm.ctypes.data, len(atm), bad.ctypes.data, len(bad), env_f.ctypes.data, len(env_f),
+                            1, ctypes.c_double(0.0), None, ctypes.byref(h))
+    assert rc == _native.GSCF_INVALID_ARGUMENT
+    assert b"l <= 2" in L.gscf_last_error()
+    env2 = env_f.copy()
+    env2[8] = 0.3
+    rc = L.gscf_plan_create(atm.ctypes.data, len(atm), bas.ctypes.data, len(bas), env2.ctypes.data, len(env2),
+                            1, ctypes.c_double(0.0), None, ctypes.byref(h))
+    assert rc == _native.GSCF_INVALID_ARGUMENT
+    rc = L.gscf_direct_jk(atm.ctypes.data, len(atm), bas.ctypes.data, len(bas), env_f.ctypes.data, len(env_f),
+                          None, 1, 13, None, 1, ctypes.c_double(0.0), 0, 1, 1, None)
+    assert rc == _native.GSCF_INVALID_ARGUMENT and b"workers" in L.gscf_last_error()

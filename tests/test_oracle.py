@@ -1,0 +1,163 @@
+"""Pins of the oracle (CPU).  Each check is chosen so that a plausible slip in the
+restatement (a dropped term, a wrong sign or index, a transposed operand)
+fails it:
+  * Boys F_m against 40-digit mpmath, closed forms and the recursion identity;
+  * Rys roots/weights against the moment identities sum w s^k = F_k (k < 2n)
+    and against the reference libcint CINTrys_roots (when oracle/_ref is built);
+  * (ss|ss) closed form, 8-fold permutational symmetry of ERI blocks;
+  * ERI / int2e_ip1 / one-electron blocks against the reference libcint build;
+  * J/K against the brute-force full-tensor contraction (the reference's own
+    J/K test oracle, tests/integrals/test_native_direct_parallel.py:45-60);
+  * RHF energies against the reference's published values
+    (examples/geometry/water_correlated_forces.py:25-30,84-87);
+  * the committed golden fixtures (tests/golden/).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from gradscf_amd import molecule as M
+from tests import systems
+
+ref_only = pytest.mark.skipif("not __import__('oracle').have_ref()", reason="oracle/_ref not built")
+
+
+def test_boys_vs_mpmath_and_closed_forms(oracle_mod):
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.dps = 40
+    for x in [0.0, 1e-12, 1e-3, 0.5, 3.0, 12.0, 29.9, 44.99, 45.01, 60.0, 150.0]:
+        f = oracle_mod.boys(16, x)
+        for m in range(17):
+            exact = mpmath.hyp1f1(m + 0.5, m + 1.5, -x) / (2 * m + 1)
+            assert abs(f[m] - float(exact)) <= 4e-16 * float(exact)
+        if x > 0:
+            assert abs(f[0] - 0.5 * math.sqrt(math.pi / x) * math.erf(math.sqrt(x))) < 1e-15
+            for m in range(16):  # downward recursion identity
+                assert abs((2 * m + 1) * f[m] - (2 * x * f[m + 1] + math.exp(-x))) < 1e-14 * (2 * m + 1) * f[m]
+    f0 = oracle_mod.boys(10, 0.0)
+    assert np.allclose(f0, [1 / (2 * m + 1) for m in range(11)], rtol=0, atol=1e-16)
+
+
+@pytest.mark.parametrize("n", range(1, 8))
+def test_rys_moment_identities(oracle_mod, n):
+    for x in [0.0, 1e-9, 0.7, 4.0, 11.0, 27.0, 55.0, 89.0, 91.0, 300.0, 5e4]:
+        s, w = oracle_mod.rys_roots(n, x)
+        assert np.all(np.diff(s) > 0) and np.all((s > 0) & (s < 1)) and np.all(w > 0)
+        f = oracle_mod.boys(2 * n - 1, x)
+        for k in range(2 * n):
+            assert abs(np.sum(w * s ** k) - f[k]) <= 5e-15 * f[k]
+
+
+@ref_only
+@pytest.mark.parametrize("n", range(1, 8))
+def test_rys_vs_libcint(oracle_mod, n):
+    # libcint's own fitted roots are accurate to ~1e-11 relative for 6-7 roots;
+    # the oracle satisfies the moment identities to 1e-15 (test above).
+    tol = 1e-12 if n <= 5 else 5e-10
+    for x in [0.0, 0.3, 2.0, 7.5, 15.0, 33.0, 49.0, 80.0, 200.0]:
+        s, w = oracle_mod.rys_roots(n, x)
+        u, wl = oracle_mod.libcint_rys_roots(n, x)
+        assert np.max(np.abs(s - u / (1 + u)) / s) < tol
+        assert np.max(np.abs(w - wl) / wl) < tol * 10
+
+
+def test_ssss_closed_form(oracle_mod):
+    """Single normalised s primitives on two centres: (ss|ss) = 2 pi^{5/2}/(pq sqrt(p+q)) K K F0(T)."""
+    spec = M.parse_molecule_spec("H 0 0 0; H 0 0 1.3", unit="Bohr")
+    basis = {"H": [[0, [0.8, 1.0]]]}
+    _, _, atm, bas, env = systems.tables(spec, basis)
+    a = 0.8
+    N = (2 * a / math.pi) ** 0.75  # normalised s primitive
+    for q in [(0, 0, 0, 0), (0, 1, 0, 1), (0, 0, 1, 1), (0, 1, 1, 0)]:
+        R = [np.array([0, 0, 0.0]), np.array([0, 0, 1.3])]
+        A, B, C, D = (R[i] for i in q)
+        p = qq = 2 * a
+        P, Q = (A + B) / 2, (C + D) / 2
+        kab = math.exp(-a * a / p * np.sum((A - B) ** 2))
+        kcd = math.exp(-a * a / qq * np.sum((C - D) ** 2))
+        T = p * qq / (p + qq) * np.sum((P - Q) ** 2)
+        f0 = 1.0 if T == 0 else 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+        exact = N ** 4 * 2 * math.pi ** 2.5 / (p * qq * math.sqrt(p + qq)) * kab * kcd * f0
+        got = oracle_mod.eri_block(q, atm, bas, env, True)[0, 0, 0, 0]
+        assert abs(got - exact) < 1e-14 * exact
+
+
+def test_eri_permutational_symmetry(oracle_mod):
+    _, _, atm, bas, env = systems.tables(systems.water(), "6-31g*", cart=True)
+    rng = np.random.default_rng(5)
+    for _ in range(12):
+        i, j, k, l = (int(v) for v in rng.integers(0, len(bas), 4))
+        b = oracle_mod.eri_block((i, j, k, l), atm, bas, env, True)
+        assert np.allclose(oracle_mod.eri_block((j, i, k, l), atm, bas, env, True), b.transpose(1, 0, 2, 3),
+                           atol=1e-14)
+        assert np.allclose(oracle_mod.eri_block((k, l, i, j), atm, bas, env, True), b.transpose(2, 3, 0, 1),
+                           atol=1e-14)
+        assert np.allclose(oracle_mod.eri_block((i, j, l, k), atm, bas, env, True), b.transpose(0, 1, 3, 2),
+                           atol=1e-14)
+
+
+@ref_only
+@pytest.mark.parametrize("basis,cart", [("6-31g*", True), ("cc-pvdz", True), ("cc-pvdz", False)])
+def test_blocks_vs_libcint(oracle_mod, basis, cart):
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), basis, cart=cart)
+    rng = np.random.default_rng(9)
+    for _ in range(25):
+        q = tuple(int(v) for v in rng.integers(0, len(bas), 4))
+        a = oracle_mod.eri_block(q, atm, bas, env, cart)
+        b = oracle_mod.libcint_block(q, atm, bas, env, cart)
+        assert np.max(np.abs(a - b)) < 3e-13
+        a1 = oracle_mod.eri_block(q, atm, bas, env, cart, deriv=1)
+        b1 = oracle_mod.libcint_block(q, atm, bas, env, cart, "int2e_ip1")
+        assert np.max(np.abs(a1 - b1)) < 3e-13
+
+
+@ref_only
+@pytest.mark.parametrize("cart", [True, False])
+def test_int1e_vs_libcint(oracle_mod, cart):
+    _, _, atm, bas, env = systems.tables(systems.water(), "cc-pvdz", cart=cart)
+    mine = oracle_mod.int1e(atm, bas, env, cart)
+    ref = oracle_mod.libcint_int1e(atm, bas, env, cart)
+    for a, b in zip(mine, ref):
+        assert np.max(np.abs(a - b)) < 1e-12
+
+
+@pytest.mark.parametrize("cart", [True, False])
+def test_jk_vs_full_tensor(oracle_mod, cart):
+    _, _, atm, bas, env = systems.tables(systems.water(), "cc-pvdz", cart=cart)
+    n = oracle_mod.nao(bas, cart)
+    d = systems.random_density(n, nb=2, seed=3, symmetric=False)
+    eri = oracle_mod.full_eri(atm, bas, env, cart)
+    j = np.einsum("pqrs,brs->bpq", eri, d)
+    k = np.einsum("prqs,brs->bpq", eri, d)
+    for workers in (1, 4):
+        out = oracle_mod.direct_jk(atm, bas, env, d, cart=cart, workers=workers)
+        assert np.max(np.abs(out[0] - j)) < 2e-13
+        assert np.max(np.abs(out[1] - k)) < 2e-13
+
+
+@ref_only
+def test_driver_on_libcint_matches_own_rys(oracle_mod):
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "6-31g*", cart=True)
+    n = oracle_mod.nao(bas, True)
+    d = systems.random_density(n, seed=4)
+    own = oracle_mod.direct_jk(atm, bas, env, d, cart=True, cutoff=1e-12, workers=4)
+    with oracle_mod.backend("libcint"):
+        ref = oracle_mod.direct_jk(atm, bas, env, d, cart=True, cutoff=1e-12, workers=4)
+    assert np.max(np.abs(own - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("basis,energy", [("6-31g*", -76.010492818245), ("sto-3g", -74.963082931333)])
+def test_published_rhf_energies(oracle_mod, basis, energy):
+    from oracle import scf
+    spec = M.parse_molecule_spec("O 0 0 0; H 0 1.43 1.11; H 0 -1.43 1.11", unit="Bohr")
+    _, _, atm, bas, env = systems.tables(spec, basis, cart=True)
+    s, t, v = oracle_mod.int1e(atm, bas, env, True)
+
+    def jk(d):
+        o = oracle_mod.direct_jk(atm, bas, env, d, cart=True, workers=4)
+        return o[0, 0], o[1, 0]
+
+    r = scf.rhf(t + v, s, spec.nelectron, spec.nuclear_repulsion(), jk)
+    assert r["converged"]
+    assert abs(r["energy"] - energy) < 1e-9
