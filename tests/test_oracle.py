@@ -161,3 +161,26 @@ def test_published_rhf_energies(oracle_mod, basis, energy):
     r = scf.rhf(t + v, s, spec.nelectron, spec.nuclear_repulsion(), jk)
     assert r["converged"]
     assert abs(r["energy"] - energy) < 1e-9
+
+
+def test_oracle_matches_golden_fixtures(oracle_mod):
+    """The own-Rys oracle reproduces the fixtures made on the reference libcint."""
+    import json
+    from pathlib import Path
+    g = Path(__file__).resolve().parent / "golden"
+    f = np.load(g / "water_631g_jk.npz")
+    _, _, atm, bas, env = systems.tables(M.water_monomer(), "6-31g", cart=True)
+    out = oracle_mod.direct_jk(atm, bas, env, f["density"], cart=True)
+    assert np.max(np.abs(out[0] - f["J"])) < 1e-12 and np.max(np.abs(out[1] - f["K"])) < 1e-12
+    f = np.load(g / "dimer_ccpvdz_sph_jk.npz")
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "cc-pvdz", cart=False)
+    out = oracle_mod.direct_jk(atm, bas, env, f["density"], cart=False, cutoff=1e-12, workers=4)
+    assert np.max(np.abs(out[0, 0] - f["J"])) < 1e-11 and np.max(np.abs(out[1, 0] - f["K"])) < 1e-11
+    e = json.loads((g / "scf_energies.json").read_text())["water_631g_cart"]["energy"]
+    from oracle import scf
+    spec = M.water_monomer()
+    _, _, atm, bas, env = systems.tables(spec, "6-31g", cart=True)
+    s, t, v = oracle_mod.int1e(atm, bas, env, True)
+    r = scf.rhf(t + v, s, spec.nelectron, spec.nuclear_repulsion(),
+                lambda d: tuple(oracle_mod.direct_jk(atm, bas, env, d, cart=True)[:, 0]))
+    assert abs(r["energy"] - e) < 1e-10
