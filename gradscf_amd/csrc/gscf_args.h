@@ -29,18 +29,54 @@ struct ShellXYZ {
 };
 
 
-// One canonical quartet class: bra pairs [bra0, bra0+nbra) of pair class P, ket
-// pairs [ket0, ...) of class Q <= P.  Bra b owns kets [0, qoff[b+1]-qoff[b]).
+// Work of one quartet class is a list of canonical quartets (bra pair, ket
+// pair), grouped into sub-tasks of similar primitive counts.  Sub-task t owns
+// quartets [q0, q0+nq) and work items [item0, item0 + S*nq): item
+// item0 + s*nq + g processes quartet q0+g for bra primitive pairs s, s+S, ...
+struct SubTask {
+  int64_t item0;
+  int64_t q0;
+  int64_t nq;
+  int32_t S;
+  int32_t pad;
+};
+
 struct JKArgs {
   const Pair* pairs;
   const PrimPair* prims;
   const ShellXYZ* shells;
-  const int64_t* qoff;
-  int bra0, ket0, nbra, same_class;
-  int64_t nq;
+  const int2* quartets;
+  const SubTask* tasks;
+  int ntasks;
+  int64_t nitems;
   const double* dens;   // [nbd][n][n] Cartesian densities
   double* jp;           // [nbj][n][n] partial J (J = Jp + Jp^T)
   double* kp;           // [nbd][n][n] partial K
+  int n, nbd, nbj, with_j, with_k;
+};
+
+// Bra-major chunk: quartets [q0, q0+nq) of one sub-task whose bras are the
+// consecutive pairs [bra0, bra0+nbra); one workgroup per chunk accumulates the
+// exchange rows of those bras (and their J blocks) in LDS.
+struct BraChunk {
+  int64_t q0;
+  int32_t nq;
+  int32_t S;
+  int32_t bra0;
+  int32_t nbra;
+};
+
+struct BraArgs {
+  const Pair* pairs;
+  const PrimPair* prims;
+  const ShellXYZ* shells;
+  const int2* quartets;
+  const BraChunk* chunks;
+  int nchunks;
+  int nbra_max;         // LDS slots per workgroup
+  const double* dens;   // [nbd][n][n]
+  double* jp;
+  double* kp;
   int n, nbd, nbj, with_j, with_k;
 };
 
@@ -67,6 +103,7 @@ constexpr int kNumPairClasses = 6;
 __host__ __device__ constexpr int pair_class(int la, int lb) { return la * (la + 1) / 2 + lb; }
 
 void launch_jk_class(int la, int lb, int lc, int ld, const JKArgs& a, hipStream_t s);
+void launch_jk_bra_class(int la, int lb, int lc, int ld, const BraArgs& a, size_t lds_bytes, hipStream_t s);
 void launch_blocks_class(int la, int lb, int lc, int ld, const BlockArgs& a, hipStream_t s);
 
 }  // namespace gscf

@@ -9,6 +9,9 @@ namespace gscf {
 void GSCF_NAME(launch_jk_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const JKArgs& a, hipStream_t s) {
   launch_jk_t<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, s);
 }
+void GSCF_NAME(launch_jk_bra_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const BraArgs& a, size_t lds, hipStream_t s) {
+  launch_jk_bra_t<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, lds, s);
+}
 void GSCF_NAME(launch_blocks_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const BlockArgs& a, hipStream_t s) {
   launch_blocks_t<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, s);
 }

@@ -107,10 +107,20 @@ struct Seg {
 
 struct QuartetClass {
   int la, lb, lc, ld;
-  int bra0, ket0, nbra, same_class;
-  int64_t nq;
-  int64_t* d_qoff;
+  int64_t nq = 0, nitems = 0;
+  std::vector<SubTask> tasks;
+  SubTask* d_tasks = nullptr;
+  int2* d_quartets = nullptr;
+  // bra-major chunks for the LDS-accumulating kernel (built for one density)
+  std::vector<BraChunk> chunks;
+  BraChunk* d_chunks = nullptr;
+  int nbra_max = 0;
 };
+
+// LDS budget of the bra-major kernel per workgroup.
+constexpr size_t kLdsBudget = 64 * 1024;
+constexpr int64_t kChunkItems = 4096;   // lane-items per workgroup before a chunk is closed
+constexpr int kChunkBrasMax = 16;
 
 struct PlanData {
   int cart = 1, n = 0, n_cart = 0, nbas = 0, natm = 0;
@@ -174,6 +184,21 @@ GSCF_CLASSES(X)
 void launch_int1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
                         const ShellXYZ* shells, const double* nuclei, int nnuc, int nc, double* s, double* t,
                         double* v, hipStream_t st);
+
+#define X(A, B, C, D) void launch_jk_bra_c##A##B##C##D(const BraArgs& a, size_t lds, hipStream_t s);
+GSCF_CLASSES(X)
+#undef X
+
+void launch_jk_bra_class(int la, int lb, int lc, int ld, const BraArgs& a, size_t lds, hipStream_t s) {
+#define X(A, B, C, D)                                      \
+  if (la == A && lb == B && lc == C && ld == D) {          \
+    launch_jk_bra_c##A##B##C##D(a, lds, s);                \
+    return;                                                \
+  }
+  GSCF_CLASSES(X)
+#undef X
+  throw Internal("no kernel for this angular-momentum class");
+}
 
 void launch_jk_class(int la, int lb, int lc, int ld, const JKArgs& a, hipStream_t s) {
 #define X(A, B, C, D)                                      \
@@ -292,6 +317,19 @@ __global__ void sandwich_kernel(const double* __restrict__ M, int nc, int n, con
     }
   }
   out[i] = s;
+}
+
+// Quartet list of one sub-task: g -> (bra0 + b, ket0 + g - qoff[b]) with qoff[b] <= g < qoff[b+1].
+__global__ void expand_kernel(const int64_t* __restrict__ qoff, int nbra, int bra0, int ket0, int64_t nq,
+                              int2* __restrict__ out) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nq) return;
+  int lo = 0, hi = nbra;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (qoff[mid] <= g) lo = mid; else hi = mid;
+  }
+  out[g] = make_int2(bra0 + lo, ket0 + (int)(g - qoff[lo]));
 }
 
 static unsigned nblocks(size_t count) { return (unsigned)((count + 255) / 256); }
@@ -593,7 +631,14 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
   }
   double qmax = 0.0;
   for (double v : q) qmax = std::max(qmax, v);
-  // retained pairs, grouped by class, Schwarz-descending inside a class
+  // retained pairs, grouped by class and by primitive-pair count (power-of-two
+  // buckets, so lanes of one wavefront run similar primitive loops), Schwarz-
+  // descending inside a (class, bucket) list
+  auto bucket = [](int np) {
+    int b = 0;
+    while ((2 << b) <= np) ++b;
+    return b;
+  };
   std::vector<int> keep;
   for (int i = 0; i < npair; ++i) {
     if (P.all_pairs[i].np == 0) continue;
@@ -602,60 +647,143 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
   }
   std::stable_sort(keep.begin(), keep.end(), [&](int x, int y) {
     if (P.all_class[x] != P.all_class[y]) return P.all_class[x] < P.all_class[y];
+    const int bx = bucket(P.all_pairs[x].np), by = bucket(P.all_pairs[y].np);
+    if (bx != by) return bx > by;
     return q[x] > q[y];
   });
-  for (int c = 0; c <= kNumPairClasses; ++c) P.class_begin[c] = 0;
   for (int i : keep) {
     P.pairs.push_back(P.all_pairs[i]);
     P.pair_q.push_back(q[i]);
   }
-  {
-    int c = 0, k = 0;
-    for (c = 0; c < kNumPairClasses; ++c) {
-      P.class_begin[c] = k;
-      while (k < (int)keep.size() && P.all_class[keep[k]] == c) ++k;
+  struct Run { int cls, bkt, beg, end, npmin, npmax; };
+  std::vector<Run> runs;
+  for (int k = 0; k < (int)keep.size();) {
+    Run r{P.all_class[keep[k]], bucket(P.all_pairs[keep[k]].np), k, k, 1 << 30, 0};
+    while (k < (int)keep.size() && P.all_class[keep[k]] == r.cls && bucket(P.all_pairs[keep[k]].np) == r.bkt) {
+      r.npmin = std::min(r.npmin, P.all_pairs[keep[k]].np);
+      r.npmax = std::max(r.npmax, P.all_pairs[keep[k]].np);
+      ++k;
     }
-    P.class_begin[kNumPairClasses] = k;
+    r.end = k;
+    runs.push_back(r);
   }
+  for (int c = 0; c <= kNumPairClasses; ++c) P.class_begin[c] = (int)keep.size();
+  for (int i = (int)runs.size() - 1; i >= 0; --i) P.class_begin[runs[i].cls] = runs[i].beg;
+  for (int c = kNumPairClasses - 1; c >= 0; --c) P.class_begin[c] = std::min(P.class_begin[c], P.class_begin[c + 1]);
   P.d_pairs = dev_upload(P.pairs, s);
-  // quartet classes
+  // quartet classes: canonical (bra run, ket run) sub-tasks with materialised quartet lists
   static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+  constexpr int kTargetPrim = 48;  // primitive quartets per work item
   P.nquartets = 0;
   for (int cb = 0; cb < kNumPairClasses; ++cb)
     for (int ck = 0; ck <= cb; ++ck) {
-      const int b0 = P.class_begin[cb], nbra = P.class_begin[cb + 1] - b0;
-      const int k0 = P.class_begin[ck], nket = P.class_begin[ck + 1] - k0;
-      if (nbra == 0 || nket == 0) continue;
-      std::vector<int64_t> qoff(nbra + 1, 0);
-      for (int b = 0; b < nbra; ++b) {
-        int cnt = nket;
-        if (cutoff > 0) {
-          // kets are Schwarz-descending: count those with Q_b Q_k >= cutoff
-          const double qb = P.pair_q[b0 + b];
-          int lo = 0, hi = nket;
-          while (lo < hi) {
-            const int mid = (lo + hi) / 2;
-            if (qb * P.pair_q[k0 + mid] < cutoff) hi = mid; else lo = mid + 1;
-          }
-          cnt = lo;
-        }
-        if (cb == ck) cnt = std::min(cnt, b + 1);
-        qoff[b + 1] = qoff[b] + cnt;
-      }
-      if (qoff[nbra] == 0) continue;
       QuartetClass qc;
-      qc.la = pl[cb][0];
-      qc.lb = pl[cb][1];
-      qc.lc = pl[ck][0];
-      qc.ld = pl[ck][1];
-      qc.bra0 = b0;
-      qc.ket0 = k0;
-      qc.nbra = nbra;
-      qc.same_class = cb == ck;
-      qc.nq = qoff[nbra];
-      qc.d_qoff = dev_upload(qoff, s);
+      qc.la = pl[cb][0]; qc.lb = pl[cb][1]; qc.lc = pl[ck][0]; qc.ld = pl[ck][1];
+      qc.nq = 0;
+      qc.nitems = 0;
+      struct Pending { std::vector<int64_t> qoff; int bra0, ket0, nbra; int64_t q0, nq; };
+      std::vector<Pending> pend;
+      for (const Run& rb : runs) {
+        if (rb.cls != cb) continue;
+        for (const Run& rk : runs) {
+          if (rk.cls != ck) continue;
+          const bool same_list = cb == ck && rb.bkt == rk.bkt;
+          if (cb == ck && rb.bkt < rk.bkt) continue;  // canonical: each unordered pair-pair once
+          const int nbra = rb.end - rb.beg, nket = rk.end - rk.beg;
+          std::vector<int64_t> qoff(nbra + 1, 0);
+          for (int b = 0; b < nbra; ++b) {
+            int cnt = nket;
+            if (cutoff > 0) {
+              const double qb = P.pair_q[rb.beg + b];
+              int lo = 0, hi = nket;
+              while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (qb * P.pair_q[rk.beg + mid] < cutoff) hi = mid; else lo = mid + 1;
+              }
+              cnt = lo;
+            }
+            if (same_list) cnt = std::min(cnt, b + 1);
+            qoff[b + 1] = qoff[b] + cnt;
+          }
+          const int64_t nq = qoff[nbra];
+          if (nq == 0) continue;
+          SubTask t;
+          t.q0 = qc.nq;
+          t.nq = nq;
+          const int64_t work = (int64_t)rb.npmax * rk.npmax;
+          t.S = (int)std::max<int64_t>(1, std::min<int64_t>(rb.npmin, (work + kTargetPrim - 1) / kTargetPrim));
+          t.pad = 0;
+          t.item0 = qc.nitems;
+          qc.nitems += (int64_t)t.S * nq;
+          qc.nq += nq;
+          qc.tasks.push_back(t);
+          pend.push_back({std::move(qoff), rb.beg, rk.beg, nbra, t.q0, nq});
+        }
+      }
+      if (qc.nq == 0) continue;
+      {
+        // bra-major chunks: consecutive bras of one sub-task, bounded LDS slots and lane-items
+        const int ncomp_a = (qc.la + 1) * (qc.la + 2) / 2, ncomp_b = (qc.lb + 1) * (qc.lb + 2) / 2;
+        const int ncomp_c = (qc.lc + 1) * (qc.lc + 2) / 2, ncomp_d = (qc.ld + 1) * (qc.ld + 2) / 2;
+        const int rows = ncomp_a + ncomp_b;
+        const int split = (ncomp_a * ncomp_b * ncomp_c * ncomp_d > 36) ? ncomp_c * ncomp_d : 1;
+        const size_t per_slot = sizeof(double) * ((size_t)rows * P.n_cart + (size_t)ncomp_a * ncomp_b);
+        qc.nbra_max = (int)std::min<size_t>(kChunkBrasMax, kLdsBudget / per_slot);
+        for (size_t ti = 0; qc.nbra_max >= 1 && ti < pend.size(); ++ti) {
+          const Pending& pd = pend[ti];
+          const SubTask& t = qc.tasks[ti];
+          const int64_t per_q = (int64_t)t.S * split;
+          int b = 0;
+          while (b < pd.nbra) {
+            const int64_t cnt = pd.qoff[b + 1] - pd.qoff[b];
+            if (cnt == 0) { ++b; continue; }
+            if (cnt * per_q > kChunkItems) {
+              // one long bra: several single-bra chunks
+              const int64_t step = std::max<int64_t>(1, kChunkItems / per_q);
+              for (int64_t k0 = 0; k0 < cnt; k0 += step) {
+                BraChunk c;
+                c.q0 = t.q0 + pd.qoff[b] + k0;
+                c.nq = (int32_t)std::min<int64_t>(step, cnt - k0);
+                c.S = t.S;
+                c.bra0 = pd.bra0 + b;
+                c.nbra = 1;
+                qc.chunks.push_back(c);
+              }
+              ++b;
+              continue;
+            }
+            BraChunk c;
+            c.q0 = t.q0 + pd.qoff[b];
+            c.S = t.S;
+            c.bra0 = pd.bra0 + b;
+            int64_t items = 0;
+            int e = b;
+            while (e < pd.nbra && e - b < qc.nbra_max) {
+              const int64_t ce = pd.qoff[e + 1] - pd.qoff[e];
+              if (items > 0 && (items + ce * per_q > kChunkItems || ce * per_q > kChunkItems)) break;
+              items += ce * per_q;
+              ++e;
+            }
+            c.nbra = e - b;
+            c.nq = (int32_t)(pd.qoff[e] - pd.qoff[b]);
+            qc.chunks.push_back(c);
+            b = e;
+          }
+        }
+        if (qc.nbra_max >= 1) qc.d_chunks = dev_upload(qc.chunks, s);
+      }
+      HIP_OK(hipMalloc(&qc.d_quartets, sizeof(int2) * qc.nq));
+      for (const Pending& pd : pend) {
+        int64_t* d_qoff = dev_upload(pd.qoff, s);
+        hipLaunchKernelGGL(expand_kernel, dim3(nblocks((size_t)pd.nq)), dim3(256), 0, s, d_qoff, pd.nbra, pd.bra0,
+                           pd.ket0, pd.nq, qc.d_quartets + pd.q0);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(s));
+        HIP_OK(hipFree(d_qoff));
+      }
+      qc.d_tasks = dev_upload(qc.tasks, s);
       P.nquartets += qc.nq;
-      P.classes.push_back(qc);
+      P.classes.push_back(std::move(qc));
     }
   build_c2s_maps(P, s);
   {
@@ -694,7 +822,11 @@ static void free_plan(PlanData& P) {
   f(P.d_c2s_idx); f(P.d_c2s_coef); f(P.d_s2c_idx); f(P.d_s2c_coef);
   f(P.d_dc); f(P.d_jp); f(P.d_kp); f(P.d_flag);
   f(P.d_pairs1e); f(P.d_nuclei);
-  for (auto& c : P.classes) f(c.d_qoff);
+  for (auto& c : P.classes) {
+    f(c.d_quartets);
+    f(c.d_tasks);
+    f(c.d_chunks);
+  }
   for (hipEvent_t e : P.ev_pool) (void)hipEventDestroy(e);
   if (P.h_flag) (void)hipHostFree(P.h_flag);
 }
@@ -748,25 +880,47 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       P.prof_recs.push_back({(int)ci, P.ev_used});
       HIP_OK(hipEventRecord(P.ev_pool[P.ev_used], s));
     }
-    JKArgs a;
-    a.pairs = P.d_pairs;
-    a.prims = P.d_prims;
-    a.shells = P.d_shells;
-    a.qoff = qc.d_qoff;
-    a.bra0 = qc.bra0;
-    a.ket0 = qc.ket0;
-    a.nbra = qc.nbra;
-    a.same_class = qc.same_class;
-    a.nq = qc.nq;
-    a.dens = P.d_dc;
-    a.jp = P.d_jp;
-    a.kp = P.d_kp;
-    a.n = nc;
-    a.nbd = with_k ? nbd : nb;
-    a.nbj = nb;
-    a.with_j = with_j;
-    a.with_k = with_k;
-    launch_jk_class(qc.la, qc.lb, qc.lc, qc.ld, a, s);
+    const int nbd_eff = with_k ? nbd : nb;
+    const int ncomp_a = (qc.la + 1) * (qc.la + 2) / 2, ncomp_b = (qc.lb + 1) * (qc.lb + 2) / 2;
+    const size_t lds = sizeof(double) * ((size_t)nbd_eff * qc.nbra_max * (ncomp_a + ncomp_b) * nc +
+                                         (size_t)nb * qc.nbra_max * ncomp_a * ncomp_b);
+    if (qc.d_chunks && lds <= kLdsBudget) {
+      BraArgs b;
+      b.pairs = P.d_pairs;
+      b.prims = P.d_prims;
+      b.shells = P.d_shells;
+      b.quartets = qc.d_quartets;
+      b.chunks = qc.d_chunks;
+      b.nchunks = (int)qc.chunks.size();
+      b.nbra_max = qc.nbra_max;
+      b.dens = P.d_dc;
+      b.jp = P.d_jp;
+      b.kp = P.d_kp;
+      b.n = nc;
+      b.nbd = nbd_eff;
+      b.nbj = nb;
+      b.with_j = with_j;
+      b.with_k = with_k;
+      launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, b, lds, s);
+    } else {
+      JKArgs a;
+      a.pairs = P.d_pairs;
+      a.prims = P.d_prims;
+      a.shells = P.d_shells;
+      a.quartets = qc.d_quartets;
+      a.tasks = qc.d_tasks;
+      a.ntasks = (int)qc.tasks.size();
+      a.nitems = qc.nitems;
+      a.dens = P.d_dc;
+      a.jp = P.d_jp;
+      a.kp = P.d_kp;
+      a.n = nc;
+      a.nbd = nbd_eff;
+      a.nbj = nb;
+      a.with_j = with_j;
+      a.with_k = with_k;
+      launch_jk_class(qc.la, qc.lb, qc.lc, qc.ld, a, s);
+    }
     if (P.prof) {
       HIP_OK(hipEventRecord(P.ev_pool[P.ev_used + 1], s));
       P.ev_used += 2;

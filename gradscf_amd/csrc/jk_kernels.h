@@ -20,13 +20,18 @@
 
 namespace gscf {
 
-__device__ __forceinline__ int find_bra(const int64_t* __restrict__ qoff, int nbra, int64_t g) {
-  int lo = 0, hi = nbra;  // qoff[lo] <= g < qoff[hi]
+// Work item -> (quartet index, primitive split index s, split count S).
+__device__ __forceinline__ void map_item(const JKArgs& a, int64_t item, int64_t& g, int& s, int& S) {
+  int lo = 0, hi = a.ntasks;  // tasks[lo].item0 <= item < tasks[hi].item0
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
-    if (qoff[mid] <= g) lo = mid; else hi = mid;
+    if (a.tasks[mid].item0 <= item) lo = mid; else hi = mid;
   }
-  return lo;
+  const SubTask t = a.tasks[lo];
+  const int64_t r = item - t.item0;
+  s = (int)(r / t.nq);
+  g = t.q0 + (r - (int64_t)s * t.nq);
+  S = t.S;
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -200,12 +205,14 @@ __device__ __forceinline__ void load_geom(const ShellXYZ& a, const ShellXYZ& b, 
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
   using K = Class<LA, LB, LC, LD>;
-  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= a.nq) return;
-  const int b = find_bra(a.qoff, a.nbra, g);
-  const int k = (int)(g - a.qoff[b]);
-  const Pair bp = a.pairs[a.bra0 + b];
-  const Pair kp = a.pairs[a.ket0 + k];
+  const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (item >= a.nitems) return;
+  int64_t g;
+  int s0, S;
+  map_item(a, item, g, s0, S);
+  const int2 qk = a.quartets[g];
+  const Pair bp = a.pairs[qk.x];
+  const Pair kp = a.pairs[qk.y];
   const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
   QuartetGeom geo;
   double AB[3], CD[3];
@@ -219,7 +226,7 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
       for (int ic = 0; ic < K::NC; ++ic)
 #pragma unroll
         for (int id = 0; id < K::ND; ++id) acc[ia][ib][ic][id] = 0.0;
-  for (int ip = 0; ip < bp.np; ++ip) {
+  for (int ip = s0; ip < bp.np; ip += S) {
     const PrimPair P = a.prims[bp.poff + ip];
     for (int kq = 0; kq < kp.np; ++kq) {
       const PrimPair Q = a.prims[kp.poff + kq];
@@ -231,7 +238,7 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
   double f = 1.0;
   if (bp.sa == bp.sb) f *= 0.5;
   if (kp.sa == kp.sb) f *= 0.5;
-  if (a.same_class && b == k) f *= 0.5;
+  if (qk.x == qk.y) f *= 0.5;
   const int n = a.n;
   const int oa = sA.ao, ob = sB.ao, oc = sC.ao, od = sD.ao;
   for (int bt = 0; bt < a.nbd; ++bt) {
@@ -341,14 +348,16 @@ template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
   using K = Class<LA, LB, LC, LD>;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t g = t / K::SPLIT;
-  if (g >= a.nq) return;
-  const int kc = (int)(t - g * K::SPLIT);
+  const int64_t item = t / K::SPLIT;
+  if (item >= a.nitems) return;
+  const int kc = (int)(t - item * K::SPLIT);
   const int ic = kc / K::ND, id = kc - (kc / K::ND) * K::ND;
-  const int b = find_bra(a.qoff, a.nbra, g);
-  const int k = (int)(g - a.qoff[b]);
-  const Pair bp = a.pairs[a.bra0 + b];
-  const Pair kp = a.pairs[a.ket0 + k];
+  int64_t g;
+  int s0, S;
+  map_item(a, item, g, s0, S);
+  const int2 qk = a.quartets[g];
+  const Pair bp = a.pairs[qk.x];
+  const Pair kp = a.pairs[qk.y];
   const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
   QuartetGeom geo;
   double AB[3], CD[3];
@@ -364,7 +373,7 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
   for (int ia = 0; ia < K::NA; ++ia)
 #pragma unroll
     for (int ib = 0; ib < K::NB; ++ib) acc[ia][ib] = 0.0;
-  for (int ip = 0; ip < bp.np; ++ip) {
+  for (int ip = s0; ip < bp.np; ip += S) {
     const PrimPair P = a.prims[bp.poff + ip];
     for (int kq = 0; kq < kp.np; ++kq) {
       const PrimPair Q = a.prims[kp.poff + kq];
@@ -376,7 +385,7 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
   double f = 1.0;
   if (bp.sa == bp.sb) f *= 0.5;
   if (kp.sa == kp.sb) f *= 0.5;
-  if (a.same_class && b == k) f *= 0.5;
+  if (qk.x == qk.y) f *= 0.5;
   const int n = a.n;
   const int oa = sA.ao, ob = sB.ao;
   const int c = sC.ao + ic, d = sD.ao + id;
@@ -436,6 +445,275 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
 }
 
 // ----------------------------------------------------------------------------
+// Bra-major J/K kernel: one workgroup per chunk of quartets sharing up to
+// nbra_max consecutive bra pairs.  All exchange terms of the 8-fold sum land in
+// rows of the bra shells (Kp[a,c], Kp[a,d], Kp[b,c], Kp[b,d]), so they are
+// accumulated with LDS atomics in per-bra row slabs [rows of A, rows of B] x n
+// and flushed to global memory once per workgroup; the bra-side J block is
+// accumulated in LDS the same way.  Only the ket-side J terms go straight to
+// global atomics.  Global FP64 atomics are executed at the memory side on this
+// platform, so their count, not FLOPs, is what this layout minimises.
+// ----------------------------------------------------------------------------
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
+  using K = Class<LA, LB, LC, LD>;
+  constexpr int ROWS = K::NA + K::NB;
+  extern __shared__ double lds[];
+  const int n = a.n;
+  const BraChunk ch = a.chunks[blockIdx.x];
+  const int krow_size = a.nbd * a.nbra_max * ROWS * n;
+  double* krow = lds;                      // [nbd][slot][ROWS][n]
+  double* jrow = lds + krow_size;          // [nbj][slot][NA][NB]
+  const int jrow_size = a.nbj * a.nbra_max * K::NBRA;
+  for (int i = threadIdx.x; i < krow_size + jrow_size; i += blockDim.x) lds[i] = 0.0;
+  __syncthreads();
+  const int64_t total = (int64_t)ch.nq * ch.S * K::SPLIT;
+  for (int64_t it = threadIdx.x; it < total; it += blockDim.x) {
+    const int kc = (int)(it % K::SPLIT);
+    const int64_t rest = it / K::SPLIT;
+    const int s0 = (int)(rest / ch.nq);
+    const int64_t g = ch.q0 + (rest - (int64_t)s0 * ch.nq);
+    const int S = ch.S;
+    const int2 qk = a.quartets[g];
+    const int slot = qk.x - ch.bra0;
+    const Pair bp = a.pairs[qk.x];
+    const Pair kp = a.pairs[qk.y];
+    const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
+    QuartetGeom geo;
+    double AB[3], CD[3];
+    load_geom(sA, sB, sC, sD, geo, AB, CD);
+    double f = 1.0;
+    if (bp.sa == bp.sb) f *= 0.5;
+    if (kp.sa == kp.sb) f *= 0.5;
+    if (qk.x == qk.y) f *= 0.5;
+    const int oa = sA.ao, ob = sB.ao;
+    if constexpr (!K::kSplit) {
+      double acc[K::NA][K::NB][K::NC][K::ND];
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+          for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id) acc[ia][ib][ic][id] = 0.0;
+      for (int ip = s0; ip < bp.np; ip += S) {
+        const PrimPair P = a.prims[bp.poff + ip];
+        for (int kq = 0; kq < kp.np; ++kq) {
+          const PrimPair Q = a.prims[kp.poff + kq];
+          PrimQ<K::NR> q;
+          prim_setup<K::NR>(P, Q, geo, q);
+          accumulate_full<LA, LB, LC, LD>(q, AB, CD, acc);
+        }
+      }
+      const int oc = sC.ao, od = sD.ao;
+      for (int bt = 0; bt < a.nbd; ++bt) {
+        const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+        if (a.with_j && bt < a.nbj) {
+          double* J = a.jp + (size_t)bt * n * n;
+          double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
+          double dcd[K::NC][K::ND], dab[K::NA][K::NB];
+#pragma unroll
+          for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id)
+              dcd[ic][id] = Dm[(size_t)(oc + ic) * n + od + id] + Dm[(size_t)(od + id) * n + oc + ic];
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < K::NB; ++ib)
+              dab[ia][ib] = Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia];
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < K::NB; ++ib) {
+              double sj = 0.0;
+#pragma unroll
+              for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < K::ND; ++id) sj += acc[ia][ib][ic][id] * dcd[ic][id];
+              atomicAdd(jr + ia * K::NB + ib, f * sj);
+            }
+#pragma unroll
+          for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id) {
+              double sj = 0.0;
+#pragma unroll
+              for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+                for (int ib = 0; ib < K::NB; ++ib) sj += acc[ia][ib][ic][id] * dab[ia][ib];
+              atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * sj);
+            }
+        }
+        if (a.with_k) {
+          double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * n;
+          double dbd[K::NB][K::ND], dbc[K::NB][K::NC], dad[K::NA][K::ND], dac[K::NA][K::NC];
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib) {
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id) dbd[ib][id] = Dm[(size_t)(ob + ib) * n + od + id];
+#pragma unroll
+            for (int ic = 0; ic < K::NC; ++ic) dbc[ib][ic] = Dm[(size_t)(ob + ib) * n + oc + ic];
+          }
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia) {
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id) dad[ia][id] = Dm[(size_t)(oa + ia) * n + od + id];
+#pragma unroll
+            for (int ic = 0; ic < K::NC; ++ic) dac[ia][ic] = Dm[(size_t)(oa + ia) * n + oc + ic];
+          }
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia) {
+#pragma unroll
+            for (int ic = 0; ic < K::NC; ++ic) {
+              double sk = 0.0;
+#pragma unroll
+              for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+                for (int id = 0; id < K::ND; ++id) sk += acc[ia][ib][ic][id] * dbd[ib][id];
+              atomicAdd(kr + (size_t)ia * n + oc + ic, f * sk);
+            }
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id) {
+              double sk = 0.0;
+#pragma unroll
+              for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < K::NC; ++ic) sk += acc[ia][ib][ic][id] * dbc[ib][ic];
+              atomicAdd(kr + (size_t)ia * n + od + id, f * sk);
+            }
+          }
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib) {
+#pragma unroll
+            for (int ic = 0; ic < K::NC; ++ic) {
+              double sk = 0.0;
+#pragma unroll
+              for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+                for (int id = 0; id < K::ND; ++id) sk += acc[ia][ib][ic][id] * dad[ia][id];
+              atomicAdd(kr + (size_t)(K::NA + ib) * n + oc + ic, f * sk);
+            }
+#pragma unroll
+            for (int id = 0; id < K::ND; ++id) {
+              double sk = 0.0;
+#pragma unroll
+              for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+                for (int ic = 0; ic < K::NC; ++ic) sk += acc[ia][ib][ic][id] * dac[ia][ic];
+              atomicAdd(kr + (size_t)(K::NA + ib) * n + od + id, f * sk);
+            }
+          }
+        }
+      }
+    } else {
+      const int ic = kc / K::ND, id = kc - (kc / K::ND) * K::ND;
+      int ec[3], ed[3];
+      cart_xyz<LC>(ic, ec);
+      cart_xyz<LD>(id, ed);
+      double wk[3][K::MM + 1];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) ket_weights<K::MM>(ec[x], ed[x], CD[x], wk[x]);
+      double acc[K::NA][K::NB];
+#pragma unroll
+      for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib) acc[ia][ib] = 0.0;
+      for (int ip = s0; ip < bp.np; ip += S) {
+        const PrimPair P = a.prims[bp.poff + ip];
+        for (int kq = 0; kq < kp.np; ++kq) {
+          const PrimPair Q = a.prims[kp.poff + kq];
+          PrimQ<K::NR> q;
+          prim_setup<K::NR>(P, Q, geo, q);
+          accumulate_split<LA, LB, LC, LD>(q, AB, wk, acc);
+        }
+      }
+      const int c = sC.ao + ic, d = sD.ao + id;
+      for (int bt = 0; bt < a.nbd; ++bt) {
+        const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+        if (a.with_j && bt < a.nbj) {
+          double* J = a.jp + (size_t)bt * n * n;
+          double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
+          const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
+          double s2 = 0.0;
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < K::NB; ++ib) {
+              atomicAdd(jr + ia * K::NB + ib, f * acc[ia][ib] * dcd);
+              s2 += acc[ia][ib] * (Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia]);
+            }
+          atomicAdd(J + (size_t)c * n + d, f * s2);
+        }
+        if (a.with_k) {
+          double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * n;
+          double db_d[K::NB], db_c[K::NB];
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib) {
+            db_d[ib] = Dm[(size_t)(ob + ib) * n + d];
+            db_c[ib] = Dm[(size_t)(ob + ib) * n + c];
+          }
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia) {
+            double sc = 0.0, sd = 0.0;
+#pragma unroll
+            for (int ib = 0; ib < K::NB; ++ib) {
+              sc += acc[ia][ib] * db_d[ib];
+              sd += acc[ia][ib] * db_c[ib];
+            }
+            atomicAdd(kr + (size_t)ia * n + c, f * sc);
+            atomicAdd(kr + (size_t)ia * n + d, f * sd);
+          }
+          double da_d[K::NA], da_c[K::NA];
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia) {
+            da_d[ia] = Dm[(size_t)(oa + ia) * n + d];
+            da_c[ia] = Dm[(size_t)(oa + ia) * n + c];
+          }
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib) {
+            double sc = 0.0, sd = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < K::NA; ++ia) {
+              sc += acc[ia][ib] * da_d[ia];
+              sd += acc[ia][ib] * da_c[ia];
+            }
+            atomicAdd(kr + (size_t)(K::NA + ib) * n + c, f * sc);
+            atomicAdd(kr + (size_t)(K::NA + ib) * n + d, f * sd);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
+  for (int i = threadIdx.x; i < krow_size; i += blockDim.x) {
+    const double v = krow[i];
+    if (v == 0.0) continue;
+    const int col = i % n;
+    const int r = (i / n) % ROWS;
+    const int slot = (i / (n * ROWS)) % a.nbra_max;
+    const int bt = i / (n * ROWS * a.nbra_max);
+    if (slot >= ch.nbra) continue;
+    const Pair bp = a.pairs[ch.bra0 + slot];
+    const int row = r < K::NA ? a.shells[bp.sa].ao + r : a.shells[bp.sb].ao + (r - K::NA);
+    atomicAdd(a.kp + (size_t)bt * n * n + (size_t)row * n + col, v);
+  }
+  for (int i = threadIdx.x; i < jrow_size; i += blockDim.x) {
+    const double v = jrow[i];
+    if (v == 0.0) continue;
+    const int e = i % K::NBRA;
+    const int slot = (i / K::NBRA) % a.nbra_max;
+    const int bt = i / (K::NBRA * a.nbra_max);
+    if (slot >= ch.nbra) continue;
+    const Pair bp = a.pairs[ch.bra0 + slot];
+    const int row = a.shells[bp.sa].ao + e / K::NB, col = a.shells[bp.sb].ao + e % K::NB;
+    atomicAdd(a.jp + (size_t)bt * n * n + (size_t)row * n + col, v);
+  }
+}
+
+// ----------------------------------------------------------------------------
 // Block kernel (explicit quartet list; one lane per (task, ket component pair))
 // ----------------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD>
@@ -489,8 +767,8 @@ __global__ __launch_bounds__(256) void blocks_kernel(BlockArgs a) {
 template <int LA, int LB, int LC, int LD>
 void launch_jk_t(const JKArgs& a, hipStream_t s) {
   using K = Class<LA, LB, LC, LD>;
-  if (a.nq <= 0) return;
-  const int64_t threads = a.nq * K::SPLIT;
+  if (a.nitems <= 0) return;
+  const int64_t threads = a.nitems * K::SPLIT;
   const int64_t blocks = (threads + 255) / 256;
   if constexpr (K::kSplit)
     hipLaunchKernelGGL((jk_split_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
@@ -505,6 +783,12 @@ void launch_blocks_t(const BlockArgs& a, hipStream_t s) {
   const int64_t threads = a.ntasks * K::NKET;
   const int64_t blocks = (threads + 255) / 256;
   hipLaunchKernelGGL((blocks_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+}
+
+template <int LA, int LB, int LC, int LD>
+void launch_jk_bra_t(const BraArgs& a, size_t lds_bytes, hipStream_t s) {
+  if (a.nchunks <= 0) return;
+  hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD>), dim3((unsigned)a.nchunks), dim3(256), lds_bytes, s, a);
 }
 
 }  // namespace gscf

@@ -57,11 +57,13 @@ def _diagonalize(f, x):
     return e, x @ c
 
 
-def _diis_solve(fock_hist, err_hist, count):
+def _diis_solve(fock_hist, gram, count):
+    """Extrapolated Fock from the ring history; `gram` = E E^T of the error ring,
+    maintained one row per push (a (8 x n^2)(n^2 x 8) product is a poor fit for
+    a GEMM: tiny output, huge reduction)."""
     m = fock_hist.shape[0]
     dev, dt = fock_hist.device, fock_hist.dtype
     valid = (torch.arange(m, device=dev) < count).to(dt)
-    gram = err_hist @ err_hist.T
     scale = torch.max(torch.abs(gram))
     scale = torch.where(scale > 0, scale, torch.ones_like(scale))
     gram = gram / scale
@@ -105,6 +107,7 @@ def rhf(plan: DirectJKPlan, nelectron: int, enuc: float, *, conv_tol=1e-10, conv
     energy = 0.0
     fh = torch.zeros((DIIS_SPACE, n, n), dtype=torch.float64, device=h.device)
     eh = torch.zeros((DIIS_SPACE, n * n), dtype=torch.float64, device=h.device)
+    gram = torch.zeros((DIIS_SPACE, DIIS_SPACE), dtype=torch.float64, device=h.device)
     head = count = 0
     converged, cycles = False, 0
     n_jk = 1
@@ -114,10 +117,13 @@ def rhf(plan: DirectJKPlan, nelectron: int, enuc: float, *, conv_tol=1e-10, conv
             err = c0.T @ (fock @ d @ s - s @ d @ fock) @ c0
             fh[head] = fock
             eh[head] = err.reshape(-1)
+            row = (eh * eh[head]).sum(dim=1)
+            gram[head, :] = row
+            gram[:, head] = row
             head = (head + 1) % DIIS_SPACE
             count = min(count + 1, DIIS_SPACE)
             if count >= 2:
-                fock_eff = _diis_solve(fh, eh, count)
+                fock_eff = _diis_solve(fh, gram, count)
         mo_e, c = _diagonalize(fock_eff, x)
         d_new = (c * occ) @ c.T
         dj, dk = jk(d_new - d)
