@@ -78,6 +78,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-builds", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed-threshold", type=float, default=1e-7)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the J/K engine runs on a single MI355X "
@@ -124,6 +125,23 @@ def main() -> int:
     value = args.steps / elapsed
     log(f"J/K: {ms_per_step:.3f} ms/build, {value:.2f} builds/s")
 
+    # mixed precision: same workload, FP32 for quartets with Q_ij Q_kl max|D| < threshold
+    thr = args.mixed_threshold
+    out_mixed = torch.empty_like(out)
+    for _ in range(args.warmup):
+        plan.jk(dens[None], symmetric=True, validate=False, out=out_mixed, mixed_threshold=thr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.jk(dens[None], symmetric=True, validate=False, out=out_mixed, mixed_threshold=thr)
+    torch.cuda.synchronize()
+    mixed_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    mixed_err = float((out_mixed - out).abs().max())
+    scf_mixed = rhf(plan, spec.nelectron, spec.nuclear_repulsion(), conv_tol=1e-10,
+                    jk_flags={"mixed_threshold": thr})
+    log(f"mixed (thr {thr:g}): {mixed_ms:.3f} ms/build, max|dJK| {mixed_err:.2e}, "
+        f"E {scf_mixed.energy:.10f} (dE {scf_mixed.energy - scf.energy:.2e}), scf {scf_mixed.timings['scf_s']:.3f} s")
+
     # per-class kernel durations (HIP events on the launch stream) over a timed pass
     plan.profile(True)
     for _ in range(args.steps):
@@ -169,6 +187,10 @@ def main() -> int:
         "scf": {"energy": scf.energy, "converged": bool(scf.converged), "cycles": scf.cycles,
                 "wall_s": scf.timings["scf_s"], "integrals_s": scf.timings["integrals_s"],
                 "jk_builds": scf.timings["jk_builds"]},
+        "mixed": {"threshold": thr, "ms_per_step": mixed_ms, "builds_per_s": 1e3 / mixed_ms,
+                  "speedup_vs_fp64": ms_per_step / mixed_ms, "max_abs_jk_diff_vs_fp64": mixed_err,
+                  "scf_energy": scf_mixed.energy, "scf_energy_err_vs_fp64": scf_mixed.energy - scf.energy,
+                  "scf_wall_s": scf_mixed.timings["scf_s"], "scf_converged": bool(scf_mixed.converged)},
         "plan_ms": plan_s * 1e3,
         "ms_per_step_including_plan": ms_per_step + plan_s * 1e3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

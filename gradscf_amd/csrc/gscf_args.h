@@ -14,7 +14,7 @@ struct PrimPair {
   double e;        // c_a c_b K_AB sqrt(2 pi^{5/2}) / p
   double aa;       // alpha_a (first-centre exponent)
   double ab;       // alpha_b
-  double pad;
+  double ip;       // 1 / p
 };
 
 struct Pair {
@@ -78,6 +78,12 @@ struct BraArgs {
   double* jp;
   double* kp;
   int n, nbd, nbj, with_j, with_k;
+  // mixed precision: FP32 for quartets with pair_q[bra] pair_q[ket] dmax6 < thr
+  int mixed;
+  double thr;
+  const double* pair_q;  // Schwarz bound per pair
+  const double* dmax;    // [nseg][nseg] max |D| over shell blocks (all batches)
+  int nseg;
 };
 
 // Explicit quartet list writing contracted Cartesian blocks:
@@ -104,6 +110,24 @@ __host__ __device__ constexpr int pair_class(int la, int lb) { return la * (la +
 
 void launch_jk_class(int la, int lb, int lc, int ld, const JKArgs& a, hipStream_t s);
 void launch_jk_bra_class(int la, int lb, int lc, int ld, const BraArgs& a, size_t lds_bytes, hipStream_t s);
+struct GradArgs {
+  const Pair* pairs;
+  const PrimPair* prims;
+  const ShellXYZ* shells;
+  const int* seg_atom;   // atm row of each segmented shell
+  const int2* quartets;
+  const SubTask* tasks;
+  int ntasks;
+  int64_t nitems;
+  const double* dens;    // Cartesian D, cot_J, cot_K (n x n); cot may be null
+  const double* cj;
+  const double* ck;
+  int n;
+  int natm;
+  double* grad;          // [natm][3]
+};
+
+void launch_grad_class(int la, int lb, int lc, int ld, const GradArgs& a, hipStream_t s);
 void launch_blocks_class(int la, int lb, int lc, int ld, const BlockArgs& a, hipStream_t s);
 
 }  // namespace gscf

@@ -87,24 +87,22 @@ __host__ __device__ constexpr int binom(int n, int k) {
 //   G(n+1,0) = C00 G(n,0) + n B10 G(n-1,0)
 //   G(n,m+1) = C0p G(n,m) + m B01 G(n,m-1) + n B00 G(n-1,m)
 // ---------------------------------------------------------------------------
-template <int NM, int MM>
-__device__ __forceinline__ void vrr(double c00, double c0p, double b10, double b01, double b00,
-                                    double base, double (&G)[NM + 1][MM + 1]) {
+template <int NM, int MM, typename R = double>
+__device__ __forceinline__ void vrr(R c00, R c0p, R b10, R b01, R b00, R base, R (&G)[NM + 1][MM + 1]) {
   G[0][0] = base;
 #pragma unroll
-  for (int n = 0; n < NM; ++n) G[n + 1][0] = c00 * G[n][0] + (n ? n * b10 * G[n - 1][0] : 0.0);
+  for (int n = 0; n < NM; ++n) G[n + 1][0] = c00 * G[n][0] + (n ? R(n) * b10 * G[n - 1][0] : R(0));
 #pragma unroll
   for (int m = 0; m < MM; ++m)
 #pragma unroll
     for (int n = 0; n <= NM; ++n)
-      G[n][m + 1] = c0p * G[n][m] + (m ? m * b01 * G[n][m - 1] : 0.0) + (n ? n * b00 * G[n - 1][m] : 0.0);
+      G[n][m + 1] = c0p * G[n][m] + (m ? R(m) * b01 * G[n][m - 1] : R(0)) + (n ? R(n) * b00 * G[n - 1][m] : R(0));
 }
 
 // Horizontal transfer (x - B) = (x - A) + (A - B) on the bra index of a 1D vector.
-template <int LA, int LB>
-__device__ __forceinline__ void hrr_bra1(const double (&g)[LA + LB + 1], double ab,
-                                        double (&h)[LA + 1][LB + 1]) {
-  double t[LB + 1][LA + LB + 1];
+template <int LA, int LB, typename R = double>
+__device__ __forceinline__ void hrr_bra1(const R (&g)[LA + LB + 1], R ab, R (&h)[LA + 1][LB + 1]) {
+  R t[LB + 1][LA + LB + 1];
 #pragma unroll
   for (int n = 0; n <= LA + LB; ++n) t[0][n] = g[n];
 #pragma unroll
@@ -118,17 +116,17 @@ __device__ __forceinline__ void hrr_bra1(const double (&g)[LA + LB + 1], double 
 }
 
 // Full 4-index transfer of one axis: H[a][b][c][d] from G[n][m].
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void hrr_full(const double (&G)[LA + LB + 1][LC + LD + 1], double ab, double cd,
-                                         double (&H)[LA + 1][LB + 1][LC + 1][LD + 1]) {
-  double X[LA + 1][LB + 1][LC + LD + 1];
+template <int LA, int LB, int LC, int LD, typename R = double>
+__device__ __forceinline__ void hrr_full(const R (&G)[LA + LB + 1][LC + LD + 1], R ab, R cd,
+                                         R (&H)[LA + 1][LB + 1][LC + 1][LD + 1]) {
+  R X[LA + 1][LB + 1][LC + LD + 1];
 #pragma unroll
   for (int m = 0; m <= LC + LD; ++m) {
-    double g[LA + LB + 1];
+    R g[LA + LB + 1];
 #pragma unroll
     for (int n = 0; n <= LA + LB; ++n) g[n] = G[n][m];
-    double h[LA + 1][LB + 1];
-    hrr_bra1<LA, LB>(g, ab, h);
+    R h[LA + 1][LB + 1];
+    hrr_bra1<LA, LB, R>(g, ab, h);
 #pragma unroll
     for (int a = 0; a <= LA; ++a)
 #pragma unroll
@@ -138,11 +136,11 @@ __device__ __forceinline__ void hrr_full(const double (&G)[LA + LB + 1][LC + LD 
   for (int a = 0; a <= LA; ++a)
 #pragma unroll
     for (int b = 0; b <= LB; ++b) {
-      double g[LC + LD + 1];
+      R g[LC + LD + 1];
 #pragma unroll
       for (int m = 0; m <= LC + LD; ++m) g[m] = X[a][b][m];
-      double h[LC + 1][LD + 1];
-      hrr_bra1<LC, LD>(g, cd, h);
+      R h[LC + 1][LD + 1];
+      hrr_bra1<LC, LD, R>(g, cd, h);
 #pragma unroll
       for (int c = 0; c <= LC; ++c)
 #pragma unroll

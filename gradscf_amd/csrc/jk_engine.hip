@@ -117,7 +117,9 @@ struct QuartetClass {
   int nbra_max = 0;
 };
 
-// LDS budget of the bra-major kernel per workgroup.
+// LDS of the bra-major kernel per workgroup: slots are packed up to 32 KB so
+// that several workgroups stay resident per CU; a single slot may use 64 KB.
+constexpr size_t kLdsPack = 32 * 1024;
 constexpr size_t kLdsBudget = 64 * 1024;
 constexpr int64_t kChunkItems = 4096;   // lane-items per workgroup before a chunk is closed
 constexpr int kChunkBrasMax = 16;
@@ -162,6 +164,11 @@ struct PlanData {
   int nnuc = 0;
   // parent-shell Schwarz bounds (nbas x nbas, lower triangle; empty when cutoff == 0)
   std::vector<double> parent_q;
+  // mixed precision: per-pair bounds on device, per-build density block maxima
+  double* d_pair_q = nullptr;
+  double* d_dmax = nullptr;
+  int* d_seg_ao = nullptr;   // [nseg][2]: Cartesian AO offset, ncart
+  int* d_seg_atom = nullptr; // atm row of each segmented shell
   // optional per-class kernel timing with HIP events on the launch stream
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -188,6 +195,20 @@ void launch_int1e_class(int la, int lb, const Pair* pairs, int npairs, const Pri
 #define X(A, B, C, D) void launch_jk_bra_c##A##B##C##D(const BraArgs& a, size_t lds, hipStream_t s);
 GSCF_CLASSES(X)
 #undef X
+#define X(A, B, C, D) void launch_grad_c##A##B##C##D(const GradArgs& a, hipStream_t s);
+GSCF_CLASSES(X)
+#undef X
+
+void launch_grad_class(int la, int lb, int lc, int ld, const GradArgs& a, hipStream_t s) {
+#define X(A, B, C, D)                                      \
+  if (la == A && lb == B && lc == C && ld == D) {          \
+    launch_grad_c##A##B##C##D(a, s);                       \
+    return;                                                \
+  }
+  GSCF_CLASSES(X)
+#undef X
+  throw Internal("no kernel for this angular-momentum class");
+}
 
 void launch_jk_bra_class(int la, int lb, int lc, int ld, const BraArgs& a, size_t lds, hipStream_t s) {
 #define X(A, B, C, D)                                      \
@@ -332,6 +353,20 @@ __global__ void expand_kernel(const int64_t* __restrict__ qoff, int nbra, int br
   out[g] = make_int2(bra0 + lo, ket0 + (int)(g - qoff[lo]));
 }
 
+// dmax[S][T] = max over batches and over the Cartesian block (S, T) of |D|.
+__global__ void dmax_kernel(const double* __restrict__ D, int nc, int nbd, const int* __restrict__ seg_ao, int nseg,
+                            double* __restrict__ dmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg * nseg) return;
+  const int S = i / nseg, T = i - (i / nseg) * nseg;
+  const int oS = seg_ao[2 * S], nS = seg_ao[2 * S + 1], oT = seg_ao[2 * T], nT = seg_ao[2 * T + 1];
+  double m = 0.0;
+  for (int b = 0; b < nbd; ++b)
+    for (int p = 0; p < nS; ++p)
+      for (int q = 0; q < nT; ++q) m = fmax(m, fabs(D[(size_t)b * nc * nc + (size_t)(oS + p) * nc + oT + q]));
+  dmax[i] = m;
+}
+
 static unsigned nblocks(size_t count) { return (unsigned)((count + 255) / 256); }
 
 // ----------------------------------------------------------------------------
@@ -432,7 +467,7 @@ static void make_pair(PlanData& P, int sa, int sb, Pair& out) {
       pp.e = A.c[i] * B.c[j] * std::exp(-mu) * s2p / p;
       pp.aa = ai;
       pp.ab = aj;
-      pp.pad = 0;
+      pp.ip = 1.0 / p;
       P.prims.push_back(pp);
       ++out.np;
     }
@@ -671,6 +706,21 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
   for (int i = (int)runs.size() - 1; i >= 0; --i) P.class_begin[runs[i].cls] = runs[i].beg;
   for (int c = kNumPairClasses - 1; c >= 0; --c) P.class_begin[c] = std::min(P.class_begin[c], P.class_begin[c + 1]);
   P.d_pairs = dev_upload(P.pairs, s);
+  {
+    std::vector<double> pq = P.pair_q;
+    if (pq.empty()) pq.push_back(0.0);
+    P.d_pair_q = dev_upload(pq, s);
+    std::vector<int> sa;
+    for (const Seg& g : P.seg) {
+      sa.push_back(g.ao_cart);
+      sa.push_back(ncart_h(g.l));
+    }
+    P.d_seg_ao = dev_upload(sa, s);
+    std::vector<int> at;
+    for (const Seg& g : P.seg) at.push_back(P.bas[g.parent * BAS_SLOTS + ATOM_OF]);
+    P.d_seg_atom = dev_upload(at, s);
+    HIP_OK(hipMalloc(&P.d_dmax, sizeof(double) * P.seg.size() * P.seg.size()));
+  }
   // quartet classes: canonical (bra run, ket run) sub-tasks with materialised quartet lists
   static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
   constexpr int kTargetPrim = 48;  // primitive quartets per work item
@@ -728,7 +778,8 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
         const int rows = ncomp_a + ncomp_b;
         const int split = (ncomp_a * ncomp_b * ncomp_c * ncomp_d > 36) ? ncomp_c * ncomp_d : 1;
         const size_t per_slot = sizeof(double) * ((size_t)rows * P.n_cart + (size_t)ncomp_a * ncomp_b);
-        qc.nbra_max = (int)std::min<size_t>(kChunkBrasMax, kLdsBudget / per_slot);
+        qc.nbra_max = per_slot > kLdsBudget ? 0
+                      : (int)std::max<size_t>(1, std::min<size_t>(kChunkBrasMax, kLdsPack / per_slot));
         for (size_t ti = 0; qc.nbra_max >= 1 && ti < pend.size(); ++ti) {
           const Pending& pd = pend[ti];
           const SubTask& t = qc.tasks[ti];
@@ -822,6 +873,7 @@ static void free_plan(PlanData& P) {
   f(P.d_c2s_idx); f(P.d_c2s_coef); f(P.d_s2c_idx); f(P.d_s2c_coef);
   f(P.d_dc); f(P.d_jp); f(P.d_kp); f(P.d_flag);
   f(P.d_pairs1e); f(P.d_nuclei);
+  f(P.d_pair_q); f(P.d_dmax); f(P.d_seg_ao); f(P.d_seg_atom);
   for (auto& c : P.classes) {
     f(c.d_quartets);
     f(c.d_tasks);
@@ -846,7 +898,10 @@ static void ensure_workspace(PlanData& P, size_t nbd) {
 }
 
 static void jk_build(PlanData& P, const double* density, int nb, double* out, int with_j, int with_k,
-                     uint32_t flags, hipStream_t s) {
+                     uint32_t flags, double mixed_threshold, hipStream_t s) {
+  const bool mixed = (flags & GSCF_FLAG_MIXED_PRECISION) != 0;
+  if (mixed && !(std::isfinite(mixed_threshold) && mixed_threshold >= 0))
+    throw InvalidArgument("mixed-precision threshold must be finite and non-negative");
   if ((with_j != 0 && with_j != 1) || (with_k != 0 && with_k != 1) || !(with_j || with_k))
     throw InvalidArgument("Direct J/K requires at least one selected operator");
   if (nb < 1) throw InvalidArgument("Direct J/K shape mismatch");
@@ -865,6 +920,11 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
   ensure_workspace(P, nbd);
   hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, density, n, nb, P.d_dc, nc,
                      P.d_c2s_idx, P.d_c2s_coef, nbd);
+  if (mixed) {
+    const int ns = (int)P.seg.size();
+    hipLaunchKernelGGL(dmax_kernel, dim3(nblocks((size_t)ns * ns)), dim3(256), 0, s, P.d_dc, nc, nbd, P.d_seg_ao,
+                       ns, P.d_dmax);
+  }
   hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_jp, ncc * nbd, 0.0);
   hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_kp, ncc * nbd, 0.0);
   for (size_t ci = 0; ci < P.classes.size(); ++ci) {
@@ -901,6 +961,11 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       b.nbj = nb;
       b.with_j = with_j;
       b.with_k = with_k;
+      b.mixed = mixed ? 1 : 0;
+      b.thr = mixed_threshold;
+      b.pair_q = P.d_pair_q;
+      b.dmax = P.d_dmax;
+      b.nseg = (int)P.seg.size();
       launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, b, lds, s);
     } else {
       JKArgs a;
@@ -950,6 +1015,40 @@ static void int1e(PlanData& P, double* s_out, double* t_out, double* v_out, hipS
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(st));
   HIP_OK(hipFree(tmp));
+}
+
+static void geometry_vjp(PlanData& P, const double* dens, const double* cj, const double* ck, double* grad,
+                         hipStream_t s) {
+  if (P.cutoff > 0) throw InvalidArgument("Screened direct J/K geometry AD requires cutoff=0");
+  if (!dens || !grad || (!cj && !ck)) throw InvalidArgument("geometry VJP needs a density, a gradient buffer and a cotangent");
+  const int n = P.n, nc = P.n_cart;
+  const size_t ncc = (size_t)nc * nc;
+  ensure_workspace(P, 3);
+  const double* src[3] = {dens, cj, ck};
+  for (int k = 0; k < 3; ++k)
+    if (src[k])
+      hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc)), dim3(256), 0, s, src[k], n, 1, P.d_dc + k * ncc, nc,
+                         P.d_c2s_idx, P.d_c2s_coef, 1);
+  for (const QuartetClass& qc : P.classes) {
+    GradArgs a;
+    a.pairs = P.d_pairs;
+    a.prims = P.d_prims;
+    a.shells = P.d_shells;
+    a.seg_atom = P.d_seg_atom;
+    a.quartets = qc.d_quartets;
+    a.tasks = qc.d_tasks;
+    a.ntasks = (int)qc.tasks.size();
+    a.nitems = qc.nitems;
+    a.dens = P.d_dc;
+    a.cj = cj ? P.d_dc + ncc : nullptr;
+    a.ck = ck ? P.d_dc + 2 * ncc : nullptr;
+    a.n = nc;
+    a.natm = P.natm;
+    a.grad = grad;
+    if ((size_t)3 * P.natm * sizeof(double) > kLdsBudget) throw InvalidArgument("too many atoms for the gradient kernel");
+    launch_grad_class(qc.la, qc.lb, qc.lc, qc.ld, a, s);
+  }
+  HIP_OK(hipGetLastError());
 }
 
 static void eri_blocks(PlanData& P, const int32_t* shls, int64_t nq, const int64_t* offsets, double* out,
@@ -1087,8 +1186,7 @@ int gscf_jk_build(gscf_plan* plan, const double* density, int32_t nb, double* ou
                   uint32_t flags, double mixed_threshold, void* stream) {
   return guarded([&] {
     if (!plan) throw InvalidArgument("NULL plan");
-    (void)mixed_threshold;
-    jk_build(plan->d, density, nb, out, with_j, with_k, flags, (hipStream_t)stream);
+    jk_build(plan->d, density, nb, out, with_j, with_k, flags, mixed_threshold, (hipStream_t)stream);
   });
 }
 
@@ -1104,7 +1202,7 @@ int gscf_direct_jk(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t
       validate_tables(p.d, atm, natm, bas, nbas, env, nenv, cart);
       if (n != p.d.n || nb < 1) throw InvalidArgument("Direct J/K shape mismatch");
       build_plan(p.d, cutoff, (hipStream_t)stream);
-      jk_build(p.d, density, nb, out, with_j, with_k, 0u, (hipStream_t)stream);
+      jk_build(p.d, density, nb, out, with_j, with_k, 0u, 0.0, (hipStream_t)stream);
       HIP_OK(hipStreamSynchronize((hipStream_t)stream));
     } catch (...) {
       free_plan(p.d);
@@ -1177,8 +1275,8 @@ int gscf_int1e(gscf_plan* plan, double* s, double* t, double* v, void* stream) {
 int gscf_jk_geometry_vjp(gscf_plan* plan, const double* density, const double* cot_j, const double* cot_k,
                          double* grad, void* stream) {
   return guarded([&] {
-    (void)plan; (void)density; (void)cot_j; (void)cot_k; (void)grad; (void)stream;
-    throw Internal("gscf_jk_geometry_vjp: not built yet");
+    if (!plan) throw InvalidArgument("NULL plan");
+    geometry_vjp(plan->d, density, cot_j, cot_k, grad, (hipStream_t)stream);
   });
 }
 

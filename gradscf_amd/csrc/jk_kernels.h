@@ -61,8 +61,8 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
                                            PrimQ<NR>& o) {
   const double pq = P.p + Q.p;
   o.ipq = 1.0 / pq;
-  o.ip = 1.0 / P.p;
-  o.iq = 1.0 / Q.p;
+  o.ip = P.ip;
+  o.iq = Q.ip;
   o.PQ[0] = P.px - Q.px;
   o.PQ[1] = P.py - Q.py;
   o.PQ[2] = P.pz - Q.pz;
@@ -81,25 +81,27 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
 }
 
 // Full block of one primitive quartet accumulated into acc[NA][NB][NC][ND].
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void accumulate_full(const PrimQ<Class<LA, LB, LC, LD>::NR>& q, const double AB[3],
-                                                const double CD[3],
-                                                double (&acc)[ncart(LA)][ncart(LB)][ncart(LC)][ncart(LD)]) {
+// The per-root coefficients are formed in FP64 and the recurrences, transfers
+// and accumulation run in R (double, or float on the mixed-precision path).
+template <int LA, int LB, int LC, int LD, typename R = double>
+__device__ __forceinline__ void accumulate_full(const PrimQ<Class<LA, LB, LC, LD>::NR>& q, const R AB[3],
+                                                const R CD[3],
+                                                R (&acc)[ncart(LA)][ncart(LB)][ncart(LC)][ncart(LD)]) {
   using K = Class<LA, LB, LC, LD>;
 #pragma unroll
   for (int r = 0; r < K::NR; ++r) {
     const double u = q.s[r];
-    const double b00 = 0.5 * u * q.ipq;
-    const double b10 = 0.5 * q.ip * (1.0 - q.qf * u);
-    const double b01 = 0.5 * q.iq * (1.0 - q.pf * u);
-    double H[3][LA + 1][LB + 1][LC + 1][LD + 1];
+    const R b00 = (R)(0.5 * u * q.ipq);
+    const R b10 = (R)(0.5 * q.ip * (1.0 - q.qf * u));
+    const R b01 = (R)(0.5 * q.iq * (1.0 - q.pf * u));
+    R H[3][LA + 1][LB + 1][LC + 1][LD + 1];
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      double G[K::NM + 1][K::MM + 1];
-      const double c00 = q.PA[x] - q.qf * q.PQ[x] * u;
-      const double c0p = q.QC[x] + q.pf * q.PQ[x] * u;
-      vrr<K::NM, K::MM>(c00, c0p, b10, b01, b00, x == 2 ? q.w[r] * q.pref : 1.0, G);
-      hrr_full<LA, LB, LC, LD>(G, AB[x], CD[x], H[x]);
+      R G[K::NM + 1][K::MM + 1];
+      const R c00 = (R)(q.PA[x] - q.qf * q.PQ[x] * u);
+      const R c0p = (R)(q.QC[x] + q.pf * q.PQ[x] * u);
+      vrr<K::NM, K::MM, R>(c00, c0p, b10, b01, b00, (R)(x == 2 ? q.w[r] * q.pref : 1.0), G);
+      hrr_full<LA, LB, LC, LD, R>(G, AB[x], CD[x], H[x]);
     }
 #pragma unroll
     for (int ia = 0; ia < K::NA; ++ia)
@@ -109,9 +111,9 @@ __device__ __forceinline__ void accumulate_full(const PrimQ<Class<LA, LB, LC, LD
         for (int ic = 0; ic < K::NC; ++ic)
 #pragma unroll
           for (int id = 0; id < K::ND; ++id) {
-            const double vx = H[0][cart_comp(LA, ia, 0)][cart_comp(LB, ib, 0)][cart_comp(LC, ic, 0)][cart_comp(LD, id, 0)];
-            const double vy = H[1][cart_comp(LA, ia, 1)][cart_comp(LB, ib, 1)][cart_comp(LC, ic, 1)][cart_comp(LD, id, 1)];
-            const double vz = H[2][cart_comp(LA, ia, 2)][cart_comp(LB, ib, 2)][cart_comp(LC, ic, 2)][cart_comp(LD, id, 2)];
+            const R vx = H[0][cart_comp(LA, ia, 0)][cart_comp(LB, ib, 0)][cart_comp(LC, ic, 0)][cart_comp(LD, id, 0)];
+            const R vy = H[1][cart_comp(LA, ia, 1)][cart_comp(LB, ib, 1)][cart_comp(LC, ic, 1)][cart_comp(LD, id, 1)];
+            const R vz = H[2][cart_comp(LA, ia, 2)][cart_comp(LB, ib, 2)][cart_comp(LC, ic, 2)][cart_comp(LD, id, 2)];
             acc[ia][ib][ic][id] += vx * vy * vz;
           }
   }
@@ -132,49 +134,49 @@ __device__ __forceinline__ void cart_xyz(int i, int (&e)[3]) {
 
 // Ket transfer weights for a fixed ket component pair (c, d) on one axis:
 // I(n; c, d) = sum_m wk[m] G(n, m), wk[m] = C(d, m-c) CD^{d-(m-c)}.
-template <int MM>
-__device__ __forceinline__ void ket_weights(int c, int d, double cd, double (&wk)[MM + 1]) {
+template <int MM, typename R = double>
+__device__ __forceinline__ void ket_weights(int c, int d, R cd, R (&wk)[MM + 1]) {
 #pragma unroll
   for (int m = 0; m <= MM; ++m) {
     const int k = m - c;
-    double v = 0.0;
+    R v = R(0);
     if (k >= 0 && k <= d) {
-      double pw = 1.0;
+      R pw = R(1);
       for (int t = 0; t < d - k; ++t) pw *= cd;
-      v = (double)binom(d, k) * pw;
+      v = (R)binom(d, k) * pw;
     }
     wk[m] = v;
   }
 }
 
 // Block slice for one ket component pair accumulated into acc[NA][NB].
-template <int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void accumulate_split(const PrimQ<Class<LA, LB, LC, LD>::NR>& q, const double AB[3],
-                                                 const double (&wk)[3][LC + LD + 1],
-                                                 double (&acc)[ncart(LA)][ncart(LB)]) {
+template <int LA, int LB, int LC, int LD, typename R = double>
+__device__ __forceinline__ void accumulate_split(const PrimQ<Class<LA, LB, LC, LD>::NR>& q, const R AB[3],
+                                                 const R (&wk)[3][LC + LD + 1],
+                                                 R (&acc)[ncart(LA)][ncart(LB)]) {
   using K = Class<LA, LB, LC, LD>;
 #pragma unroll
   for (int r = 0; r < K::NR; ++r) {
     const double u = q.s[r];
-    const double b00 = 0.5 * u * q.ipq;
-    const double b10 = 0.5 * q.ip * (1.0 - q.qf * u);
-    const double b01 = 0.5 * q.iq * (1.0 - q.pf * u);
-    double h[3][LA + 1][LB + 1];
+    const R b00 = (R)(0.5 * u * q.ipq);
+    const R b10 = (R)(0.5 * q.ip * (1.0 - q.qf * u));
+    const R b01 = (R)(0.5 * q.iq * (1.0 - q.pf * u));
+    R h[3][LA + 1][LB + 1];
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
-      double G[K::NM + 1][K::MM + 1];
-      const double c00 = q.PA[x] - q.qf * q.PQ[x] * u;
-      const double c0p = q.QC[x] + q.pf * q.PQ[x] * u;
-      vrr<K::NM, K::MM>(c00, c0p, b10, b01, b00, x == 2 ? q.w[r] * q.pref : 1.0, G);
-      double g1[K::NM + 1];
+      R G[K::NM + 1][K::MM + 1];
+      const R c00 = (R)(q.PA[x] - q.qf * q.PQ[x] * u);
+      const R c0p = (R)(q.QC[x] + q.pf * q.PQ[x] * u);
+      vrr<K::NM, K::MM, R>(c00, c0p, b10, b01, b00, (R)(x == 2 ? q.w[r] * q.pref : 1.0), G);
+      R g1[K::NM + 1];
 #pragma unroll
       for (int n = 0; n <= K::NM; ++n) {
-        double s = 0.0;
+        R s = R(0);
 #pragma unroll
         for (int m = 0; m <= K::MM; ++m) s += wk[x][m] * G[n][m];
         g1[n] = s;
       }
-      hrr_bra1<LA, LB>(g1, AB[x], h[x]);
+      hrr_bra1<LA, LB, R>(g1, AB[x], h[x]);
     }
 #pragma unroll
     for (int ia = 0; ia < K::NA; ++ia)
@@ -454,6 +456,240 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
 // global atomics.  Global FP64 atomics are executed at the memory side on this
 // platform, so their count, not FLOPs, is what this layout minimises.
 // ----------------------------------------------------------------------------
+// One work item of the bra-major kernel in real type R: evaluate the quartet
+// (or its ket-component slice) over bra primitive pairs s0, s0+S, ... and
+// digest it (exchange rows / bra J block into LDS, ket J block to global).
+template <int LA, int LB, int LC, int LD, typename R>
+__device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const Pair& kp, const int2 qk, int slot,
+                                         int s0, int S, int kc, double* __restrict__ krow,
+                                         double* __restrict__ jrow) {
+  using K = Class<LA, LB, LC, LD>;
+  constexpr int ROWS = K::NA + K::NB;
+  const int n = a.n;
+  const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
+  QuartetGeom geo;
+  double ABd[3], CDd[3];
+  load_geom(sA, sB, sC, sD, geo, ABd, CDd);
+  const R AB[3] = {(R)ABd[0], (R)ABd[1], (R)ABd[2]};
+  const R CD[3] = {(R)CDd[0], (R)CDd[1], (R)CDd[2]};
+  double f = 1.0;
+  if (bp.sa == bp.sb) f *= 0.5;
+  if (kp.sa == kp.sb) f *= 0.5;
+  if (qk.x == qk.y) f *= 0.5;
+  const int oa = sA.ao, ob = sB.ao;
+  if constexpr (!K::kSplit) {
+    R acc[K::NA][K::NB][K::NC][K::ND];
+#pragma unroll
+    for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+          for (int id = 0; id < K::ND; ++id) acc[ia][ib][ic][id] = R(0);
+    for (int ip = s0; ip < bp.np; ip += S) {
+      const PrimPair P = a.prims[bp.poff + ip];
+      for (int kq = 0; kq < kp.np; ++kq) {
+        const PrimPair Q = a.prims[kp.poff + kq];
+        PrimQ<K::NR> q;
+        prim_setup<K::NR>(P, Q, geo, q);
+        accumulate_full<LA, LB, LC, LD, R>(q, AB, CD, acc);
+      }
+    }
+    const int oc = sC.ao, od = sD.ao;
+    for (int bt = 0; bt < a.nbd; ++bt) {
+      const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+      if (a.with_j && bt < a.nbj) {
+        double* J = a.jp + (size_t)bt * n * n;
+        double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
+        double dcd[K::NC][K::ND], dab[K::NA][K::NB];
+#pragma unroll
+        for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+          for (int id = 0; id < K::ND; ++id)
+            dcd[ic][id] = Dm[(size_t)(oc + ic) * n + od + id] + Dm[(size_t)(od + id) * n + oc + ic];
+#pragma unroll
+        for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib)
+            dab[ia][ib] = Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia];
+#pragma unroll
+        for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib) {
+            double sj = 0.0;
+#pragma unroll
+            for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+              for (int id = 0; id < K::ND; ++id) sj += (double)acc[ia][ib][ic][id] * dcd[ic][id];
+            atomicAdd(jr + ia * K::NB + ib, f * sj);
+          }
+#pragma unroll
+        for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+          for (int id = 0; id < K::ND; ++id) {
+            double sj = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+              for (int ib = 0; ib < K::NB; ++ib) sj += (double)acc[ia][ib][ic][id] * dab[ia][ib];
+            atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * sj);
+          }
+      }
+      if (a.with_k) {
+        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * n;
+        double dbd[K::NB][K::ND], dbc[K::NB][K::NC], dad[K::NA][K::ND], dac[K::NA][K::NC];
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib) {
+#pragma unroll
+          for (int id = 0; id < K::ND; ++id) dbd[ib][id] = Dm[(size_t)(ob + ib) * n + od + id];
+#pragma unroll
+          for (int ic = 0; ic < K::NC; ++ic) dbc[ib][ic] = Dm[(size_t)(ob + ib) * n + oc + ic];
+        }
+#pragma unroll
+        for (int ia = 0; ia < K::NA; ++ia) {
+#pragma unroll
+          for (int id = 0; id < K::ND; ++id) dad[ia][id] = Dm[(size_t)(oa + ia) * n + od + id];
+#pragma unroll
+          for (int ic = 0; ic < K::NC; ++ic) dac[ia][ic] = Dm[(size_t)(oa + ia) * n + oc + ic];
+        }
+#pragma unroll
+        for (int ia = 0; ia < K::NA; ++ia) {
+#pragma unroll
+          for (int ic = 0; ic < K::NC; ++ic) {
+            double sk = 0.0;
+#pragma unroll
+            for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+              for (int id = 0; id < K::ND; ++id) sk += (double)acc[ia][ib][ic][id] * dbd[ib][id];
+            atomicAdd(kr + (size_t)ia * n + oc + ic, f * sk);
+          }
+#pragma unroll
+          for (int id = 0; id < K::ND; ++id) {
+            double sk = 0.0;
+#pragma unroll
+            for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+              for (int ic = 0; ic < K::NC; ++ic) sk += (double)acc[ia][ib][ic][id] * dbc[ib][ic];
+            atomicAdd(kr + (size_t)ia * n + od + id, f * sk);
+          }
+        }
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib) {
+#pragma unroll
+          for (int ic = 0; ic < K::NC; ++ic) {
+            double sk = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+              for (int id = 0; id < K::ND; ++id) sk += (double)acc[ia][ib][ic][id] * dad[ia][id];
+            atomicAdd(kr + (size_t)(K::NA + ib) * n + oc + ic, f * sk);
+          }
+#pragma unroll
+          for (int id = 0; id < K::ND; ++id) {
+            double sk = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+              for (int ic = 0; ic < K::NC; ++ic) sk += (double)acc[ia][ib][ic][id] * dac[ia][ic];
+            atomicAdd(kr + (size_t)(K::NA + ib) * n + od + id, f * sk);
+          }
+        }
+      }
+    }
+  } else {
+    const int ic = kc / K::ND, id = kc - (kc / K::ND) * K::ND;
+    int ec[3], ed[3];
+    cart_xyz<LC>(ic, ec);
+    cart_xyz<LD>(id, ed);
+    R wk[3][K::MM + 1];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) ket_weights<K::MM, R>(ec[x], ed[x], CD[x], wk[x]);
+    R acc[K::NA][K::NB];
+#pragma unroll
+    for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < K::NB; ++ib) acc[ia][ib] = R(0);
+    for (int ip = s0; ip < bp.np; ip += S) {
+      const PrimPair P = a.prims[bp.poff + ip];
+      for (int kq = 0; kq < kp.np; ++kq) {
+        const PrimPair Q = a.prims[kp.poff + kq];
+        PrimQ<K::NR> q;
+        prim_setup<K::NR>(P, Q, geo, q);
+        accumulate_split<LA, LB, LC, LD, R>(q, AB, wk, acc);
+      }
+    }
+    const int c = sC.ao + ic, d = sD.ao + id;
+    for (int bt = 0; bt < a.nbd; ++bt) {
+      const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+      if (a.with_j && bt < a.nbj) {
+        double* J = a.jp + (size_t)bt * n * n;
+        double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
+        const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
+        double s2 = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib) {
+            atomicAdd(jr + ia * K::NB + ib, f * (double)acc[ia][ib] * dcd);
+            s2 += (double)acc[ia][ib] * (Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia]);
+          }
+        atomicAdd(J + (size_t)c * n + d, f * s2);
+      }
+      if (a.with_k) {
+        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * n;
+        double db_d[K::NB], db_c[K::NB];
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib) {
+          db_d[ib] = Dm[(size_t)(ob + ib) * n + d];
+          db_c[ib] = Dm[(size_t)(ob + ib) * n + c];
+        }
+#pragma unroll
+        for (int ia = 0; ia < K::NA; ++ia) {
+          double sc = 0.0, sd = 0.0;
+#pragma unroll
+          for (int ib = 0; ib < K::NB; ++ib) {
+            sc += (double)acc[ia][ib] * db_d[ib];
+            sd += (double)acc[ia][ib] * db_c[ib];
+          }
+          atomicAdd(kr + (size_t)ia * n + c, f * sc);
+          atomicAdd(kr + (size_t)ia * n + d, f * sd);
+        }
+        double da_d[K::NA], da_c[K::NA];
+#pragma unroll
+        for (int ia = 0; ia < K::NA; ++ia) {
+          da_d[ia] = Dm[(size_t)(oa + ia) * n + d];
+          da_c[ia] = Dm[(size_t)(oa + ia) * n + c];
+        }
+#pragma unroll
+        for (int ib = 0; ib < K::NB; ++ib) {
+          double sc = 0.0, sd = 0.0;
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia) {
+            sc += (double)acc[ia][ib] * da_d[ia];
+            sd += (double)acc[ia][ib] * da_c[ia];
+          }
+          atomicAdd(kr + (size_t)(K::NA + ib) * n + c, f * sc);
+          atomicAdd(kr + (size_t)(K::NA + ib) * n + d, f * sd);
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Bra-major J/K kernel: one workgroup per chunk of quartets sharing up to
+// nbra_max consecutive bra pairs.  All exchange terms of the 8-fold sum land in
+// rows of the bra shells (Kp[a,c], Kp[a,d], Kp[b,c], Kp[b,d]), so they are
+// accumulated with LDS atomics in per-bra row slabs [rows of A, rows of B] x n
+// and flushed to global memory once per workgroup; the bra-side J block is
+// accumulated in LDS the same way.  Only the ket-side J terms go straight to
+// global atomics.  Global FP64 atomics are executed at the memory side on this
+// platform, so their count, not FLOPs, is what this layout minimises.
+// With a.mixed set, a quartet whose Q_ij Q_kl max|D| (max over the six density
+// blocks it touches) is below a.thr is evaluated in FP32 (roots, weights and
+// prefactors stay FP64, digestion accumulates in FP64).
+// ----------------------------------------------------------------------------
 template <int LA, int LB, int LC, int LD>
 __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
   using K = Class<LA, LB, LC, LD>;
@@ -473,218 +709,23 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
     const int64_t rest = it / K::SPLIT;
     const int s0 = (int)(rest / ch.nq);
     const int64_t g = ch.q0 + (rest - (int64_t)s0 * ch.nq);
-    const int S = ch.S;
     const int2 qk = a.quartets[g];
     const int slot = qk.x - ch.bra0;
     const Pair bp = a.pairs[qk.x];
     const Pair kp = a.pairs[qk.y];
-    const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
-    QuartetGeom geo;
-    double AB[3], CD[3];
-    load_geom(sA, sB, sC, sD, geo, AB, CD);
-    double f = 1.0;
-    if (bp.sa == bp.sb) f *= 0.5;
-    if (kp.sa == kp.sb) f *= 0.5;
-    if (qk.x == qk.y) f *= 0.5;
-    const int oa = sA.ao, ob = sB.ao;
-    if constexpr (!K::kSplit) {
-      double acc[K::NA][K::NB][K::NC][K::ND];
-#pragma unroll
-      for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-        for (int ib = 0; ib < K::NB; ++ib)
-#pragma unroll
-          for (int ic = 0; ic < K::NC; ++ic)
-#pragma unroll
-            for (int id = 0; id < K::ND; ++id) acc[ia][ib][ic][id] = 0.0;
-      for (int ip = s0; ip < bp.np; ip += S) {
-        const PrimPair P = a.prims[bp.poff + ip];
-        for (int kq = 0; kq < kp.np; ++kq) {
-          const PrimPair Q = a.prims[kp.poff + kq];
-          PrimQ<K::NR> q;
-          prim_setup<K::NR>(P, Q, geo, q);
-          accumulate_full<LA, LB, LC, LD>(q, AB, CD, acc);
-        }
-      }
-      const int oc = sC.ao, od = sD.ao;
-      for (int bt = 0; bt < a.nbd; ++bt) {
-        const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
-        if (a.with_j && bt < a.nbj) {
-          double* J = a.jp + (size_t)bt * n * n;
-          double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
-          double dcd[K::NC][K::ND], dab[K::NA][K::NB];
-#pragma unroll
-          for (int ic = 0; ic < K::NC; ++ic)
-#pragma unroll
-            for (int id = 0; id < K::ND; ++id)
-              dcd[ic][id] = Dm[(size_t)(oc + ic) * n + od + id] + Dm[(size_t)(od + id) * n + oc + ic];
-#pragma unroll
-          for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-            for (int ib = 0; ib < K::NB; ++ib)
-              dab[ia][ib] = Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia];
-#pragma unroll
-          for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-            for (int ib = 0; ib < K::NB; ++ib) {
-              double sj = 0.0;
-#pragma unroll
-              for (int ic = 0; ic < K::NC; ++ic)
-#pragma unroll
-                for (int id = 0; id < K::ND; ++id) sj += acc[ia][ib][ic][id] * dcd[ic][id];
-              atomicAdd(jr + ia * K::NB + ib, f * sj);
-            }
-#pragma unroll
-          for (int ic = 0; ic < K::NC; ++ic)
-#pragma unroll
-            for (int id = 0; id < K::ND; ++id) {
-              double sj = 0.0;
-#pragma unroll
-              for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-                for (int ib = 0; ib < K::NB; ++ib) sj += acc[ia][ib][ic][id] * dab[ia][ib];
-              atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * sj);
-            }
-        }
-        if (a.with_k) {
-          double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * n;
-          double dbd[K::NB][K::ND], dbc[K::NB][K::NC], dad[K::NA][K::ND], dac[K::NA][K::NC];
-#pragma unroll
-          for (int ib = 0; ib < K::NB; ++ib) {
-#pragma unroll
-            for (int id = 0; id < K::ND; ++id) dbd[ib][id] = Dm[(size_t)(ob + ib) * n + od + id];
-#pragma unroll
-            for (int ic = 0; ic < K::NC; ++ic) dbc[ib][ic] = Dm[(size_t)(ob + ib) * n + oc + ic];
-          }
-#pragma unroll
-          for (int ia = 0; ia < K::NA; ++ia) {
-#pragma unroll
-            for (int id = 0; id < K::ND; ++id) dad[ia][id] = Dm[(size_t)(oa + ia) * n + od + id];
-#pragma unroll
-            for (int ic = 0; ic < K::NC; ++ic) dac[ia][ic] = Dm[(size_t)(oa + ia) * n + oc + ic];
-          }
-#pragma unroll
-          for (int ia = 0; ia < K::NA; ++ia) {
-#pragma unroll
-            for (int ic = 0; ic < K::NC; ++ic) {
-              double sk = 0.0;
-#pragma unroll
-              for (int ib = 0; ib < K::NB; ++ib)
-#pragma unroll
-                for (int id = 0; id < K::ND; ++id) sk += acc[ia][ib][ic][id] * dbd[ib][id];
-              atomicAdd(kr + (size_t)ia * n + oc + ic, f * sk);
-            }
-#pragma unroll
-            for (int id = 0; id < K::ND; ++id) {
-              double sk = 0.0;
-#pragma unroll
-              for (int ib = 0; ib < K::NB; ++ib)
-#pragma unroll
-                for (int ic = 0; ic < K::NC; ++ic) sk += acc[ia][ib][ic][id] * dbc[ib][ic];
-              atomicAdd(kr + (size_t)ia * n + od + id, f * sk);
-            }
-          }
-#pragma unroll
-          for (int ib = 0; ib < K::NB; ++ib) {
-#pragma unroll
-            for (int ic = 0; ic < K::NC; ++ic) {
-              double sk = 0.0;
-#pragma unroll
-              for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-                for (int id = 0; id < K::ND; ++id) sk += acc[ia][ib][ic][id] * dad[ia][id];
-              atomicAdd(kr + (size_t)(K::NA + ib) * n + oc + ic, f * sk);
-            }
-#pragma unroll
-            for (int id = 0; id < K::ND; ++id) {
-              double sk = 0.0;
-#pragma unroll
-              for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-                for (int ic = 0; ic < K::NC; ++ic) sk += acc[ia][ib][ic][id] * dac[ia][ic];
-              atomicAdd(kr + (size_t)(K::NA + ib) * n + od + id, f * sk);
-            }
-          }
-        }
-      }
-    } else {
-      const int ic = kc / K::ND, id = kc - (kc / K::ND) * K::ND;
-      int ec[3], ed[3];
-      cart_xyz<LC>(ic, ec);
-      cart_xyz<LD>(id, ed);
-      double wk[3][K::MM + 1];
-#pragma unroll
-      for (int x = 0; x < 3; ++x) ket_weights<K::MM>(ec[x], ed[x], CD[x], wk[x]);
-      double acc[K::NA][K::NB];
-#pragma unroll
-      for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-        for (int ib = 0; ib < K::NB; ++ib) acc[ia][ib] = 0.0;
-      for (int ip = s0; ip < bp.np; ip += S) {
-        const PrimPair P = a.prims[bp.poff + ip];
-        for (int kq = 0; kq < kp.np; ++kq) {
-          const PrimPair Q = a.prims[kp.poff + kq];
-          PrimQ<K::NR> q;
-          prim_setup<K::NR>(P, Q, geo, q);
-          accumulate_split<LA, LB, LC, LD>(q, AB, wk, acc);
-        }
-      }
-      const int c = sC.ao + ic, d = sD.ao + id;
-      for (int bt = 0; bt < a.nbd; ++bt) {
-        const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
-        if (a.with_j && bt < a.nbj) {
-          double* J = a.jp + (size_t)bt * n * n;
-          double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
-          const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
-          double s2 = 0.0;
-#pragma unroll
-          for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-            for (int ib = 0; ib < K::NB; ++ib) {
-              atomicAdd(jr + ia * K::NB + ib, f * acc[ia][ib] * dcd);
-              s2 += acc[ia][ib] * (Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia]);
-            }
-          atomicAdd(J + (size_t)c * n + d, f * s2);
-        }
-        if (a.with_k) {
-          double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * n;
-          double db_d[K::NB], db_c[K::NB];
-#pragma unroll
-          for (int ib = 0; ib < K::NB; ++ib) {
-            db_d[ib] = Dm[(size_t)(ob + ib) * n + d];
-            db_c[ib] = Dm[(size_t)(ob + ib) * n + c];
-          }
-#pragma unroll
-          for (int ia = 0; ia < K::NA; ++ia) {
-            double sc = 0.0, sd = 0.0;
-#pragma unroll
-            for (int ib = 0; ib < K::NB; ++ib) {
-              sc += acc[ia][ib] * db_d[ib];
-              sd += acc[ia][ib] * db_c[ib];
-            }
-            atomicAdd(kr + (size_t)ia * n + c, f * sc);
-            atomicAdd(kr + (size_t)ia * n + d, f * sd);
-          }
-          double da_d[K::NA], da_c[K::NA];
-#pragma unroll
-          for (int ia = 0; ia < K::NA; ++ia) {
-            da_d[ia] = Dm[(size_t)(oa + ia) * n + d];
-            da_c[ia] = Dm[(size_t)(oa + ia) * n + c];
-          }
-#pragma unroll
-          for (int ib = 0; ib < K::NB; ++ib) {
-            double sc = 0.0, sd = 0.0;
-#pragma unroll
-            for (int ia = 0; ia < K::NA; ++ia) {
-              sc += acc[ia][ib] * da_d[ia];
-              sd += acc[ia][ib] * da_c[ia];
-            }
-            atomicAdd(kr + (size_t)(K::NA + ib) * n + c, f * sc);
-            atomicAdd(kr + (size_t)(K::NA + ib) * n + d, f * sd);
-          }
-        }
-      }
+    bool lowp = false;
+    if (a.mixed) {
+      const double* dm = a.dmax;
+      const int ns = a.nseg;
+      const double d6 = fmax(fmax(fmax(dm[bp.sa * ns + bp.sb], dm[kp.sa * ns + kp.sb]),
+                                  fmax(dm[bp.sa * ns + kp.sa], dm[bp.sa * ns + kp.sb])),
+                             fmax(dm[bp.sb * ns + kp.sa], dm[bp.sb * ns + kp.sb]));
+      lowp = a.pair_q[qk.x] * a.pair_q[qk.y] * d6 < a.thr;
     }
+    if (lowp)
+      bra_item<LA, LB, LC, LD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow);
+    else
+      bra_item<LA, LB, LC, LD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow);
   }
   __syncthreads();
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)

@@ -74,8 +74,12 @@ class DirectJKPlan:
             self._handle = None
 
     def jk(self, density: torch.Tensor, *, with_j=True, with_k=True, symmetric=None,
-           validate=True, out: torch.Tensor | None = None) -> torch.Tensor:
-        """density (nb, n, n) float64 on the GPU -> out (nops, nb, n, n)."""
+           validate=True, out: torch.Tensor | None = None, mixed_threshold: float | None = None) -> torch.Tensor:
+        """density (nb, n, n) float64 on the GPU -> out (nops, nb, n, n).
+
+        mixed_threshold: evaluate in FP32 every quartet whose Q_ij Q_kl max|D|
+        (max over the six density blocks it touches) is below this value;
+        roots/weights, prefactors and all J/K accumulation stay FP64."""
         d = density
         if d.dim() == 2:
             d = d[None]
@@ -90,9 +94,13 @@ class DirectJKPlan:
         if symmetric is None:
             symmetric = bool(torch.equal(d, d.transpose(1, 2)))
         flags = (_native.FLAG_SYMMETRIC_DENSITY if symmetric else 0) | (0 if validate else _native.FLAG_NO_VALIDATE)
+        thr = 0.0
+        if mixed_threshold is not None:
+            flags |= _native.FLAG_MIXED_PRECISION
+            thr = float(mixed_threshold)
         _native.check(_native.lib().gscf_jk_build(
             self._handle, d.data_ptr(), nb, out.data_ptr(), int(bool(with_j)), int(bool(with_k)),
-            flags, ctypes.c_double(0.0), _stream_handle()))
+            flags, ctypes.c_double(thr), _stream_handle()))
         return out
 
     def schwarz(self) -> np.ndarray:
@@ -115,6 +123,28 @@ class DirectJKPlan:
         _native.check(_native.lib().gscf_plan_class_times(self._handle, cap, cls.ctypes.data, ms.ctypes.data,
                                                           la.ctypes.data, ctypes.byref(cnt)))
         return [(tuple(int(v) for v in cls[i]), float(ms[i]), int(la[i])) for i in range(cnt.value)]
+
+    def jk_geometry_vjp(self, density, cot_j=None, cot_k=None) -> torch.Tensor:
+        """d/dR [sum(cot_j * J(D)) + sum(cot_k * K(D))] for every atm-row coordinate,
+        as (natm, 3); the reference's coordinate VJP of direct J/K (cutoff must be 0)."""
+        dev = _device()
+        n = self.nao
+
+        def prep(x):
+            if x is None:
+                return None
+            x = x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x))
+            x = x.to(dev, torch.float64).contiguous()
+            if tuple(x.shape) != (n, n):
+                raise ValueError("geometry VJP inputs must be (n, n)")
+            return x
+
+        d, cj, ck = prep(density), prep(cot_j), prep(cot_k)
+        grad = torch.zeros((len(self.atm), 3), dtype=torch.float64, device=dev)
+        _native.check(_native.lib().gscf_jk_geometry_vjp(
+            self._handle, d.data_ptr(), None if cj is None else cj.data_ptr(),
+            None if ck is None else ck.data_ptr(), grad.data_ptr(), _stream_handle()))
+        return grad
 
     def int1e(self):
         """(S, T, V) as (n, n) float64 GPU tensors in the plan's AO basis."""

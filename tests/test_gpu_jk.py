@@ -229,3 +229,87 @@ def test_gpu_matches_golden_fixtures():
     assert abs(np.sum(d * k) - f["trDK"]) < 1e-9 * abs(f["trDK"])
     e = json.loads((g / "scf_energies.json").read_text())["water_631g_cart"]["energy"]
     assert abs(RHF(M.water_monomer(), "6-31g", cart=True).kernel() - e) < 1e-9
+
+
+def test_mixed_precision_jk_close_to_fp64():
+    """FP32 quartets below the threshold: J/K stay close to FP64; threshold 0 reproduces FP64."""
+    import torch
+    from gradscf_amd import molecule as M
+    from gradscf_amd.jk import DirectJKPlan
+    _, _, atm, bas, env = systems.tables(M.water_cluster(4, seed=3, box=6.0), "6-31g*", cart=True)
+    plan = DirectJKPlan(atm, bas, env, cart=True, cutoff=1e-12)
+    n = plan.nao
+    c = torch.as_tensor(np.random.default_rng(5).normal(size=(n, 20)) / np.sqrt(n), device="cuda")
+    d = (c @ c.T)[None].contiguous()
+    ref = plan.jk(d, symmetric=True)
+    same = plan.jk(d, symmetric=True, mixed_threshold=0.0)
+    assert float((same - ref).abs().max()) < 1e-12
+    mixed = plan.jk(d, symmetric=True, mixed_threshold=1e-7)
+    err = float((mixed - ref).abs().max())
+    assert err < 1e-6, err
+    loose = plan.jk(d, symmetric=True, mixed_threshold=1e30)   # every quartet in FP32
+    err_all = float((loose - ref).abs().max())
+    assert err_all < 1e-3 and err_all > err
+
+
+def test_mixed_precision_scf_energy_within_micro_hartree():
+    from gradscf_amd import molecule as M
+    from gradscf_amd.scf import RHF
+    spec = M.water_cluster(4, seed=3, box=6.0)
+    e64 = RHF(spec, "6-31g*", cart=True, direct_scf_tol=1e-12).kernel()
+    emix = RHF(spec, "6-31g*", cart=True, direct_scf_tol=1e-12, mixed_threshold=1e-7).kernel()
+    assert abs(emix - e64) < 1e-6, (emix, e64)
+
+
+@pytest.mark.parametrize("basis,cart", [("6-31g*", True), ("cc-pvdz", False)])
+def test_geometry_vjp_matches_oracle(oracle_mod, basis, cart):
+    """Coordinate VJP of J/K vs the oracle's s1 loop over libcint-convention int2e_ip1
+    (csrc/direct_jk_geometry.cc:68-84), nonsymmetric cotangents."""
+    from gradscf_amd.jk import DirectJKPlan
+    _, _, atm, bas, env = systems.tables(systems.water(), basis, cart=cart)
+    n = oracle_mod.nao(bas, cart)
+    rng = np.random.default_rng(21)
+    d = systems.random_density(n, seed=22)[0]
+    cj = rng.normal(size=(n, n))
+    ck = rng.normal(size=(n, n))
+    plan = DirectJKPlan(atm, bas, env, cart=cart, cutoff=0.0)
+    got = plan.jk_geometry_vjp(_cuda(d), _cuda(cj), _cuda(ck)).cpu().numpy()
+    ref = oracle_mod.jk_geometry_vjp(atm, bas, env, d, cj, ck, cart=cart)
+    assert np.max(np.abs(got - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
+    assert np.max(np.abs(got.sum(axis=0))) < 1e-9  # translation sum rule
+
+
+def test_geometry_vjp_finite_difference():
+    """E(R) = tr(D J)/2 - tr(D K)/4 at fixed D: VJP gradient vs central differences
+    (step 1e-4 bohr) for a few coordinate slots."""
+    import torch
+    from gradscf_amd.jk import DirectJKPlan, direct_jk
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "6-31g*", cart=True)
+    plan = DirectJKPlan(atm, bas, env, cart=True, cutoff=0.0)
+    n = plan.nao
+    c = np.random.default_rng(3).normal(size=(n, 10)) / np.sqrt(n)
+    d = c @ c.T
+    dt = _cuda(d)
+    g = plan.jk_geometry_vjp(dt, 0.5 * dt, -0.25 * dt).cpu().numpy()
+
+    def energy(env_):
+        j, k = direct_jk(atm, bas, env_, dt, cart=True)
+        return float(0.5 * torch.sum(dt * j) - 0.25 * torch.sum(dt * k))
+
+    h = 1e-4
+    for row in (6, 9, 14):
+        for x in range(3):
+            e1, e2 = env.copy(), env.copy()
+            e1[atm[row, 1] + x] += h
+            e2[atm[row, 1] + x] -= h
+            fd = (energy(e1) - energy(e2)) / (2 * h)
+            assert abs(fd - g[row, x]) < 2e-7, (row, x, fd, g[row, x])
+
+
+def test_screened_geometry_vjp_rejected():
+    from gradscf_amd.jk import DirectJKPlan
+    _, _, atm, bas, env = systems.tables(systems.water(), "6-31g", cart=True)
+    plan = DirectJKPlan(atm, bas, env, cart=True, cutoff=1e-12)
+    d = _cuda(np.eye(13))
+    with pytest.raises(ValueError, match="cutoff=0"):
+        plan.jk_geometry_vjp(d, d, None)

@@ -184,3 +184,34 @@ def test_oracle_matches_golden_fixtures(oracle_mod):
     r = scf.rhf(t + v, s, spec.nelectron, spec.nuclear_repulsion(),
                 lambda d: tuple(oracle_mod.direct_jk(atm, bas, env, d, cart=True)[:, 0]))
     assert abs(r["energy"] - e) < 1e-10
+
+
+def test_oracle_geometry_vjp_pins(oracle_mod):
+    """Coordinate VJP: translation sum rule, central finite differences of
+    E = tr(D J)/2 - tr(D K)/4, and (with _ref) the same loop on libcint int2e_ip1."""
+    _, _, atm, bas, env = systems.tables(systems.water(), "6-31g", cart=True)
+    n = 13
+    rng = np.random.default_rng(0)
+    d = rng.normal(size=(n, n))
+    d = d + d.T
+    g = oracle_mod.jk_geometry_vjp(atm, bas, env, d, 0.5 * d, -0.25 * d, cart=True)
+    assert np.abs(g.sum(axis=0)).max() < 1e-11
+
+    def energy(e):
+        o = oracle_mod.direct_jk(atm, bas, e, d, cart=True)
+        return 0.5 * np.sum(d * o[0, 0]) - 0.25 * np.sum(d * o[1, 0])
+
+    h = 1e-4
+    for row in (3, 5, 10):
+        for x in range(3):
+            e1, e2 = env.copy(), env.copy()
+            e1[atm[row, 1] + x] += h
+            e2[atm[row, 1] + x] -= h
+            fd = (energy(e1) - energy(e2)) / (2 * h)
+            assert abs(fd - g[row, x]) < 3e-6 * max(1.0, abs(fd))
+    if oracle_mod.have_ref():
+        cj, ck = rng.normal(size=(n, n)), rng.normal(size=(n, n))
+        own = oracle_mod.jk_geometry_vjp(atm, bas, env, d, cj, ck, cart=True)
+        with oracle_mod.backend("libcint"):
+            ref = oracle_mod.jk_geometry_vjp(atm, bas, env, d, cj, ck, cart=True)
+        assert np.max(np.abs(own - ref)) < 1e-10
