@@ -13,8 +13,8 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgradscf_jk.so"
 
-EXPORTS = ("gscf_last_error", "gscf_plan_create", "gscf_plan_destroy", "gscf_plan_get_info",
-           "gscf_plan_get_schwarz", "gscf_plan_profile", "gscf_plan_class_times",
+EXPORTS = ("gscf_last_error", "gscf_plan_create", "gscf_plan_create_batch", "gscf_plan_get_molecules",
+           "gscf_plan_destroy", "gscf_plan_get_info", "gscf_plan_get_schwarz", "gscf_plan_profile", "gscf_plan_class_times",
            "gscf_jk_build", "gscf_direct_jk", "gscf_eri_blocks", "gscf_int1e", "gscf_jk_geometry_vjp")
 
 GSCF_OK, GSCF_INVALID_ARGUMENT, GSCF_INTERNAL = 0, 1, 2
@@ -26,7 +26,8 @@ FLAG_NO_VALIDATE = 4
 class PlanInfo(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("n_cart", ctypes.c_int32), ("nbas", ctypes.c_int32),
                 ("nseg", ctypes.c_int32), ("npairs", ctypes.c_int64), ("nquartets", ctypes.c_int64),
-                ("cutoff", ctypes.c_double), ("plan_ms", ctypes.c_double)]
+                ("cutoff", ctypes.c_double), ("plan_ms", ctypes.c_double), ("nmol", ctypes.c_int32),
+                ("nc_max", ctypes.c_int32), ("nn_total", ctypes.c_int64)]
 
 
 class NativeError(RuntimeError):
@@ -46,6 +47,9 @@ def lib() -> ctypes.CDLL:
     vp, i32, i64, dbl, u32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_uint32
     handle.gscf_last_error.restype = ctypes.c_char_p
     handle.gscf_plan_create.argtypes = [vp, i32, vp, i32, vp, i32, i32, dbl, vp, ctypes.POINTER(vp)]
+    handle.gscf_plan_create_batch.argtypes = [vp, i32, vp, i32, vp, i32, i32, vp, vp, i32, dbl, vp,
+                                              ctypes.POINTER(vp)]
+    handle.gscf_plan_get_molecules.argtypes = [vp, i32, vp, vp]
     handle.gscf_plan_destroy.argtypes = [vp]
     handle.gscf_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
     handle.gscf_plan_get_schwarz.argtypes = [vp, vp]

@@ -62,10 +62,13 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
     if (bp.sa == bp.sb) f *= 0.5;
     if (kp.sa == kp.sb) f *= 0.5;
     if (qk.x == qk.y) f *= 0.5;
-    const int n = a.n;
+    const MolInfo mi = a.mols[sA.mol];
+    const int n = mi.n;
     const int oa = sA.ao, ob = sB.ao;
     const int c = sC.ao + ic, d = sD.ao + id;
-    const double* Dm = a.dens;
+    const double* Dm = a.dens + mi.base;
+    const double* CJ = a.cj ? a.cj + mi.base : nullptr;
+    const double* CK = a.ck ? a.ck + mi.base : nullptr;
     // 8-permutation weight of every bra component for this ket component
     double W[NA][NB];
 #pragma unroll
@@ -74,12 +77,12 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs a) {
       for (int ib = 0; ib < NB; ++ib) {
         const int p = oa + ia, q = ob + ib;
         double w = 0.0;
-        if (a.cj) {
-          w += (a.cj[(size_t)p * n + q] + a.cj[(size_t)q * n + p]) * (Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c]);
-          w += (a.cj[(size_t)c * n + d] + a.cj[(size_t)d * n + c]) * (Dm[(size_t)p * n + q] + Dm[(size_t)q * n + p]);
+        if (CJ) {
+          w += (CJ[(size_t)p * n + q] + CJ[(size_t)q * n + p]) * (Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c]);
+          w += (CJ[(size_t)c * n + d] + CJ[(size_t)d * n + c]) * (Dm[(size_t)p * n + q] + Dm[(size_t)q * n + p]);
         }
-        if (a.ck) {
-          const double* K = a.ck;
+        if (CK) {
+          const double* K = CK;
           w += K[(size_t)p * n + c] * Dm[(size_t)q * n + d] + K[(size_t)q * n + c] * Dm[(size_t)p * n + d] +
                K[(size_t)p * n + d] * Dm[(size_t)q * n + c] + K[(size_t)q * n + d] * Dm[(size_t)p * n + c] +
                K[(size_t)c * n + p] * Dm[(size_t)d * n + q] + K[(size_t)d * n + p] * Dm[(size_t)c * n + q] +

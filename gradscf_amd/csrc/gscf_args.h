@@ -24,8 +24,20 @@ struct Pair {
 
 struct ShellXYZ {
   double x, y, z;
-  int ao;          // Cartesian AO offset
+  int ao;          // Cartesian AO offset inside its molecule
   int l;
+  int mol;         // molecule of a batched plan
+  int lseg;        // segmented-shell index inside its molecule
+};
+
+// Per-molecule layout of a (batched) plan: Cartesian matrices of molecule m
+// live at base .. base + n*n of every density/J/K batch; dbase/nseg address
+// its shell-block density maxima.
+struct MolInfo {
+  int64_t base;
+  int64_t dbase;
+  int32_t n;
+  int32_t nseg;
 };
 
 
@@ -49,9 +61,11 @@ struct JKArgs {
   const SubTask* tasks;
   int ntasks;
   int64_t nitems;
-  const double* dens;   // [nbd][n][n] Cartesian densities
-  double* jp;           // [nbj][n][n] partial J (J = Jp + Jp^T)
-  double* kp;           // [nbd][n][n] partial K
+  const double* dens;   // [nbd][sum_m n_m^2] Cartesian densities
+  double* jp;           // [nbj][...] partial J (J = Jp + Jp^T)
+  double* kp;           // [nbd][...] partial K
+  const MolInfo* mols;
+  int64_t bstride;      // elements per density batch
   int n, nbd, nbj, with_j, with_k;
 };
 
@@ -74,16 +88,17 @@ struct BraArgs {
   const BraChunk* chunks;
   int nchunks;
   int nbra_max;         // LDS slots per workgroup
-  const double* dens;   // [nbd][n][n]
+  const double* dens;   // [nbd][sum_m n_m^2]
   double* jp;
   double* kp;
-  int n, nbd, nbj, with_j, with_k;
+  const MolInfo* mols;
+  int64_t bstride;
+  int n, nbd, nbj, with_j, with_k;  // n: widest molecule (LDS row length)
   // mixed precision: FP32 for quartets with pair_q[bra] pair_q[ket] dmax6 < thr
   int mixed;
   double thr;
   const double* pair_q;  // Schwarz bound per pair
-  const double* dmax;    // [nseg][nseg] max |D| over shell blocks (all batches)
-  int nseg;
+  const double* dmax;    // per molecule [nseg][nseg] max |D| over shell blocks (all batches)
 };
 
 // Explicit quartet list writing contracted Cartesian blocks:
@@ -119,10 +134,10 @@ struct GradArgs {
   const SubTask* tasks;
   int ntasks;
   int64_t nitems;
-  const double* dens;    // Cartesian D, cot_J, cot_K (n x n); cot may be null
+  const double* dens;    // Cartesian D, cot_J, cot_K (per molecule n x n); cot may be null
   const double* cj;
   const double* ck;
-  int n;
+  const MolInfo* mols;
   int natm;
   double* grad;          // [natm][3]
 };

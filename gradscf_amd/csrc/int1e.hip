@@ -15,10 +15,10 @@ struct Int1eArgs {
   const Pair* pairs;
   const PrimPair* prims;
   const ShellXYZ* shells;
-  const double* nuclei;   // [nnuc][4]: x, y, z, Z
+  const double* nuclei;   // [nnuc][5]: x, y, z, Z, molecule
   int nnuc;
   int npairs;
-  int nc;                 // Cartesian AO count
+  const MolInfo* mols;    // Cartesian matrix of each molecule
   double* s;
   double* t;
   double* v;
@@ -90,7 +90,8 @@ __global__ __launch_bounds__(256) void int1e_kernel(Int1eArgs a) {
       }
     // nuclear attraction
     for (int c = 0; c < a.nnuc; ++c) {
-      const double* C = a.nuclei + 4 * c;
+      const double* C = a.nuclei + 5 * c;
+      if ((int)C[4] != A.mol) continue;
       const double pc[3] = {Pr[0] - C[0], Pr[1] - C[1], Pr[2] - C[2]};
       const double T = p * (pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2]);
       double u[NR], w[NR];
@@ -119,12 +120,14 @@ __global__ __launch_bounds__(256) void int1e_kernel(Int1eArgs a) {
       }
     }
   }
-  const int nc = a.nc;
+  const MolInfo mi = a.mols[A.mol];
+  const int nc = mi.n;
 #pragma unroll
   for (int ia = 0; ia < NA; ++ia)
 #pragma unroll
     for (int ib = 0; ib < NB; ++ib) {
-      const size_t x = (size_t)(A.ao + ia) * nc + B.ao + ib, y = (size_t)(B.ao + ib) * nc + A.ao + ia;
+      const size_t x = mi.base + (size_t)(A.ao + ia) * nc + B.ao + ib;
+      const size_t y = mi.base + (size_t)(B.ao + ib) * nc + A.ao + ia;
       a.s[x] = accS[ia][ib]; a.s[y] = accS[ia][ib];
       a.t[x] = accT[ia][ib]; a.t[y] = accT[ia][ib];
       a.v[x] = accV[ia][ib]; a.v[y] = accV[ia][ib];
@@ -132,10 +135,10 @@ __global__ __launch_bounds__(256) void int1e_kernel(Int1eArgs a) {
 }
 
 void launch_int1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
-                        const ShellXYZ* shells, const double* nuclei, int nnuc, int nc, double* s, double* t,
-                        double* v, hipStream_t st) {
+                        const ShellXYZ* shells, const double* nuclei, int nnuc, const MolInfo* mols, double* s,
+                        double* t, double* v, hipStream_t st) {
   if (npairs <= 0) return;
-  Int1eArgs a{pairs, prims, shells, nuclei, nnuc, npairs, nc, s, t, v};
+  Int1eArgs a{pairs, prims, shells, nuclei, nnuc, npairs, mols, s, t, v};
   const dim3 grid((npairs + 255) / 256), block(256);
 #define L1E(A, B)                                                          \
   if (la == A && lb == B) {                                                \

@@ -101,8 +101,17 @@ static T* dev_upload(const std::vector<T>& v, hipStream_t s) {
 struct Seg {
   int parent, ctr, l, np;
   double r[3];
-  int ao_cart;
+  int ao_cart;               // Cartesian AO offset inside its molecule
+  int mol, lseg;             // molecule, segment index inside the molecule
   std::vector<double> a, c;  // exponents, coefficients * s/p factor
+};
+
+// Per-molecule AO layout on the device for the transform kernels: matrices of
+// molecule m start at sbase (requested basis), cbase (Cartesian) and dbase
+// (shell-block maxima) of every batch; ao0/aoc0 index the global AO maps.
+struct MolAO {
+  int64_t sbase, cbase, dbase;
+  int32_t n, nc, ao0, aoc0, seg0, nseg;
 };
 
 struct QuartetClass {
@@ -127,14 +136,20 @@ constexpr int kChunkBrasMax = 16;
 struct PlanData {
   int cart = 1, n = 0, n_cart = 0, nbas = 0, natm = 0;
   double cutoff = 0.0, plan_ms = 0.0;
-  std::vector<int> atm, bas, ao, ao_cart;
+  std::vector<int> atm, bas, ao, ao_cart;  // ao/ao_cart: global AO offsets of every shell
   std::vector<double> env;
+  // molecules: atm rows [mol_atm[m], mol_atm[m+1]), bas rows [mol_bas[m], mol_bas[m+1])
+  int nmol = 1;
+  std::vector<int> mol_atm, mol_bas, mol_seg;  // (nmol + 1) offsets
+  std::vector<MolAO> mols;                     // nmol + 1 (the last holds the totals)
+  std::vector<int64_t> mol_pair0;              // first all_pairs index of each molecule
+  int nc_max = 0;                              // widest molecule (Cartesian AOs)
+  int64_t nn_tot = 0, ncc_tot = 0, nd_tot = 0; // packed matrix sizes of one batch
   std::vector<Seg> seg;
   std::vector<PrimPair> prims;
-  // all segmented pairs (unscreened, unsorted) for block queries
+  // all intra-molecule segmented pairs (unscreened) for block queries
   std::vector<Pair> all_pairs;
   std::vector<int> all_class;
-  std::map<std::pair<int, int>, int> pair_index;  // (S,T) S>=T -> all_pairs index
   // screened, class-sorted pairs
   std::vector<Pair> pairs;
   std::vector<double> pair_q;
@@ -150,6 +165,8 @@ struct PlanData {
   double* d_c2s_coef = nullptr;
   int* d_s2c_idx = nullptr;     // [n][3] Cartesian AOs feeding a spherical AO
   double* d_s2c_coef = nullptr;
+  MolInfo* d_molinfo = nullptr; // kernel view of the molecule layout
+  MolAO* d_molao = nullptr;
   // workspace
   double* d_dc = nullptr;
   double* d_jp = nullptr;
@@ -157,12 +174,13 @@ struct PlanData {
   size_t ws_nbd = 0;
   int* d_flag = nullptr;
   int* h_flag = nullptr;
-  // one-electron: all segmented pairs sorted by class, nuclei (x, y, z, Z)
+  // one-electron: all segmented pairs sorted by class, nuclei (x, y, z, Z, molecule)
   Pair* d_pairs1e = nullptr;
   int p1e_begin[kNumPairClasses + 1] = {0};
   double* d_nuclei = nullptr;
   int nnuc = 0;
-  // parent-shell Schwarz bounds (nbas x nbas, lower triangle; empty when cutoff == 0)
+  // parent-shell Schwarz bounds, one nbas_m x nbas_m block per molecule
+  // (lower triangle, local shell indices; empty when cutoff == 0)
   std::vector<double> parent_q;
   // mixed precision: per-pair bounds on device, per-build density block maxima
   double* d_pair_q = nullptr;
@@ -189,8 +207,8 @@ GSCF_CLASSES(X)
 #undef X
 
 void launch_int1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
-                        const ShellXYZ* shells, const double* nuclei, int nnuc, int nc, double* s, double* t,
-                        double* v, hipStream_t st);
+                        const ShellXYZ* shells, const double* nuclei, int nnuc, const MolInfo* mols, double* s,
+                        double* t, double* v, hipStream_t st);
 
 #define X(A, B, C, D) void launch_jk_bra_c##A##B##C##D(const BraArgs& a, size_t lds, hipStream_t s);
 GSCF_CLASSES(X)
@@ -259,82 +277,116 @@ __global__ void fill_kernel(double* p, size_t count, double v) {
   if (i < count) p[i] = v;
 }
 
+// Flags and counters are cleared with a kernel, not hipMemsetAsync, so that
+// every launch sequence of a build can be captured into a HIP graph.
+__global__ void fill_int_kernel(int* p, int count, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
 __global__ void finite_kernel(const double* p, size_t count, int* flag) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count && !isfinite(p[i])) atomicOr(flag, 1);
 }
 
-// Dc[b][P][Q] = sum T[p,P] T[q,Q] D[src b][p][q] (or D^T when transpose)
-__global__ void dens_to_cart_kernel(const double* __restrict__ D, int n, int nb, double* __restrict__ Dc, int nc,
+// Molecule owning element r of a packed batch: the last m with base(m) <= r,
+// where base is sbase (which = 0), cbase (1) or dbase (2).
+__device__ __forceinline__ int find_mol(const MolAO* __restrict__ M, int nmol, int64_t r, int which) {
+  int lo = 0, hi = nmol;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    const int64_t b = which == 0 ? M[mid].sbase : (which == 1 ? M[mid].cbase : M[mid].dbase);
+    if (b <= r) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Dc[b][P][Q] = sum T[p,P] T[q,Q] D[src b][p][q] (or D^T when transpose), per
+// molecule; the maps hold molecule-local indices at global AO positions.
+__global__ void dens_to_cart_kernel(const double* __restrict__ D, int64_t nn_tot, int nb, double* __restrict__ Dc,
+                                    int64_t ncc_tot, const MolAO* __restrict__ M, int nmol,
                                     const int* __restrict__ idx, const double* __restrict__ coef, int nbd) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t total = (size_t)nbd * nc * nc;
+  const size_t total = (size_t)nbd * ncc_tot;
   if (i >= total) return;
-  const int bt = (int)(i / ((size_t)nc * nc));
-  const int rem = (int)(i - (size_t)bt * nc * nc);
-  const int P = rem / nc, Q = rem - (rem / nc) * nc;
+  const int bt = (int)(i / ncc_tot);
+  const int64_t r = (int64_t)(i - (size_t)bt * ncc_tot);
+  const MolAO m = M[find_mol(M, nmol, r, 1)];
+  const int rem = (int)(r - m.cbase);
+  const int P = rem / m.nc, Q = rem - (rem / m.nc) * m.nc;
   const bool tr = bt >= nb;
-  const double* Db = D + (size_t)(tr ? bt - nb : bt) * n * n;
+  const int n = m.n;
+  const double* Db = D + (size_t)(tr ? bt - nb : bt) * nn_tot + m.sbase;
+  const int gP = m.aoc0 + P, gQ = m.aoc0 + Q;
   double s = 0.0;
   for (int e1 = 0; e1 < 2; ++e1) {
-    const double c1 = coef[P * 2 + e1];
+    const double c1 = coef[gP * 2 + e1];
     if (c1 == 0.0) continue;
-    const int p = idx[P * 2 + e1];
+    const int p = idx[gP * 2 + e1];
     for (int e2 = 0; e2 < 2; ++e2) {
-      const double c2 = coef[Q * 2 + e2];
+      const double c2 = coef[gQ * 2 + e2];
       if (c2 == 0.0) continue;
-      const int q = idx[Q * 2 + e2];
+      const int q = idx[gQ * 2 + e2];
       s += c1 * c2 * (tr ? Db[(size_t)q * n + p] : Db[(size_t)p * n + q]);
     }
   }
   Dc[i] = s;
 }
 
-// out[op][b][p][q]: J = T (Jp + Jp^T) T^T, K = T (Kp_b + Kp_b'^T) T^T
-__global__ void finalize_kernel(const double* __restrict__ jp, const double* __restrict__ kp, int nc, int n, int nb,
-                                int sym, int with_j, int with_k, const int* __restrict__ idx,
+// out[op][b][p][q]: J = T (Jp + Jp^T) T^T, K = T (Kp_b + Kp_b'^T) T^T, per molecule
+__global__ void finalize_kernel(const double* __restrict__ jp, const double* __restrict__ kp, int64_t ncc_tot,
+                                int64_t nn_tot, int nb, int sym, int with_j, int with_k,
+                                const MolAO* __restrict__ M, int nmol, const int* __restrict__ idx,
                                 const double* __restrict__ coef, double* __restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nops = with_j + with_k;
-  const size_t total = (size_t)nops * nb * n * n;
+  const size_t total = (size_t)nops * nb * nn_tot;
   if (i >= total) return;
-  const int opb = (int)(i / ((size_t)n * n));
-  const int rem = (int)(i - (size_t)opb * n * n);
-  const int p = rem / n, q = rem - (rem / n) * n;
+  const int opb = (int)(i / nn_tot);
+  const int64_t r = (int64_t)(i - (size_t)opb * nn_tot);
+  const MolAO m = M[find_mol(M, nmol, r, 0)];
+  const int rem = (int)(r - m.sbase);
+  const int p = rem / m.n, q = rem - (rem / m.n) * m.n;
   const int op = opb / nb, b = opb - (opb / nb) * nb;
   const bool isj = with_j && op == 0;
-  const double* A = isj ? jp + (size_t)b * nc * nc : kp + (size_t)b * nc * nc;
-  const double* B = isj ? A : kp + (size_t)(sym ? b : nb + b) * nc * nc;
+  const double* A = (isj ? jp + (size_t)b * ncc_tot : kp + (size_t)b * ncc_tot) + m.cbase;
+  const double* B = isj ? A : kp + (size_t)(sym ? b : nb + b) * ncc_tot + m.cbase;
+  const int nc = m.nc, gp = m.ao0 + p, gq = m.ao0 + q;
   double s = 0.0;
   for (int e1 = 0; e1 < 3; ++e1) {
-    const double c1 = coef[p * 3 + e1];
+    const double c1 = coef[gp * 3 + e1];
     if (c1 == 0.0) continue;
-    const int P = idx[p * 3 + e1];
+    const int P = idx[gp * 3 + e1];
     for (int e2 = 0; e2 < 3; ++e2) {
-      const double c2 = coef[q * 3 + e2];
+      const double c2 = coef[gq * 3 + e2];
       if (c2 == 0.0) continue;
-      const int Q = idx[q * 3 + e2];
+      const int Q = idx[gq * 3 + e2];
       s += c1 * c2 * (A[(size_t)P * nc + Q] + B[(size_t)Q * nc + P]);
     }
   }
   out[i] = s;
 }
 
-// out[p][q] = sum T[p,P] T[q,Q] M[P][Q] (Cartesian -> requested AO basis)
-__global__ void sandwich_kernel(const double* __restrict__ M, int nc, int n, const int* __restrict__ idx,
-                                const double* __restrict__ coef, double* __restrict__ out) {
+// out[p][q] = sum T[p,P] T[q,Q] M[P][Q] (Cartesian -> requested AO basis), per molecule
+__global__ void sandwich_kernel(const double* __restrict__ Mc, int64_t nn_tot, const MolAO* __restrict__ M, int nmol,
+                                const int* __restrict__ idx, const double* __restrict__ coef,
+                                double* __restrict__ out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)n * n) return;
-  const int p = (int)(i / n), q = (int)(i - (size_t)(i / n) * n);
+  if (i >= (size_t)nn_tot) return;
+  const MolAO m = M[find_mol(M, nmol, (int64_t)i, 0)];
+  const int rem = (int)((int64_t)i - m.sbase);
+  const int p = rem / m.n, q = rem - (rem / m.n) * m.n;
+  const double* A = Mc + m.cbase;
+  const int gp = m.ao0 + p, gq = m.ao0 + q;
   double s = 0.0;
   for (int e1 = 0; e1 < 3; ++e1) {
-    const double c1 = coef[p * 3 + e1];
+    const double c1 = coef[gp * 3 + e1];
     if (c1 == 0.0) continue;
-    const int P = idx[p * 3 + e1];
+    const int P = idx[gp * 3 + e1];
     for (int e2 = 0; e2 < 3; ++e2) {
-      const double c2 = coef[q * 3 + e2];
+      const double c2 = coef[gq * 3 + e2];
       if (c2 == 0.0) continue;
-      s += c1 * c2 * M[(size_t)P * nc + idx[q * 3 + e2]];
+      s += c1 * c2 * A[(size_t)P * m.nc + idx[gq * 3 + e2]];
     }
   }
   out[i] = s;
@@ -353,17 +405,23 @@ __global__ void expand_kernel(const int64_t* __restrict__ qoff, int nbra, int br
   out[g] = make_int2(bra0 + lo, ket0 + (int)(g - qoff[lo]));
 }
 
-// dmax[S][T] = max over batches and over the Cartesian block (S, T) of |D|.
-__global__ void dmax_kernel(const double* __restrict__ D, int nc, int nbd, const int* __restrict__ seg_ao, int nseg,
-                            double* __restrict__ dmax) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nseg * nseg) return;
-  const int S = i / nseg, T = i - (i / nseg) * nseg;
+// dmax[m][S][T] = max over batches and over the Cartesian block (S, T) of |D_m|
+// (S, T segment indices inside molecule m).
+__global__ void dmax_kernel(const double* __restrict__ D, int64_t ncc_tot, int nbd, const MolAO* __restrict__ M,
+                            int nmol, int64_t nd_tot, const int* __restrict__ seg_ao, double* __restrict__ dmax) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nd_tot) return;
+  const MolAO mo = M[find_mol(M, nmol, i, 2)];
+  const int r = (int)(i - mo.dbase);
+  const int S = mo.seg0 + r / mo.nseg, T = mo.seg0 + r % mo.nseg;
   const int oS = seg_ao[2 * S], nS = seg_ao[2 * S + 1], oT = seg_ao[2 * T], nT = seg_ao[2 * T + 1];
+  const int nc = mo.nc;
   double m = 0.0;
-  for (int b = 0; b < nbd; ++b)
+  for (int b = 0; b < nbd; ++b) {
+    const double* Db = D + (size_t)b * ncc_tot + mo.cbase;
     for (int p = 0; p < nS; ++p)
-      for (int q = 0; q < nT; ++q) m = fmax(m, fabs(D[(size_t)b * nc * nc + (size_t)(oS + p) * nc + oT + q]));
+      for (int q = 0; q < nT; ++q) m = fmax(m, fabs(Db[(size_t)(oS + p) * nc + oT + q]));
+  }
   dmax[i] = m;
 }
 
@@ -372,10 +430,31 @@ static unsigned nblocks(size_t count) { return (unsigned)((count + 255) / 256); 
 // ----------------------------------------------------------------------------
 // plan construction
 // ----------------------------------------------------------------------------
+// mol_atm / mol_bas (nmol + 1 offsets each, NULL for one molecule) split the
+// concatenated tables into molecules; every shell must sit on an atom of its
+// own molecule.
 static void validate_tables(PlanData& P, const int32_t* atm, int natm, const int32_t* bas, int nbas,
-                            const double* env, int nenv, int cart) {
+                            const double* env, int nenv, int cart, int nmol = 1, const int32_t* mol_atm = nullptr,
+                            const int32_t* mol_bas = nullptr) {
   if (!atm || !bas || !env || natm < 0 || nbas < 0 || nenv < 20 || (cart != 0 && cart != 1))
     throw InvalidArgument("Invalid libcint table dimensions or operator");
+  if (nmol < 1 || (nmol > 1 && (!mol_atm || !mol_bas)))
+    throw InvalidArgument("A batched plan needs per-molecule atm and bas offsets");
+  P.nmol = nmol;
+  P.mol_atm.assign(nmol + 1, 0);
+  P.mol_bas.assign(nmol + 1, 0);
+  if (mol_atm && mol_bas) {
+    P.mol_atm.assign(mol_atm, mol_atm + nmol + 1);
+    P.mol_bas.assign(mol_bas, mol_bas + nmol + 1);
+  } else {
+    P.mol_atm[1] = natm;
+    P.mol_bas[1] = nbas;
+  }
+  if (P.mol_atm[0] != 0 || P.mol_bas[0] != 0 || P.mol_atm[nmol] != natm || P.mol_bas[nmol] != nbas)
+    throw InvalidArgument("Molecule offsets must start at 0 and end at natm / nbas");
+  for (int m = 0; m < nmol; ++m)
+    if (P.mol_atm[m + 1] < P.mol_atm[m] || P.mol_bas[m + 1] <= P.mol_bas[m])
+      throw InvalidArgument("Every molecule needs a non-decreasing atom range and at least one shell");
   P.natm = natm;
   P.nbas = nbas;
   P.cart = cart;
@@ -398,46 +477,91 @@ static void validate_tables(PlanData& P, const int32_t* atm, int natm, const int
   }
   P.ao.assign(nbas + 1, 0);
   P.ao_cart.assign(nbas + 1, 0);
-  for (int s = 0; s < nbas; ++s) {
-    const int* row = P.bas.data() + s * BAS_SLOTS;
-    const int l = row[ANG_OF], np = row[NPRIM_OF], nc = row[NCTR_OF];
-    if (row[ATOM_OF] < 0 || row[ATOM_OF] >= natm || l < 0 || l > 12 || np < 1 || np > 64 || nc < 1 ||
-        nc > 64 || row[KAPPA_OF] != 0 || !inside(row[PTR_EXP], np) || !inside(row[PTR_COEFF], int64_t(np) * nc))
-      throw InvalidArgument("Invalid or unsupported libcint basis table");
-    for (int p = 0; p < np; ++p)
-      if (P.env[row[PTR_EXP] + p] <= 0.) throw InvalidArgument("Gaussian exponents must be positive");
-    if (l > kMaxL)
-      throw InvalidArgument("The MI355X J/K engine supports shells up to d (l <= 2)");
-    P.ao[s + 1] = P.ao[s] + nc * (cart ? ncart_h(l) : 2 * l + 1);
-    P.ao_cart[s + 1] = P.ao_cart[s] + nc * ncart_h(l);
-  }
+  for (int m = 0; m < nmol; ++m)
+    for (int s = P.mol_bas[m]; s < P.mol_bas[m + 1]; ++s) {
+      const int* row = P.bas.data() + s * BAS_SLOTS;
+      const int l = row[ANG_OF], np = row[NPRIM_OF], nc = row[NCTR_OF];
+      if (row[ATOM_OF] < P.mol_atm[m] || row[ATOM_OF] >= P.mol_atm[m + 1] || l < 0 || l > 12 || np < 1 ||
+          np > 64 || nc < 1 || nc > 64 || row[KAPPA_OF] != 0 || !inside(row[PTR_EXP], np) ||
+          !inside(row[PTR_COEFF], int64_t(np) * nc))
+        throw InvalidArgument("Invalid or unsupported libcint basis table");
+      for (int p = 0; p < np; ++p)
+        if (P.env[row[PTR_EXP] + p] <= 0.) throw InvalidArgument("Gaussian exponents must be positive");
+      if (l > kMaxL)
+        throw InvalidArgument("The MI355X J/K engine supports shells up to d (l <= 2)");
+      P.ao[s + 1] = P.ao[s] + nc * (cart ? ncart_h(l) : 2 * l + 1);
+      P.ao_cart[s + 1] = P.ao_cart[s] + nc * ncart_h(l);
+    }
   P.n = P.ao.back();
   P.n_cart = P.ao_cart.back();
   if (P.n < 1) throw InvalidArgument("Invalid AO count");
+  // per-molecule matrix layout (segment fields are filled by build_segments)
+  P.mols.assign(nmol + 1, MolAO{});
+  int64_t sb = 0, cb = 0;
+  P.nc_max = 0;
+  for (int m = 0; m < nmol; ++m) {
+    MolAO& M = P.mols[m];
+    M.ao0 = P.ao[P.mol_bas[m]];
+    M.aoc0 = P.ao_cart[P.mol_bas[m]];
+    M.n = P.ao[P.mol_bas[m + 1]] - M.ao0;
+    M.nc = P.ao_cart[P.mol_bas[m + 1]] - M.aoc0;
+    M.sbase = sb;
+    M.cbase = cb;
+    sb += (int64_t)M.n * M.n;
+    cb += (int64_t)M.nc * M.nc;
+    P.nc_max = std::max(P.nc_max, M.nc);
+  }
+  P.mols[nmol].sbase = sb;
+  P.mols[nmol].cbase = cb;
+  P.nn_tot = sb;
+  P.ncc_tot = cb;
 }
 
 static void build_segments(PlanData& P) {
   const double pi = 3.14159265358979323846;
-  for (int s = 0; s < P.nbas; ++s) {
-    const int* row = P.bas.data() + s * BAS_SLOTS;
-    const int l = row[ANG_OF], np = row[NPRIM_OF], nc = row[NCTR_OF];
-    const double fsp = l == 0 ? 0.5 / std::sqrt(pi) : (l == 1 ? std::sqrt(3.0 / (4.0 * pi)) : 1.0);
-    const double* r = P.env.data() + P.atm[row[ATOM_OF] * ATM_SLOTS + PTR_COORD];
-    for (int c = 0; c < nc; ++c) {
-      Seg g;
-      g.parent = s;
-      g.ctr = c;
-      g.l = l;
-      g.np = np;
-      g.r[0] = r[0]; g.r[1] = r[1]; g.r[2] = r[2];
-      g.ao_cart = P.ao_cart[s] + c * ncart_h(l);
-      for (int p = 0; p < np; ++p) {
-        g.a.push_back(P.env[row[PTR_EXP] + p]);
-        g.c.push_back(P.env[row[PTR_COEFF] + c * np + p] * fsp);
+  P.mol_seg.assign(P.nmol + 1, 0);
+  int64_t db = 0;
+  for (int m = 0; m < P.nmol; ++m) {
+    P.mol_seg[m] = (int)P.seg.size();
+    for (int s = P.mol_bas[m]; s < P.mol_bas[m + 1]; ++s) {
+      const int* row = P.bas.data() + s * BAS_SLOTS;
+      const int l = row[ANG_OF], np = row[NPRIM_OF], nc = row[NCTR_OF];
+      const double fsp = l == 0 ? 0.5 / std::sqrt(pi) : (l == 1 ? std::sqrt(3.0 / (4.0 * pi)) : 1.0);
+      const double* r = P.env.data() + P.atm[row[ATOM_OF] * ATM_SLOTS + PTR_COORD];
+      for (int c = 0; c < nc; ++c) {
+        Seg g;
+        g.parent = s;
+        g.ctr = c;
+        g.l = l;
+        g.np = np;
+        g.r[0] = r[0]; g.r[1] = r[1]; g.r[2] = r[2];
+        g.ao_cart = P.ao_cart[s] - P.mols[m].aoc0 + c * ncart_h(l);
+        g.mol = m;
+        g.lseg = (int)P.seg.size() - P.mol_seg[m];
+        for (int p = 0; p < np; ++p) {
+          g.a.push_back(P.env[row[PTR_EXP] + p]);
+          g.c.push_back(P.env[row[PTR_COEFF] + c * np + p] * fsp);
+        }
+        P.seg.push_back(std::move(g));
       }
-      P.seg.push_back(std::move(g));
     }
+    MolAO& M = P.mols[m];
+    M.seg0 = P.mol_seg[m];
+    M.nseg = (int)P.seg.size() - M.seg0;
+    M.dbase = db;
+    db += (int64_t)M.nseg * M.nseg;
   }
+  P.mol_seg[P.nmol] = (int)P.seg.size();
+  P.mols[P.nmol].seg0 = (int)P.seg.size();
+  P.mols[P.nmol].dbase = db;
+  P.nd_tot = db;
+}
+
+// all_pairs index of the intra-molecule segment pair (S, T), S >= T.
+static int64_t pair_idx(const PlanData& P, int S, int T) {
+  const int m = P.seg[S].mol;
+  const int64_t s = S - P.mol_seg[m], t = T - P.mol_seg[m];
+  return P.mol_pair0[m] + s * (s + 1) / 2 + t;
 }
 
 // Primitive pairs of segmented shells (A, B) with l(A) >= l(B).  Pairs whose
@@ -571,8 +695,11 @@ static void compute_schwarz(PlanData& P, std::vector<double>& seg_q, hipStream_t
 static void build_c2s_maps(PlanData& P, hipStream_t s) {
   std::vector<int> cidx((size_t)P.n_cart * 2, 0), sidx((size_t)P.n * 3, 0);
   std::vector<double> ccoef((size_t)P.n_cart * 2, 0.0), scoef((size_t)P.n * 3, 0.0);
-  for (int sh = 0; sh < P.nbas; ++sh) {
+  // positions are global AO indices, stored indices are local to the molecule
+  for (int mo = 0; mo < P.nmol; ++mo)
+  for (int sh = P.mol_bas[mo]; sh < P.mol_bas[mo + 1]; ++sh) {
     const int l = P.bas[sh * BAS_SLOTS + ANG_OF], nc = P.bas[sh * BAS_SLOTS + NCTR_OF];
+    const int aoc0 = P.mols[mo].aoc0, ao0 = P.mols[mo].ao0;
     const int nf = ncart_h(l);
     const int ns = P.cart ? nf : 2 * l + 1;
     std::vector<std::vector<double>> T;
@@ -589,7 +716,7 @@ static void build_c2s_maps(PlanData& P, hipStream_t s) {
         for (int f = 0; f < nf; ++f)
           if (T[m][f] != 0.0) {
             if (k >= 3) throw Internal("c2s map overflow");
-            sidx[(size_t)(sbase + m) * 3 + k] = cb + f;
+            sidx[(size_t)(sbase + m) * 3 + k] = cb + f - aoc0;
             scoef[(size_t)(sbase + m) * 3 + k] = T[m][f];
             ++k;
           }
@@ -599,7 +726,7 @@ static void build_c2s_maps(PlanData& P, hipStream_t s) {
         for (int m = 0; m < ns; ++m)
           if (T[m][f] != 0.0) {
             if (k >= 2) throw Internal("c2s map overflow");
-            cidx[(size_t)(cb + f) * 2 + k] = sbase + m;
+            cidx[(size_t)(cb + f) * 2 + k] = sbase + m - ao0;
             ccoef[(size_t)(cb + f) * 2 + k] = T[m][f];
             ++k;
           }
@@ -625,63 +752,93 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
     sh[i].z = P.seg[i].r[2];
     sh[i].ao = P.seg[i].ao_cart;
     sh[i].l = P.seg[i].l;
+    sh[i].mol = P.seg[i].mol;
+    sh[i].lseg = P.seg[i].lseg;
   }
   P.d_shells = dev_upload(sh, s);
-  // all segmented pairs S >= T
-  for (int S = 0; S < nseg; ++S)
-    for (int T = 0; T <= S; ++T) {
-      int a = S, b = T;
-      if (P.seg[b].l > P.seg[a].l) std::swap(a, b);
-      Pair pr;
-      make_pair(P, a, b, pr);
-      P.pair_index[{S, T}] = (int)P.all_pairs.size();
-      P.all_pairs.push_back(pr);
-      P.all_class.push_back(pair_class(P.seg[a].l, P.seg[b].l));
+  // molecule table for the kernels
+  {
+    std::vector<MolInfo> mi(P.nmol);
+    for (int m = 0; m < P.nmol; ++m) {
+      mi[m].base = P.mols[m].cbase;
+      mi[m].dbase = P.mols[m].dbase;
+      mi[m].n = P.mols[m].nc;
+      mi[m].nseg = P.mols[m].nseg;
     }
+    P.d_molinfo = dev_upload(mi, s);
+    P.d_molao = dev_upload(P.mols, s);
+  }
+  // all intra-molecule segmented pairs S >= T, molecule by molecule
+  P.mol_pair0.assign(P.nmol + 1, 0);
+  for (int m = 0; m < P.nmol; ++m) {
+    P.mol_pair0[m] = (int64_t)P.all_pairs.size();
+    for (int S = P.mol_seg[m]; S < P.mol_seg[m + 1]; ++S)
+      for (int T = P.mol_seg[m]; T <= S; ++T) {
+        int a = S, b = T;
+        if (P.seg[b].l > P.seg[a].l) std::swap(a, b);
+        Pair pr;
+        make_pair(P, a, b, pr);
+        P.all_pairs.push_back(pr);
+        P.all_class.push_back(pair_class(P.seg[a].l, P.seg[b].l));
+      }
+  }
+  P.mol_pair0[P.nmol] = (int64_t)P.all_pairs.size();
   if (P.prims.empty()) P.prims.push_back(PrimPair{});
   P.d_prims = dev_upload(P.prims, s);
   P.d_all_pairs = dev_upload(P.all_pairs, s);
   // Schwarz bounds (parent-shell maxima, as the reference defines them per shell pair)
   const int npair = (int)P.all_pairs.size();
   std::vector<double> q(npair, 1.0);
+  std::vector<double> qmax(P.nmol, 0.0);
   if (cutoff > 0) {
     std::vector<double> seg_q;
     compute_schwarz(P, seg_q, s);
-    std::map<std::pair<int, int>, double> parent_max;
-    for (int S = 0; S < nseg; ++S)
-      for (int T = 0; T <= S; ++T) {
-        const int i = P.pair_index[{S, T}];
-        const int ps = P.seg[S].parent, pt = P.seg[T].parent;
-        auto key = std::make_pair(std::max(ps, pt), std::min(ps, pt));
-        double& m = parent_max[key];
-        m = std::max(m, seg_q[i]);
-      }
-    for (int S = 0; S < nseg; ++S)
-      for (int T = 0; T <= S; ++T) {
-        const int ps = P.seg[S].parent, pt = P.seg[T].parent;
-        q[P.pair_index[{S, T}]] = std::sqrt(parent_max[{std::max(ps, pt), std::min(ps, pt)}]);
-      }
-    P.parent_q.assign((size_t)P.nbas * P.nbas, 0.0);
-    for (auto& kv : parent_max) P.parent_q[(size_t)kv.first.first * P.nbas + kv.first.second] = std::sqrt(kv.second);
+    std::vector<int64_t> qbase(P.nmol + 1, 0);
+    for (int m = 0; m < P.nmol; ++m) {
+      const int64_t nb = P.mol_bas[m + 1] - P.mol_bas[m];
+      qbase[m + 1] = qbase[m] + nb * nb;
+    }
+    // parent_q[qbase[m] + i*nb + j], i >= j local parent shells: max over segment pairs
+    P.parent_q.assign((size_t)qbase[P.nmol], 0.0);
+    auto pslot = [&](int S, int T) -> double& {
+      const int m = P.seg[S].mol;
+      const int nb = P.mol_bas[m + 1] - P.mol_bas[m];
+      const int ps = P.seg[S].parent - P.mol_bas[m], pt = P.seg[T].parent - P.mol_bas[m];
+      return P.parent_q[(size_t)qbase[m] + (size_t)std::max(ps, pt) * nb + std::min(ps, pt)];
+    };
+    for (int m = 0; m < P.nmol; ++m)
+      for (int S = P.mol_seg[m]; S < P.mol_seg[m + 1]; ++S)
+        for (int T = P.mol_seg[m]; T <= S; ++T) {
+          double& v = pslot(S, T);
+          v = std::max(v, seg_q[pair_idx(P, S, T)]);
+        }
+    for (double& v : P.parent_q) v = std::sqrt(v);
+    for (int m = 0; m < P.nmol; ++m)
+      for (int S = P.mol_seg[m]; S < P.mol_seg[m + 1]; ++S)
+        for (int T = P.mol_seg[m]; T <= S; ++T) q[pair_idx(P, S, T)] = pslot(S, T);
   }
-  double qmax = 0.0;
-  for (double v : q) qmax = std::max(qmax, v);
-  // retained pairs, grouped by class and by primitive-pair count (power-of-two
-  // buckets, so lanes of one wavefront run similar primitive loops), Schwarz-
-  // descending inside a (class, bucket) list
+  for (int i = 0; i < npair; ++i) {
+    const int m = P.seg[P.all_pairs[i].sa].mol;
+    qmax[m] = std::max(qmax[m], q[i]);
+  }
+  // retained pairs, grouped by class, molecule and primitive-pair count
+  // (power-of-two buckets, so lanes of one wavefront run similar primitive
+  // loops), Schwarz-descending inside a (class, molecule, bucket) list
   auto bucket = [](int np) {
     int b = 0;
     while ((2 << b) <= np) ++b;
     return b;
   };
+  auto mol_of = [&](int i) { return P.seg[P.all_pairs[i].sa].mol; };
   std::vector<int> keep;
   for (int i = 0; i < npair; ++i) {
     if (P.all_pairs[i].np == 0) continue;
-    if (cutoff > 0 && q[i] * qmax < cutoff) continue;
+    if (cutoff > 0 && q[i] * qmax[mol_of(i)] < cutoff) continue;
     keep.push_back(i);
   }
   std::stable_sort(keep.begin(), keep.end(), [&](int x, int y) {
     if (P.all_class[x] != P.all_class[y]) return P.all_class[x] < P.all_class[y];
+    if (mol_of(x) != mol_of(y)) return mol_of(x) < mol_of(y);
     const int bx = bucket(P.all_pairs[x].np), by = bucket(P.all_pairs[y].np);
     if (bx != by) return bx > by;
     return q[x] > q[y];
@@ -690,11 +847,12 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
     P.pairs.push_back(P.all_pairs[i]);
     P.pair_q.push_back(q[i]);
   }
-  struct Run { int cls, bkt, beg, end, npmin, npmax; };
+  struct Run { int cls, mol, bkt, beg, end, npmin, npmax; };
   std::vector<Run> runs;
   for (int k = 0; k < (int)keep.size();) {
-    Run r{P.all_class[keep[k]], bucket(P.all_pairs[keep[k]].np), k, k, 1 << 30, 0};
-    while (k < (int)keep.size() && P.all_class[keep[k]] == r.cls && bucket(P.all_pairs[keep[k]].np) == r.bkt) {
+    Run r{P.all_class[keep[k]], mol_of(keep[k]), bucket(P.all_pairs[keep[k]].np), k, k, 1 << 30, 0};
+    while (k < (int)keep.size() && P.all_class[keep[k]] == r.cls && mol_of(keep[k]) == r.mol &&
+           bucket(P.all_pairs[keep[k]].np) == r.bkt) {
       r.npmin = std::min(r.npmin, P.all_pairs[keep[k]].np);
       r.npmax = std::max(r.npmax, P.all_pairs[keep[k]].np);
       ++k;
@@ -702,6 +860,9 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
     r.end = k;
     runs.push_back(r);
   }
+  // runs of each (class, molecule), for the intra-molecule quartet pairing below
+  std::map<std::pair<int, int>, std::vector<int>> runs_of;
+  for (int i = 0; i < (int)runs.size(); ++i) runs_of[{runs[i].cls, runs[i].mol}].push_back(i);
   for (int c = 0; c <= kNumPairClasses; ++c) P.class_begin[c] = (int)keep.size();
   for (int i = (int)runs.size() - 1; i >= 0; --i) P.class_begin[runs[i].cls] = runs[i].beg;
   for (int c = kNumPairClasses - 1; c >= 0; --c) P.class_begin[c] = std::min(P.class_begin[c], P.class_begin[c + 1]);
@@ -719,7 +880,7 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
     std::vector<int> at;
     for (const Seg& g : P.seg) at.push_back(P.bas[g.parent * BAS_SLOTS + ATOM_OF]);
     P.d_seg_atom = dev_upload(at, s);
-    HIP_OK(hipMalloc(&P.d_dmax, sizeof(double) * P.seg.size() * P.seg.size()));
+    HIP_OK(hipMalloc(&P.d_dmax, sizeof(double) * std::max<int64_t>(P.nd_tot, 1)));
   }
   // quartet classes: canonical (bra run, ket run) sub-tasks with materialised quartet lists
   static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
@@ -735,8 +896,10 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
       std::vector<Pending> pend;
       for (const Run& rb : runs) {
         if (rb.cls != cb) continue;
-        for (const Run& rk : runs) {
-          if (rk.cls != ck) continue;
+        auto it = runs_of.find({ck, rb.mol});
+        if (it == runs_of.end()) continue;
+        for (int rki : it->second) {
+          const Run& rk = runs[rki];
           const bool same_list = cb == ck && rb.bkt == rk.bkt;
           if (cb == ck && rb.bkt < rk.bkt) continue;  // canonical: each unordered pair-pair once
           const int nbra = rb.end - rb.beg, nket = rk.end - rk.beg;
@@ -777,7 +940,7 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
         const int ncomp_c = (qc.lc + 1) * (qc.lc + 2) / 2, ncomp_d = (qc.ld + 1) * (qc.ld + 2) / 2;
         const int rows = ncomp_a + ncomp_b;
         const int split = (ncomp_a * ncomp_b * ncomp_c * ncomp_d > 36) ? ncomp_c * ncomp_d : 1;
-        const size_t per_slot = sizeof(double) * ((size_t)rows * P.n_cart + (size_t)ncomp_a * ncomp_b);
+        const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)ncomp_a * ncomp_b);
         qc.nbra_max = per_slot > kLdsBudget ? 0
                       : (int)std::max<size_t>(1, std::min<size_t>(kChunkBrasMax, kLdsPack / per_slot));
         for (size_t ti = 0; qc.nbra_max >= 1 && ti < pend.size(); ++ti) {
@@ -850,14 +1013,15 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
     P.p1e_begin[kNumPairClasses] = k;
     P.d_pairs1e = dev_upload(p1e, s);
     std::vector<double> nuc;
-    for (int a = 0; a < P.natm; ++a) {
-      const int z = P.atm[a * ATM_SLOTS + CHARGE_OF];
-      if (z == 0) continue;
-      const double* r = P.env.data() + P.atm[a * ATM_SLOTS + PTR_COORD];
-      nuc.insert(nuc.end(), {r[0], r[1], r[2], (double)z});
-    }
-    P.nnuc = (int)(nuc.size() / 4);
-    if (nuc.empty()) nuc.assign(4, 0.0);
+    for (int m = 0; m < P.nmol; ++m)
+      for (int a = P.mol_atm[m]; a < P.mol_atm[m + 1]; ++a) {
+        const int z = P.atm[a * ATM_SLOTS + CHARGE_OF];
+        if (z == 0) continue;
+        const double* r = P.env.data() + P.atm[a * ATM_SLOTS + PTR_COORD];
+        nuc.insert(nuc.end(), {r[0], r[1], r[2], (double)z, (double)m});
+      }
+    P.nnuc = (int)(nuc.size() / 5);
+    if (nuc.empty()) nuc.assign(5, -1.0);
     P.d_nuclei = dev_upload(nuc, s);
   }
   HIP_OK(hipMalloc(&P.d_flag, sizeof(int)));
@@ -874,6 +1038,7 @@ static void free_plan(PlanData& P) {
   f(P.d_dc); f(P.d_jp); f(P.d_kp); f(P.d_flag);
   f(P.d_pairs1e); f(P.d_nuclei);
   f(P.d_pair_q); f(P.d_dmax); f(P.d_seg_ao); f(P.d_seg_atom);
+  f(P.d_molinfo); f(P.d_molao);
   for (auto& c : P.classes) {
     f(c.d_quartets);
     f(c.d_tasks);
@@ -890,7 +1055,7 @@ static void ensure_workspace(PlanData& P, size_t nbd) {
     p = nullptr;
   };
   f(P.d_dc); f(P.d_jp); f(P.d_kp);
-  const size_t nn = (size_t)P.n_cart * P.n_cart;
+  const size_t nn = (size_t)P.ncc_tot;
   HIP_OK(hipMalloc(&P.d_dc, sizeof(double) * nn * nbd));
   HIP_OK(hipMalloc(&P.d_jp, sizeof(double) * nn * nbd));
   HIP_OK(hipMalloc(&P.d_kp, sizeof(double) * nn * nbd));
@@ -906,10 +1071,9 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     throw InvalidArgument("Direct J/K requires at least one selected operator");
   if (nb < 1) throw InvalidArgument("Direct J/K shape mismatch");
   if (!density || !out) throw InvalidArgument("Direct J/K requires device density and output buffers");
-  const int n = P.n, nc = P.n_cart;
-  const size_t nn = (size_t)n * n, ncc = (size_t)nc * nc;
-  if (!(flags & 4u)) {
-    HIP_OK(hipMemsetAsync(P.d_flag, 0, sizeof(int), s));
+  const size_t nn = (size_t)P.nn_tot, ncc = (size_t)P.ncc_tot;
+  if (!(flags & GSCF_FLAG_NO_VALIDATE)) {
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, P.d_flag, 1, 0);
     hipLaunchKernelGGL(finite_kernel, dim3(nblocks(nn * nb)), dim3(256), 0, s, density, nn * nb, P.d_flag);
     HIP_OK(hipMemcpyAsync(P.h_flag, P.d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -918,13 +1082,11 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
   const bool sym = (flags & GSCF_FLAG_SYMMETRIC_DENSITY) != 0;
   const int nbd = (with_k && !sym) ? 2 * nb : nb;
   ensure_workspace(P, nbd);
-  hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, density, n, nb, P.d_dc, nc,
-                     P.d_c2s_idx, P.d_c2s_coef, nbd);
-  if (mixed) {
-    const int ns = (int)P.seg.size();
-    hipLaunchKernelGGL(dmax_kernel, dim3(nblocks((size_t)ns * ns)), dim3(256), 0, s, P.d_dc, nc, nbd, P.d_seg_ao,
-                       ns, P.d_dmax);
-  }
+  hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, density, P.nn_tot, nb, P.d_dc,
+                     P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, nbd);
+  if (mixed)
+    hipLaunchKernelGGL(dmax_kernel, dim3(nblocks((size_t)P.nd_tot)), dim3(256), 0, s, P.d_dc, P.ncc_tot, nbd,
+                       P.d_molao, P.nmol, P.nd_tot, P.d_seg_ao, P.d_dmax);
   hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_jp, ncc * nbd, 0.0);
   hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_kp, ncc * nbd, 0.0);
   for (size_t ci = 0; ci < P.classes.size(); ++ci) {
@@ -942,7 +1104,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     }
     const int nbd_eff = with_k ? nbd : nb;
     const int ncomp_a = (qc.la + 1) * (qc.la + 2) / 2, ncomp_b = (qc.lb + 1) * (qc.lb + 2) / 2;
-    const size_t lds = sizeof(double) * ((size_t)nbd_eff * qc.nbra_max * (ncomp_a + ncomp_b) * nc +
+    const size_t lds = sizeof(double) * ((size_t)nbd_eff * qc.nbra_max * (ncomp_a + ncomp_b) * P.nc_max +
                                          (size_t)nb * qc.nbra_max * ncomp_a * ncomp_b);
     if (qc.d_chunks && lds <= kLdsBudget) {
       BraArgs b;
@@ -956,7 +1118,9 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       b.dens = P.d_dc;
       b.jp = P.d_jp;
       b.kp = P.d_kp;
-      b.n = nc;
+      b.mols = P.d_molinfo;
+      b.bstride = P.ncc_tot;
+      b.n = P.nc_max;
       b.nbd = nbd_eff;
       b.nbj = nb;
       b.with_j = with_j;
@@ -965,7 +1129,6 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       b.thr = mixed_threshold;
       b.pair_q = P.d_pair_q;
       b.dmax = P.d_dmax;
-      b.nseg = (int)P.seg.size();
       launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, b, lds, s);
     } else {
       JKArgs a;
@@ -979,7 +1142,9 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       a.dens = P.d_dc;
       a.jp = P.d_jp;
       a.kp = P.d_kp;
-      a.n = nc;
+      a.mols = P.d_molinfo;
+      a.bstride = P.ncc_tot;
+      a.n = P.nc_max;
       a.nbd = nbd_eff;
       a.nbj = nb;
       a.with_j = with_j;
@@ -992,13 +1157,14 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     }
   }
   hipLaunchKernelGGL(finalize_kernel, dim3(nblocks((size_t)(with_j + with_k) * nb * nn)), dim3(256), 0, s,
-                     P.d_jp, P.d_kp, nc, n, nb, sym ? 1 : 0, with_j, with_k, P.d_s2c_idx, P.d_s2c_coef, out);
+                     P.d_jp, P.d_kp, P.ncc_tot, P.nn_tot, nb, sym ? 1 : 0, with_j, with_k, P.d_molao, P.nmol,
+                     P.d_s2c_idx, P.d_s2c_coef, out);
   HIP_OK(hipGetLastError());
 }
 
 static void int1e(PlanData& P, double* s_out, double* t_out, double* v_out, hipStream_t st) {
   if (!s_out || !t_out || !v_out) throw InvalidArgument("int1e requires three device output buffers");
-  const size_t ncc = (size_t)P.n_cart * P.n_cart;
+  const size_t ncc = (size_t)P.ncc_tot;
   double* tmp = nullptr;
   HIP_OK(hipMalloc(&tmp, sizeof(double) * 3 * ncc));
   hipLaunchKernelGGL(fill_kernel, dim3(nblocks(3 * ncc)), dim3(256), 0, st, tmp, 3 * ncc, 0.0);
@@ -1006,12 +1172,12 @@ static void int1e(PlanData& P, double* s_out, double* t_out, double* v_out, hipS
   for (int c = 0; c < kNumPairClasses; ++c) {
     const int b = P.p1e_begin[c], e = P.p1e_begin[c + 1];
     launch_int1e_class(pl[c][0], pl[c][1], P.d_pairs1e + b, e - b, P.d_prims, P.d_shells, P.d_nuclei, P.nnuc,
-                       P.n_cart, tmp, tmp + ncc, tmp + 2 * ncc, st);
+                       P.d_molinfo, tmp, tmp + ncc, tmp + 2 * ncc, st);
   }
   double* outs[3] = {s_out, t_out, v_out};
   for (int k = 0; k < 3; ++k)
-    hipLaunchKernelGGL(sandwich_kernel, dim3(nblocks((size_t)P.n * P.n)), dim3(256), 0, st, tmp + k * ncc,
-                       P.n_cart, P.n, P.d_s2c_idx, P.d_s2c_coef, outs[k]);
+    hipLaunchKernelGGL(sandwich_kernel, dim3(nblocks((size_t)P.nn_tot)), dim3(256), 0, st, tmp + k * ncc,
+                       P.nn_tot, P.d_molao, P.nmol, P.d_s2c_idx, P.d_s2c_coef, outs[k]);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(st));
   HIP_OK(hipFree(tmp));
@@ -1021,14 +1187,13 @@ static void geometry_vjp(PlanData& P, const double* dens, const double* cj, cons
                          hipStream_t s) {
   if (P.cutoff > 0) throw InvalidArgument("Screened direct J/K geometry AD requires cutoff=0");
   if (!dens || !grad || (!cj && !ck)) throw InvalidArgument("geometry VJP needs a density, a gradient buffer and a cotangent");
-  const int n = P.n, nc = P.n_cart;
-  const size_t ncc = (size_t)nc * nc;
+  const size_t ncc = (size_t)P.ncc_tot;
   ensure_workspace(P, 3);
   const double* src[3] = {dens, cj, ck};
   for (int k = 0; k < 3; ++k)
     if (src[k])
-      hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc)), dim3(256), 0, s, src[k], n, 1, P.d_dc + k * ncc, nc,
-                         P.d_c2s_idx, P.d_c2s_coef, 1);
+      hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc)), dim3(256), 0, s, src[k], P.nn_tot, 1,
+                         P.d_dc + k * ncc, P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, 1);
   for (const QuartetClass& qc : P.classes) {
     GradArgs a;
     a.pairs = P.d_pairs;
@@ -1042,7 +1207,7 @@ static void geometry_vjp(PlanData& P, const double* dens, const double* cj, cons
     a.dens = P.d_dc;
     a.cj = cj ? P.d_dc + ncc : nullptr;
     a.ck = ck ? P.d_dc + 2 * ncc : nullptr;
-    a.n = nc;
+    a.mols = P.d_molinfo;
     a.natm = P.natm;
     a.grad = grad;
     if ((size_t)3 * P.natm * sizeof(double) > kLdsBudget) throw InvalidArgument("too many atoms for the gradient kernel");
@@ -1068,6 +1233,8 @@ static void eri_blocks(PlanData& P, const int32_t* shls, int64_t nq, const int64
     for (int k = 0; k < 4; ++k) {
       sh[k] = shls[q * 4 + k];
       if (sh[k] < 0 || sh[k] >= P.nbas) throw InvalidArgument("ERI block shell index out of range");
+      if (P.seg[seg_first[sh[k]]].mol != P.seg[seg_first[sh[0]]].mol)
+        throw InvalidArgument("ERI block shells must belong to one molecule");
       const int l = P.bas[sh[k] * BAS_SLOTS + ANG_OF], nc = P.bas[sh[k] * BAS_SLOTS + NCTR_OF];
       dims[k] = nc * ncart_h(l);
     }
@@ -1084,7 +1251,7 @@ static void eri_blocks(PlanData& P, const int32_t* shls, int64_t nq, const int64
             // positions of each requested slot's segment inside its pair
             auto pair_of = [&](int x, int y, int& idx, int& first_pos, int& second_pos) {
               const int S = std::max(sg[x], sg[y]), T = std::min(sg[x], sg[y]);
-              idx = P.pair_index.at({S, T});
+              idx = (int)pair_idx(P, S, T);
               const Pair& pr = P.all_pairs[idx];
               if (pr.sa == sg[x] && (pr.sb == sg[y])) {
                 first_pos = x;
@@ -1159,6 +1326,40 @@ int gscf_plan_create(const int32_t* atm, int32_t natm, const int32_t* bas, int32
   });
 }
 
+int gscf_plan_create_batch(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                           int32_t nenv, int32_t nmol, const int32_t* mol_atm, const int32_t* mol_bas, int32_t cart,
+                           double cutoff, void* stream, gscf_plan** plan) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("plan output pointer is NULL");
+    *plan = nullptr;
+    if (nmol < 1 || !mol_atm || !mol_bas) throw InvalidArgument("A batched plan needs per-molecule atm and bas offsets");
+    auto t0 = std::chrono::steady_clock::now();
+    gscf_plan* p = new gscf_plan();
+    try {
+      validate_tables(p->d, atm, natm, bas, nbas, env, nenv, cart, nmol, mol_atm, mol_bas);
+      build_plan(p->d, cutoff, (hipStream_t)stream);
+    } catch (...) {
+      free_plan(p->d);
+      delete p;
+      throw;
+    }
+    p->d.plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *plan = p;
+  });
+}
+
+int gscf_plan_get_molecules(const gscf_plan* plan, int32_t max_mol, int32_t* n, int64_t* offset) {
+  return guarded([&] {
+    if (!plan || !n || !offset) throw InvalidArgument("NULL plan or output");
+    const PlanData& P = plan->d;
+    if (max_mol < P.nmol) throw InvalidArgument("molecule arrays too small");
+    for (int m = 0; m < P.nmol; ++m) {
+      n[m] = P.mols[m].n;
+      offset[m] = P.mols[m].sbase;
+    }
+  });
+}
+
 int gscf_plan_destroy(gscf_plan* plan) {
   return guarded([&] {
     if (!plan) return;
@@ -1179,6 +1380,9 @@ int gscf_plan_get_info(const gscf_plan* plan, gscf_plan_info* info) {
     info->nquartets = P.nquartets;
     info->cutoff = P.cutoff;
     info->plan_ms = P.plan_ms;
+    info->nmol = P.nmol;
+    info->nc_max = P.nc_max;
+    info->nn_total = P.nn_tot;
   });
 }
 

@@ -241,12 +241,13 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
   if (bp.sa == bp.sb) f *= 0.5;
   if (kp.sa == kp.sb) f *= 0.5;
   if (qk.x == qk.y) f *= 0.5;
-  const int n = a.n;
+  const MolInfo mi = a.mols[sA.mol];
+  const int n = mi.n;
   const int oa = sA.ao, ob = sB.ao, oc = sC.ao, od = sD.ao;
   for (int bt = 0; bt < a.nbd; ++bt) {
-    const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+    const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
     if (a.with_j && bt < a.nbj) {
-      double* J = a.jp + (size_t)bt * n * n;
+      double* J = a.jp + (size_t)bt * a.bstride + mi.base;
       double dcd[K::NC][K::ND], dab[K::NA][K::NB];
 #pragma unroll
       for (int ic = 0; ic < K::NC; ++ic)
@@ -282,7 +283,7 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
         }
     }
     if (a.with_k) {
-      double* Kp = a.kp + (size_t)bt * n * n;
+      double* Kp = a.kp + (size_t)bt * a.bstride + mi.base;
       double dbd[K::NB][K::ND], dbc[K::NB][K::NC], dad[K::NA][K::ND], dac[K::NA][K::NC];
 #pragma unroll
       for (int ib = 0; ib < K::NB; ++ib) {
@@ -388,13 +389,14 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
   if (bp.sa == bp.sb) f *= 0.5;
   if (kp.sa == kp.sb) f *= 0.5;
   if (qk.x == qk.y) f *= 0.5;
-  const int n = a.n;
+  const MolInfo mi = a.mols[sA.mol];
+  const int n = mi.n;
   const int oa = sA.ao, ob = sB.ao;
   const int c = sC.ao + ic, d = sD.ao + id;
   for (int bt = 0; bt < a.nbd; ++bt) {
-    const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+    const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
     if (a.with_j && bt < a.nbj) {
-      double* J = a.jp + (size_t)bt * n * n;
+      double* J = a.jp + (size_t)bt * a.bstride + mi.base;
       const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
       double s2 = 0.0;
 #pragma unroll
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
       atomicAdd(J + (size_t)c * n + d, f * s2);
     }
     if (a.with_k) {
-      double* Kp = a.kp + (size_t)bt * n * n;
+      double* Kp = a.kp + (size_t)bt * a.bstride + mi.base;
       double db_d[K::NB], db_c[K::NB];
 #pragma unroll
       for (int ib = 0; ib < K::NB; ++ib) {
@@ -465,8 +467,10 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
                                          double* __restrict__ jrow) {
   using K = Class<LA, LB, LC, LD>;
   constexpr int ROWS = K::NA + K::NB;
-  const int n = a.n;
+  const int ns = a.n;  // LDS row length
   const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
+  const MolInfo mi = a.mols[sA.mol];
+  const int n = mi.n;
   QuartetGeom geo;
   double ABd[3], CDd[3];
   load_geom(sA, sB, sC, sD, geo, ABd, CDd);
@@ -498,9 +502,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
     }
     const int oc = sC.ao, od = sD.ao;
     for (int bt = 0; bt < a.nbd; ++bt) {
-      const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+      const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
       if (a.with_j && bt < a.nbj) {
-        double* J = a.jp + (size_t)bt * n * n;
+        double* J = a.jp + (size_t)bt * a.bstride + mi.base;
         double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
         double dcd[K::NC][K::ND], dab[K::NA][K::NB];
 #pragma unroll
@@ -537,7 +541,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
           }
       }
       if (a.with_k) {
-        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * n;
+        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
         double dbd[K::NB][K::ND], dbc[K::NB][K::NC], dad[K::NA][K::ND], dac[K::NA][K::NC];
 #pragma unroll
         for (int ib = 0; ib < K::NB; ++ib) {
@@ -562,7 +566,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ib = 0; ib < K::NB; ++ib)
 #pragma unroll
               for (int id = 0; id < K::ND; ++id) sk += (double)acc[ia][ib][ic][id] * dbd[ib][id];
-            atomicAdd(kr + (size_t)ia * n + oc + ic, f * sk);
+            atomicAdd(kr + (size_t)ia * ns + oc + ic, f * sk);
           }
 #pragma unroll
           for (int id = 0; id < K::ND; ++id) {
@@ -571,7 +575,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ib = 0; ib < K::NB; ++ib)
 #pragma unroll
               for (int ic = 0; ic < K::NC; ++ic) sk += (double)acc[ia][ib][ic][id] * dbc[ib][ic];
-            atomicAdd(kr + (size_t)ia * n + od + id, f * sk);
+            atomicAdd(kr + (size_t)ia * ns + od + id, f * sk);
           }
         }
 #pragma unroll
@@ -583,7 +587,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ia = 0; ia < K::NA; ++ia)
 #pragma unroll
               for (int id = 0; id < K::ND; ++id) sk += (double)acc[ia][ib][ic][id] * dad[ia][id];
-            atomicAdd(kr + (size_t)(K::NA + ib) * n + oc + ic, f * sk);
+            atomicAdd(kr + (size_t)(K::NA + ib) * ns + oc + ic, f * sk);
           }
 #pragma unroll
           for (int id = 0; id < K::ND; ++id) {
@@ -592,7 +596,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ia = 0; ia < K::NA; ++ia)
 #pragma unroll
               for (int ic = 0; ic < K::NC; ++ic) sk += (double)acc[ia][ib][ic][id] * dac[ia][ic];
-            atomicAdd(kr + (size_t)(K::NA + ib) * n + od + id, f * sk);
+            atomicAdd(kr + (size_t)(K::NA + ib) * ns + od + id, f * sk);
           }
         }
       }
@@ -621,9 +625,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
     }
     const int c = sC.ao + ic, d = sD.ao + id;
     for (int bt = 0; bt < a.nbd; ++bt) {
-      const double* __restrict__ Dm = a.dens + (size_t)bt * n * n;
+      const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
       if (a.with_j && bt < a.nbj) {
-        double* J = a.jp + (size_t)bt * n * n;
+        double* J = a.jp + (size_t)bt * a.bstride + mi.base;
         double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
         const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
         double s2 = 0.0;
@@ -637,7 +641,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
         atomicAdd(J + (size_t)c * n + d, f * s2);
       }
       if (a.with_k) {
-        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * n;
+        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
         double db_d[K::NB], db_c[K::NB];
 #pragma unroll
         for (int ib = 0; ib < K::NB; ++ib) {
@@ -652,8 +656,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             sc += (double)acc[ia][ib] * db_d[ib];
             sd += (double)acc[ia][ib] * db_c[ib];
           }
-          atomicAdd(kr + (size_t)ia * n + c, f * sc);
-          atomicAdd(kr + (size_t)ia * n + d, f * sd);
+          atomicAdd(kr + (size_t)ia * ns + c, f * sc);
+          atomicAdd(kr + (size_t)ia * ns + d, f * sd);
         }
         double da_d[K::NA], da_c[K::NA];
 #pragma unroll
@@ -669,8 +673,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             sc += (double)acc[ia][ib] * da_d[ia];
             sd += (double)acc[ia][ib] * da_c[ia];
           }
-          atomicAdd(kr + (size_t)(K::NA + ib) * n + c, f * sc);
-          atomicAdd(kr + (size_t)(K::NA + ib) * n + d, f * sd);
+          atomicAdd(kr + (size_t)(K::NA + ib) * ns + c, f * sc);
+          atomicAdd(kr + (size_t)(K::NA + ib) * ns + d, f * sd);
         }
       }
     }
@@ -695,9 +699,9 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
   using K = Class<LA, LB, LC, LD>;
   constexpr int ROWS = K::NA + K::NB;
   extern __shared__ double lds[];
-  const int n = a.n;
+  const int ns = a.n;  // LDS row length (widest molecule)
   const BraChunk ch = a.chunks[blockIdx.x];
-  const int krow_size = a.nbd * a.nbra_max * ROWS * n;
+  const int krow_size = a.nbd * a.nbra_max * ROWS * ns;
   double* krow = lds;                      // [nbd][slot][ROWS][n]
   double* jrow = lds + krow_size;          // [nbj][slot][NA][NB]
   const int jrow_size = a.nbj * a.nbra_max * K::NBRA;
@@ -715,11 +719,14 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
     const Pair kp = a.pairs[qk.y];
     bool lowp = false;
     if (a.mixed) {
-      const double* dm = a.dmax;
-      const int ns = a.nseg;
-      const double d6 = fmax(fmax(fmax(dm[bp.sa * ns + bp.sb], dm[kp.sa * ns + kp.sb]),
-                                  fmax(dm[bp.sa * ns + kp.sa], dm[bp.sa * ns + kp.sb])),
-                             fmax(dm[bp.sb * ns + kp.sa], dm[bp.sb * ns + kp.sb]));
+      const int la = a.shells[bp.sa].lseg, lb = a.shells[bp.sb].lseg;
+      const int lc = a.shells[kp.sa].lseg, ld = a.shells[kp.sb].lseg;
+      const MolInfo mi = a.mols[a.shells[bp.sa].mol];
+      const double* dm = a.dmax + mi.dbase;
+      const int nsg = mi.nseg;
+      const double d6 = fmax(fmax(fmax(dm[la * nsg + lb], dm[lc * nsg + ld]),
+                                  fmax(dm[la * nsg + lc], dm[la * nsg + ld])),
+                             fmax(dm[lb * nsg + lc], dm[lb * nsg + ld]));
       lowp = a.pair_q[qk.x] * a.pair_q[qk.y] * d6 < a.thr;
     }
     if (lowp)
@@ -732,14 +739,15 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
   for (int i = threadIdx.x; i < krow_size; i += blockDim.x) {
     const double v = krow[i];
     if (v == 0.0) continue;
-    const int col = i % n;
-    const int r = (i / n) % ROWS;
-    const int slot = (i / (n * ROWS)) % a.nbra_max;
-    const int bt = i / (n * ROWS * a.nbra_max);
+    const int col = i % ns;
+    const int r = (i / ns) % ROWS;
+    const int slot = (i / (ns * ROWS)) % a.nbra_max;
+    const int bt = i / (ns * ROWS * a.nbra_max);
     if (slot >= ch.nbra) continue;
     const Pair bp = a.pairs[ch.bra0 + slot];
+    const MolInfo mi = a.mols[a.shells[bp.sa].mol];
     const int row = r < K::NA ? a.shells[bp.sa].ao + r : a.shells[bp.sb].ao + (r - K::NA);
-    atomicAdd(a.kp + (size_t)bt * n * n + (size_t)row * n + col, v);
+    atomicAdd(a.kp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
   }
   for (int i = threadIdx.x; i < jrow_size; i += blockDim.x) {
     const double v = jrow[i];
@@ -749,8 +757,9 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
     const int bt = i / (K::NBRA * a.nbra_max);
     if (slot >= ch.nbra) continue;
     const Pair bp = a.pairs[ch.bra0 + slot];
+    const MolInfo mi = a.mols[a.shells[bp.sa].mol];
     const int row = a.shells[bp.sa].ao + e / K::NB, col = a.shells[bp.sb].ao + e % K::NB;
-    atomicAdd(a.jp + (size_t)bt * n * n + (size_t)row * n + col, v);
+    atomicAdd(a.jp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
   }
 }
 

@@ -40,10 +40,42 @@ def _stream_handle() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-class DirectJKPlan:
-    """Owns one device plan (gscf_plan) for fixed tables and Schwarz cutoff."""
+def concat_tables(tables):
+    """Concatenate per-molecule libcint tables [(atm, bas, env), ...] into one
+    (atm, bas, env) plus the molecule offsets (mol_atm, mol_bas) of a batched
+    plan: env pointers and atom indices are shifted by the preceding sizes."""
+    atms, bass, envs = [], [], []
+    mol_atm, mol_bas = [0], [0]
+    env_off = 0
+    for atm, bas, env in tables:
+        atm = np.array(atm, dtype=np.int32, copy=True)
+        bas = np.array(bas, dtype=np.int32, copy=True)
+        env = np.asarray(env, dtype=np.float64)
+        atm[:, 1] += env_off            # PTR_COORD
+        atm[:, 4] = np.where(atm[:, 4] > 0, atm[:, 4] + env_off, atm[:, 4])  # PTR_ZETA when set
+        bas[:, 0] += mol_atm[-1]        # ATOM_OF
+        bas[:, 5] += env_off            # PTR_EXP
+        bas[:, 6] += env_off            # PTR_COEFF
+        atms.append(atm)
+        bass.append(bas)
+        envs.append(env)
+        env_off += len(env)
+        mol_atm.append(mol_atm[-1] + len(atm))
+        mol_bas.append(mol_bas[-1] + len(bas))
+    return (np.concatenate(atms), np.concatenate(bass), np.concatenate(envs),
+            np.asarray(mol_atm, dtype=np.int32), np.asarray(mol_bas, dtype=np.int32))
 
-    def __init__(self, atm, bas, env, *, cart: bool, cutoff: float = 0.0):
+
+class DirectJKPlan:
+    """Owns one device plan (gscf_plan) for fixed tables and Schwarz cutoff.
+
+    With mol_atm / mol_bas (offsets of nmol molecules in concatenated tables,
+    see concat_tables) the plan is batched: every molecule is screened and
+    contracted on its own, and densities / results are packed, molecule m's
+    n_m x n_m matrix starting at element mol_offset[m] of a density of
+    nn_total elements (pack / unpack convert)."""
+
+    def __init__(self, atm, bas, env, *, cart: bool, cutoff: float = 0.0, mol_atm=None, mol_bas=None):
         L = _native.lib()
         self.atm = np.ascontiguousarray(atm, dtype=np.int32)
         self.bas = np.ascontiguousarray(bas, dtype=np.int32)
@@ -54,15 +86,52 @@ class DirectJKPlan:
         self.cutoff = float(cutoff)
         _device()
         handle = ctypes.c_void_p()
-        _native.check(L.gscf_plan_create(
-            self.atm.ctypes.data, len(self.atm), self.bas.ctypes.data, len(self.bas),
-            self.env.ctypes.data, len(self.env), int(self.cart), ctypes.c_double(self.cutoff),
-            _stream_handle(), ctypes.byref(handle)))
+        self.mol_atm = np.asarray([0, len(self.atm)] if mol_atm is None else mol_atm, dtype=np.int32)
+        self.mol_bas = np.asarray([0, len(self.bas)] if mol_bas is None else mol_bas, dtype=np.int32)
+        if mol_atm is None and mol_bas is None:
+            _native.check(L.gscf_plan_create(
+                self.atm.ctypes.data, len(self.atm), self.bas.ctypes.data, len(self.bas),
+                self.env.ctypes.data, len(self.env), int(self.cart), ctypes.c_double(self.cutoff),
+                _stream_handle(), ctypes.byref(handle)))
+        else:
+            ma = np.ascontiguousarray(mol_atm, dtype=np.int32)
+            mb = np.ascontiguousarray(mol_bas, dtype=np.int32)
+            if ma.ndim != 1 or ma.shape != mb.shape or len(ma) < 2:
+                raise ValueError("A batched plan needs per-molecule atm and bas offsets")
+            _native.check(L.gscf_plan_create_batch(
+                self.atm.ctypes.data, len(self.atm), self.bas.ctypes.data, len(self.bas),
+                self.env.ctypes.data, len(self.env), len(ma) - 1, ma.ctypes.data, mb.ctypes.data,
+                int(self.cart), ctypes.c_double(self.cutoff), _stream_handle(), ctypes.byref(handle)))
         self._handle = handle
         info = _native.PlanInfo()
         _native.check(L.gscf_plan_get_info(handle, ctypes.byref(info)))
         self.info = {k: getattr(info, k) for k, _ in _native.PlanInfo._fields_}
         self.nao = int(info.n)
+        self.nmol = int(info.nmol)
+        self.nn_total = int(info.nn_total)
+        self.mol_n = np.zeros(self.nmol, dtype=np.int32)
+        self.mol_offset = np.zeros(self.nmol, dtype=np.int64)
+        _native.check(L.gscf_plan_get_molecules(handle, self.nmol, self.mol_n.ctypes.data,
+                                                self.mol_offset.ctypes.data))
+
+    @classmethod
+    def batch(cls, tables, *, cart: bool, cutoff: float = 0.0) -> "DirectJKPlan":
+        """Batched plan over per-molecule tables [(atm, bas, env), ...]."""
+        atm, bas, env, mol_atm, mol_bas = concat_tables(tables)
+        return cls(atm, bas, env, cart=cart, cutoff=cutoff, mol_atm=mol_atm, mol_bas=mol_bas)
+
+    def pack(self, mats) -> torch.Tensor:
+        """Per-molecule (..., n_m, n_m) tensors -> packed (..., nn_total) tensor."""
+        if len(mats) != self.nmol:
+            raise ValueError("one matrix per molecule is required")
+        lead = tuple(mats[0].shape[:-2])
+        return torch.cat([m.reshape(*lead, -1) for m in mats], dim=-1).contiguous()
+
+    def unpack(self, packed: torch.Tensor) -> list:
+        """Packed (..., nn_total) tensor -> list of (..., n_m, n_m) views."""
+        lead = tuple(packed.shape[:-1])
+        return [packed[..., int(o):int(o) + int(n) * int(n)].reshape(*lead, int(n), int(n))
+                for o, n in zip(self.mol_offset, self.mol_n)]
 
     def __del__(self):
         h = getattr(self, "_handle", None)
@@ -81,18 +150,29 @@ class DirectJKPlan:
         (max over the six density blocks it touches) is below this value;
         roots/weights, prefactors and all J/K accumulation stay FP64."""
         d = density
-        if d.dim() == 2:
-            d = d[None]
         n = self.nao
-        if d.dim() != 3 or d.shape[1:] != (n, n) or d.dtype != torch.float64 or not d.is_cuda:
+        if self.nmol == 1:
+            if d.dim() == 2:
+                d = d[None]
+            ok = d.dim() == 3 and tuple(d.shape[1:]) == (n, n)
+            tail = (n, n)
+        else:
+            if d.dim() == 1:
+                d = d[None]
+            ok = d.dim() == 2 and d.shape[1] == self.nn_total
+            tail = (self.nn_total,)
+        if not ok or d.dtype != torch.float64 or not d.is_cuda:
             raise ValueError("Direct J/K shape mismatch")
         d = d.contiguous()
         nb = d.shape[0]
         nops = int(bool(with_j)) + int(bool(with_k))
         if out is None:
-            out = torch.empty((nops, nb, n, n), dtype=torch.float64, device=d.device)
+            out = torch.empty((nops, nb) + tail, dtype=torch.float64, device=d.device)
         if symmetric is None:
-            symmetric = bool(torch.equal(d, d.transpose(1, 2)))
+            if self.nmol == 1:
+                symmetric = bool(torch.equal(d, d.transpose(1, 2)))
+            else:
+                symmetric = all(bool(torch.equal(m, m.transpose(-1, -2))) for m in self.unpack(d))
         flags = (_native.FLAG_SYMMETRIC_DENSITY if symmetric else 0) | (0 if validate else _native.FLAG_NO_VALIDATE)
         thr = 0.0
         if mixed_threshold is not None:
@@ -103,11 +183,21 @@ class DirectJKPlan:
             flags, ctypes.c_double(thr), _stream_handle()))
         return out
 
-    def schwarz(self) -> np.ndarray:
-        """Parent-shell Schwarz bounds (nbas x nbas, lower triangle) of a screened plan."""
-        q = np.zeros((len(self.bas), len(self.bas)))
-        _native.check(_native.lib().gscf_plan_get_schwarz(self._handle, q.ctypes.data))
-        return q
+    def schwarz(self):
+        """Parent-shell Schwarz bounds (nbas x nbas, lower triangle) of a screened
+        plan; a list of per-molecule blocks for a batched plan."""
+        if self.nmol == 1:
+            q = np.zeros((len(self.bas), len(self.bas)))
+            _native.check(_native.lib().gscf_plan_get_schwarz(self._handle, q.ctypes.data))
+            return q
+        counts = np.diff(self.mol_bas).astype(np.int64)
+        flat = np.zeros(int(np.sum(counts ** 2)))
+        _native.check(_native.lib().gscf_plan_get_schwarz(self._handle, flat.ctypes.data))
+        out, o = [], 0
+        for c in counts:
+            out.append(flat[o:o + c * c].reshape(c, c))
+            o += c * c
+        return out
 
     def profile(self, enable: bool) -> None:
         """Start (and reset) or stop per-class HIP-event timing of jk() launches."""
@@ -135,8 +225,8 @@ class DirectJKPlan:
                 return None
             x = x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x))
             x = x.to(dev, torch.float64).contiguous()
-            if tuple(x.shape) != (n, n):
-                raise ValueError("geometry VJP inputs must be (n, n)")
+            if tuple(x.shape) != ((n, n) if self.nmol == 1 else (self.nn_total,)):
+                raise ValueError("geometry VJP inputs must be (n, n) (packed for a batched plan)")
             return x
 
         d, cj, ck = prep(density), prep(cot_j), prep(cot_k)
@@ -147,9 +237,11 @@ class DirectJKPlan:
         return grad
 
     def int1e(self):
-        """(S, T, V) as (n, n) float64 GPU tensors in the plan's AO basis."""
+        """(S, T, V) as (n, n) float64 GPU tensors in the plan's AO basis
+        (packed (nn_total,) tensors for a batched plan)."""
         dev = _device()
-        s, t, v = (torch.empty((self.nao, self.nao), dtype=torch.float64, device=dev) for _ in range(3))
+        shape = (self.nao, self.nao) if self.nmol == 1 else (self.nn_total,)
+        s, t, v = (torch.empty(shape, dtype=torch.float64, device=dev) for _ in range(3))
         _native.check(_native.lib().gscf_int1e(self._handle, s.data_ptr(), t.data_ptr(), v.data_ptr(),
                                                _stream_handle()))
         return s, t, v

@@ -46,6 +46,9 @@ typedef struct {
   int64_t nquartets;    /* retained canonical segmented shell quartets */
   double cutoff;        /* Schwarz cutoff */
   double plan_ms;       /* wall time of gscf_plan_create */
+  int32_t nmol;         /* molecules (1 unless built by gscf_plan_create_batch) */
+  int32_t nc_max;       /* Cartesian AO count of the widest molecule */
+  int64_t nn_total;     /* elements of one packed density: sum over molecules of n_m^2 */
 } gscf_plan_info;
 
 /* Last error message of the calling thread. */
@@ -63,8 +66,29 @@ int gscf_plan_create(const int32_t* atm, int32_t natm, const int32_t* bas, int32
 int gscf_plan_destroy(gscf_plan* plan);
 int gscf_plan_get_info(const gscf_plan* plan, gscf_plan_info* info);
 
+/* Batched plan over nmol independent molecules (no counterpart in the
+ * reference, whose batch rule maps molecules one call at a time,
+ * backends/native/jk.py:253-317).  The tables are the molecules' libcint
+ * tables concatenated; molecule m owns atm rows [mol_atm[m], mol_atm[m+1]) and
+ * bas rows [mol_bas[m], mol_bas[m+1]) (nmol + 1 offsets each, starting at 0),
+ * and its shells must sit on its own atoms.  Pairs, quartets, screening
+ * (Schwarz maximum per molecule) and AO blocks never cross molecules, so the
+ * result for every molecule equals a single-molecule plan of its tables.
+ * Densities, results, S/T/V and cotangents of a batched plan are PACKED:
+ * molecule m's n_m x n_m matrix starts at element offset[m] (from
+ * gscf_plan_get_molecules) of each density, and one density has nn_total
+ * elements.  A single-molecule plan is the case nmol = 1 (offset 0, n x n). */
+int gscf_plan_create_batch(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas,
+                           const double* env, int32_t nenv, int32_t nmol, const int32_t* mol_atm,
+                           const int32_t* mol_bas, int32_t cart, double cutoff, void* stream,
+                           gscf_plan** plan);
+/* AO count n[m] and packed element offset offset[m] of every molecule. */
+int gscf_plan_get_molecules(const gscf_plan* plan, int32_t max_mol, int32_t* n, int64_t* offset);
+
 /* Parent-shell Schwarz bounds q[i*nbas + j] (i >= j) used by the plan
- * (csrc/coulomb_shells.h:60-77); only for plans built with cutoff > 0. */
+ * (csrc/coulomb_shells.h:60-77); only for plans built with cutoff > 0.  For a
+ * batched plan, one nbas_m x nbas_m block per molecule, in molecule order,
+ * with local shell indices. */
 int gscf_plan_get_schwarz(const gscf_plan* plan, double* q);
 
 /* Measurement hooks (no counterpart in the reference): while enabled, each
@@ -76,7 +100,11 @@ int gscf_plan_class_times(gscf_plan* plan, int32_t max_classes, int32_t* classes
                           int64_t* launches, int32_t* count);
 
 /* J/K for nb densities: density F64[nb,n,n] -> out F64[nops,nb,n,n],
- * nops = with_j + with_k (J block first), on `stream`.  Asynchronous. */
+ * nops = with_j + with_k (J block first), on `stream`.  Asynchronous.
+ * For a batched plan, n*n reads nn_total (packed molecules).  With
+ * GSCF_FLAG_NO_VALIDATE the call makes no host synchronisation and issues
+ * kernel launches only (no memset or copy), so it may be captured into a HIP
+ * graph once the workspace for nb exists (after one uncaptured call). */
 int gscf_jk_build(gscf_plan* plan, const double* density, int32_t nb, double* out,
                   int32_t with_j, int32_t with_k, uint32_t flags, double mixed_threshold,
                   void* stream);

@@ -1,0 +1,113 @@
+"""Batched plans: many molecules of uneven size in one plan and one launch per
+quartet class.  A batched plan must reproduce, molecule by molecule, what a
+single-molecule plan of the same tables gives (same pairs, same per-molecule
+Schwarz screening, same mixed-precision decisions), and through that the
+oracle.  Integer plan statistics (retained pairs and quartets) must add up
+exactly; J/K elements agree to 1e-12 (only the order of FP64 atomic additions
+differs); the smallest molecules are also checked against the oracle at the
+1e-10 FP64 parity bar.
+"""
+import numpy as np
+import pytest
+
+from gradscf_amd import molecule as M
+from tests import systems
+
+pytestmark = pytest.mark.gpu
+
+
+def _mols():
+    # uneven sizes, shared generator rules with the 1..8-water batch workload
+    return [M.water_monomer(), M.water_cluster(3, seed=1003, box=8.0), M.water_cluster(1, seed=1001, box=8.0),
+            M.water_cluster(2, seed=1002, box=8.0)]
+
+
+def _tables(basis, cart):
+    return [systems.tables(spec, basis, cart=cart)[2:] for spec in _mols()]
+
+
+def _single_plans(tables, cart, cutoff):
+    from gradscf_amd.jk import DirectJKPlan
+    return [DirectJKPlan(a, b, e, cart=cart, cutoff=cutoff) for a, b, e in tables]
+
+
+@pytest.mark.parametrize("basis,cart,cutoff", [("6-31g*", True, 0.0), ("6-31g*", True, 1e-12),
+                                               ("cc-pvdz", False, 1e-12)])
+def test_batch_matches_single_molecule_plans(basis, cart, cutoff):
+    import torch
+    from gradscf_amd.jk import DirectJKPlan
+    tabs = _tables(basis, cart)
+    singles = _single_plans(tabs, cart, cutoff)
+    plan = DirectJKPlan.batch(tabs, cart=cart, cutoff=cutoff)
+    assert plan.nmol == len(tabs)
+    assert [int(n) for n in plan.mol_n] == [p.nao for p in singles]
+    assert plan.info["npairs"] == sum(p.info["npairs"] for p in singles)
+    assert plan.info["nquartets"] == sum(p.info["nquartets"] for p in singles)
+    dens = [torch.as_tensor(systems.random_density(p.nao, nb=2, seed=10 + i, symmetric=False), device="cuda")
+            for i, p in enumerate(singles)]
+    out = plan.jk(plan.pack(dens))
+    got = plan.unpack(out)
+    for p, d, g in zip(singles, dens, got):
+        ref = p.jk(d)
+        assert float((g - ref).abs().max()) < 1e-12
+
+
+def test_batch_matches_oracle(oracle_mod):
+    import torch
+    from gradscf_amd.jk import DirectJKPlan
+    tabs = _tables("6-31g*", True)[:3]
+    plan = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
+    dens = [systems.random_density(int(n), nb=1, seed=20 + i, symmetric=True) for i, n in enumerate(plan.mol_n)]
+    out = plan.jk(plan.pack([torch.as_tensor(d, device="cuda") for d in dens]))
+    for (atm, bas, env), d, g in zip(tabs, dens, plan.unpack(out)):
+        ref = oracle_mod.direct_jk(atm, bas, env, d, cart=True, cutoff=1e-12, workers=4)
+        assert np.max(np.abs(g.cpu().numpy() - ref)) < 1e-10
+
+
+def test_batch_mixed_precision_matches_single():
+    import torch
+    from gradscf_amd.jk import DirectJKPlan
+    tabs = _tables("6-31g*", True)
+    singles = _single_plans(tabs, True, 1e-12)
+    plan = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
+    dens = [torch.as_tensor(systems.random_density(p.nao, nb=1, seed=30 + i), device="cuda") * 1e-3
+            for i, p in enumerate(singles)]
+    got = plan.unpack(plan.jk(plan.pack(dens), symmetric=True, mixed_threshold=1e-7))
+    for p, d, g in zip(singles, dens, got):
+        ref = p.jk(d, symmetric=True, mixed_threshold=1e-7)
+        assert float((g - ref).abs().max()) < 1e-12
+
+
+@pytest.mark.parametrize("cart", [True, False])
+def test_batch_int1e_and_schwarz_match_single(cart):
+    from gradscf_amd.jk import DirectJKPlan
+    tabs = _tables("cc-pvdz", cart)
+    singles = _single_plans(tabs, cart, 1e-12)
+    plan = DirectJKPlan.batch(tabs, cart=cart, cutoff=1e-12)
+    s, t, v = plan.int1e()
+    for p, a, b, c in zip(singles, plan.unpack(s), plan.unpack(t), plan.unpack(v)):
+        rs, rt, rv = p.int1e()
+        assert float((a - rs).abs().max()) < 1e-13
+        assert float((b - rt).abs().max()) < 1e-13
+        assert float((c - rv).abs().max()) < 1e-12
+    for p, q in zip(singles, plan.schwarz()):
+        assert np.array_equal(p.schwarz(), q)
+
+
+def test_batch_geometry_vjp_matches_single():
+    import torch
+    from gradscf_amd.jk import DirectJKPlan
+    tabs = _tables("6-31g*", True)[:2]
+    singles = _single_plans(tabs, True, 0.0)
+    plan = DirectJKPlan.batch(tabs, cart=True, cutoff=0.0)
+    ds, cjs, cks = [], [], []
+    for i, p in enumerate(singles):
+        ds.append(torch.as_tensor(systems.random_density(p.nao, seed=40 + i)[0], device="cuda"))
+        cjs.append(torch.as_tensor(systems.random_density(p.nao, seed=50 + i)[0], device="cuda"))
+        cks.append(torch.as_tensor(systems.random_density(p.nao, seed=60 + i)[0], device="cuda"))
+    g = plan.jk_geometry_vjp(plan.pack(ds), plan.pack(cjs), plan.pack(cks))
+    o = 0
+    for p, d, cj, ck in zip(singles, ds, cjs, cks):
+        ref = p.jk_geometry_vjp(d, cj, ck)
+        assert float((g[o:o + len(p.atm)] - ref).abs().max()) < 1e-11
+        o += len(p.atm)

@@ -172,3 +172,51 @@ def test_c_abi_rejects_bad_tables_without_gpu():
     rc = L.gscf_direct_jk(atm.ctypes.data, len(atm), bas.ctypes.data, len(bas), env_f.ctypes.data, len(env_f),
                           None, 1, 13, None, 1, ctypes.c_double(0.0), 0, 1, 1, None)
     assert rc == _native.GSCF_INVALID_ARGUMENT and b"workers" in L.gscf_last_error()
+
+
+def test_concat_tables_preserves_each_molecule():
+    """Batched tables: every molecule's shells read the same exponents,
+    coefficients and centre as in its own tables, and sit on its own atoms."""
+    from gradscf_amd.jk import concat_tables
+    specs = [M.water_monomer(), M.water_cluster(2, seed=1002, box=8.0), M.water_cluster(1, seed=1001, box=8.0)]
+    tabs = [systems.tables(s, "6-31g*")[2:] for s in specs]
+    atm, bas, env, mol_atm, mol_bas = concat_tables(tabs)
+    assert list(np.diff(mol_atm)) == [len(t[0]) for t in tabs]
+    assert list(np.diff(mol_bas)) == [len(t[1]) for t in tabs] and mol_bas[-1] == len(bas)
+    for m, (a0, b0, e0) in enumerate(tabs):
+        for i, row in enumerate(b0):
+            r = bas[mol_bas[m] + i]
+            assert mol_atm[m] <= r[0] < mol_atm[m + 1]
+            np.testing.assert_array_equal(env[r[5]:r[5] + r[2]], e0[row[5]:row[5] + row[2]])
+            np.testing.assert_array_equal(env[r[6]:r[6] + r[2] * r[3]], e0[row[6]:row[6] + row[2] * row[3]])
+            ra = atm[r[0]]
+            np.testing.assert_array_equal(env[ra[1]:ra[1] + 3], e0[a0[row[0]][1]:a0[row[0]][1] + 3])
+
+
+def test_c_abi_rejects_bad_batch_offsets_without_gpu():
+    from gradscf_amd import _native
+    from gradscf_amd.jk import concat_tables
+    if not (ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so").is_file():
+        pytest.skip("engine not built")
+    L = _native.lib()
+    tabs = [systems.tables(M.water_monomer(), "6-31g")[2:], systems.tables(M.water_monomer(), "sto-3g")[2:]]
+    atm, bas, env, mol_atm, mol_bas = concat_tables(tabs)
+    h = ctypes.c_void_p()
+
+    def create(a, b, ma, mb, nmol=2):
+        return L.gscf_plan_create_batch(a.ctypes.data, len(a), b.ctypes.data, len(b), env.ctypes.data, len(env),
+                                        nmol, ma.ctypes.data, mb.ctypes.data, 1, ctypes.c_double(0.0), None,
+                                        ctypes.byref(h))
+
+    bad_end = mol_bas.copy()
+    bad_end[-1] -= 1
+    assert create(atm, bas, mol_atm, bad_end) == _native.GSCF_INVALID_ARGUMENT
+    assert b"end at" in L.gscf_last_error()
+    empty = mol_bas.copy()
+    empty[1] = 0
+    assert create(atm, bas, mol_atm, empty) == _native.GSCF_INVALID_ARGUMENT
+    cross = bas.copy()
+    cross[mol_bas[1], 0] = 0  # a shell of molecule 1 placed on an atom of molecule 0
+    assert create(atm, cross, mol_atm, mol_bas) == _native.GSCF_INVALID_ARGUMENT
+    assert b"basis table" in L.gscf_last_error()
+    assert create(atm, bas, mol_atm, mol_bas, nmol=0) == _native.GSCF_INVALID_ARGUMENT
