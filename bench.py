@@ -71,6 +71,214 @@ def cpu_baseline(atm, bas, env, dens, cutoff, builds, j_gpu, k_gpu):
             "ms_per_build": med * 1e3, "max_abs_diff_vs_gpu": {"J": err_j, "K": err_k}}
 
 
+CLASS_LETTERS = "spdf"
+HBM_PROFILE = os.path.join(ROOT, "profiles", "jk_class_kernels_water16_ccpvdz_hbm.txt")
+
+
+def _kernel_class(name: str):
+    """'void gscf::jk_bra_kernel<2, 1, 1, 0>(gscf::BraArgs)' -> ('jk_bra_kernel', 'dpps')."""
+    import re
+    m = re.search(r"(\w+)<(\d), (\d), (\d), (\d)>", name)
+    if not m:
+        return None, None
+    return m.group(1), "".join(CLASS_LETTERS[int(v)] for v in m.groups()[1:])
+
+
+def copy_calibration() -> int:
+    """A device copy of known size (256 MiB read + 256 MiB written, 3 times):
+    the reference reading for the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 1 << 25
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)  # one elementwise kernel: n*8 bytes read, n*8 written
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n * 8, "launches": 3}), flush=True)
+    return 0
+
+
+def summarize_profile(paths) -> int:
+    """Summarise rocprofv3 CSV output (--output-format csv): per-kernel
+    statistics from *kernel_stats.csv and, from *counter_collection.csv, the
+    average of every counter per dispatch of each kernel."""
+    import csv
+    import glob
+    for path in paths:
+        files = sorted(glob.glob(os.path.join(path, "**", "*.csv"), recursive=True))
+        print(f"# {path}")
+        for f in files:
+            if f.endswith("kernel_stats.csv"):
+                rows = list(csv.DictReader(open(f)))
+                print(f"{'kernel':72s} {'calls':>6s} {'total_ms':>9s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}")
+                for r in rows:
+                    print(f"{r['Name'][:72]:72s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:9.3f} "
+                          f"{float(r['AverageNs']) / 1e3:9.2f} {float(r['MinNs']) / 1e3:9.2f} "
+                          f"{float(r['MaxNs']) / 1e3:9.2f} {float(r['Percentage']):6.2f}")
+            elif f.endswith("counter_collection.csv"):
+                acc = {}
+                for r in csv.DictReader(open(f)):
+                    key = (r["Kernel_Name"], r["Counter_Name"])
+                    s = acc.setdefault(key, [0.0, set()])
+                    s[0] += float(r["Counter_Value"])
+                    s[1].add(r["Dispatch_Id"])
+                by = {}
+                for (k, c), (v, ds) in acc.items():
+                    by.setdefault(k, {})[c] = (v / len(ds), len(ds))
+                print("# counters: average per dispatch (n = dispatches)")
+                for k in sorted(by):
+                    kind, cls = _kernel_class(k)
+                    label = f"{kind}[{cls}]" if kind else k[:60]
+                    n = max(d for _, d in by[k].values())
+                    vals = " ".join(f"{c}={v:.6g}" for c, (v, _) in sorted(by[k].items()))
+                    print(f"{label:40s} n={n:<5d} {vals}")
+    return 0
+
+
+def traffic_table(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir) -> int:
+    """HBM bytes per launch of every J/K class kernel: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) from two counter runs of the workload, each
+    scaled by bytes / counter of the copy calibration kernel; printed as a
+    table and as one TRAFFIC_JSON line."""
+    import csv
+    import glob
+
+    def per_kernel(path, counter):
+        acc = {}
+        for f in glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] != counter:
+                    continue
+                s = acc.setdefault(r["Kernel_Name"], {})
+                s[r["Dispatch_Id"]] = s.get(r["Dispatch_Id"], 0.0) + float(r["Counter_Value"])
+        return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+
+    copy_bytes = (1 << 25) * 8
+    factors = {}
+    for counter, path in (("FETCH_SIZE", cal_fetch_dir), ("WRITE_SIZE", cal_write_dir)):
+        vals = [v for k, v in per_kernel(path, counter).items() if "elementwise" in k]
+        if not vals:
+            print(f"no calibration kernel in {path}")
+            return 1
+        factors[counter] = copy_bytes / (max(vals) * 1024.0)
+    fetch = per_kernel(fetch_dir, "FETCH_SIZE")
+    write = per_kernel(write_dir, "WRITE_SIZE")
+    table = {}
+    print(f"calibration (bytes per counter byte): FETCH_SIZE {factors['FETCH_SIZE']:.4f}, "
+          f"WRITE_SIZE {factors['WRITE_SIZE']:.4f}")
+    print(f"{'kernel':32s} {'fetch_MB':>10s} {'write_MB':>10s} {'total_MB':>10s}")
+    for k in sorted(fetch):
+        kind, cls = _kernel_class(k)
+        if kind not in ("jk_bra_kernel", "jk_full_kernel", "jk_split_kernel"):
+            continue
+        fb = fetch[k] * 1024.0 * factors["FETCH_SIZE"]
+        wb = write.get(k, 0.0) * 1024.0 * factors["WRITE_SIZE"]
+        table[cls] = {"kernel": kind, "fetch_bytes": fb, "write_bytes": wb, "bytes_per_launch": fb + wb}
+        print(f"{kind + '[' + cls + ']':32s} {fb / 1e6:10.3f} {wb / 1e6:10.3f} {(fb + wb) / 1e6:10.3f}")
+    print("TRAFFIC_JSON " + json.dumps({"factors": factors, "classes": table}))
+    return 0
+
+
+def profiled_traffic(cls: str):
+    """HBM bytes per launch of the class kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, scaled by the copy
+    calibration factor recorded in the same file); None when absent."""
+    try:
+        text = open(HBM_PROFILE).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        if line.startswith("TRAFFIC_JSON "):
+            table = json.loads(line[len("TRAFFIC_JSON "):])
+            rec = table.get("classes", {}).get(cls)
+            return None if rec is None else rec["bytes_per_launch"]
+    return None
+
+
+def batch_section(thr: float, reps: int = 5) -> dict:
+    """256 seeded water clusters (H2O)_n, n ~ U{1..8} (default_rng(256), cluster
+    i seeded 1000+i, 8 A cube), 6-31G* Cartesian, Schwarz cutoff 1e-12: batched
+    FP64 and mixed SCF with every iteration's J/K for all molecules in one
+    graph replay; J/K throughput at the converged densities, batched mixed vs
+    batched FP64 vs one plain FP64 call per molecule."""
+    import numpy as np
+    import torch
+    from gradscf_amd import molecule as M
+    from gradscf_amd.basis import prepare_basis
+    from gradscf_amd.batch_scf import _GraphJK, rhf_batch
+    from gradscf_amd.jk import DirectJKPlan
+    from gradscf_amd.packing import pack
+
+    specs = M.batch_water_clusters(256, seed=256, base_seed=1000, nmax=8, box=8.0)
+    tabs = [pack(*prepare_basis(s, "6-31g*", cart=True)) for s in specs]
+    nel = [s.nelectron for s in specs]
+    enuc = [s.nuclear_repulsion() for s in specs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    log(f"batch: {plan.nmol} molecules, {int(plan.mol_n.sum())} AO, {plan.info['nquartets']} quartets, "
+        f"plan {plan_s * 1e3:.0f} ms")
+    ref = rhf_batch(plan, nel, enuc, use_graph=True)
+    log(f"batch FP64 SCF: {ref.timings['scf_s']:.3f} s, {len(ref.iteration_s)} iterations, "
+        f"converged {int(ref.converged.sum())}/{plan.nmol}")
+    mix = rhf_batch(plan, nel, enuc, use_graph=True, mixed_threshold=thr)
+    err = np.abs(mix.energies - ref.energies)
+    it_ms = np.asarray(mix.iteration_s) * 1e3
+    log(f"batch mixed SCF: {mix.timings['scf_s']:.3f} s, {len(mix.iteration_s)} iterations, "
+        f"max |dE| {err.max():.2e}, p50 {np.percentile(it_ms, 50):.2f} ms p99 {np.percentile(it_ms, 99):.2f} ms")
+
+    # J/K throughput at the converged densities
+    dens = ref.density
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps * 1e3
+
+    g64 = _GraphJK(plan, None)
+    gmx = _GraphJK(plan, thr)
+    ms64 = timed(lambda: g64(dens))
+    msmx = timed(lambda: gmx(dens))
+    j64 = g64(dens)
+    jmx = gmx(dens)
+    jk_err = max(float((jmx[0] - j64[0]).abs().max()), float((jmx[1] - j64[1]).abs().max()))
+    singles = [DirectJKPlan(a, b, e, cart=True, cutoff=1e-12) for a, b, e in tabs]
+    parts = [d[None].contiguous() for d in plan.unpack(dens)]
+    outs = [torch.empty((2, 1, p.nao, p.nao), dtype=torch.float64, device="cuda") for p in singles]
+
+    def per_molecule():
+        for p, d, o in zip(singles, parts, outs):
+            p.jk(d, symmetric=True, validate=False, out=o)
+
+    ms_single = timed(per_molecule)
+    log(f"batch J/K: FP64 batched {ms64:.2f} ms, mixed batched {msmx:.2f} ms, FP64 per-molecule calls "
+        f"{ms_single:.2f} ms (mixed batched speedup {ms_single / msmx:.2f}x vs plain per-molecule, "
+        f"{ms64 / msmx:.2f}x vs FP64 batched)")
+    return {
+        "workload": "256 water clusters (H2O)_n, n~U{1..8} (default_rng(256), cluster i seed 1000+i, 8 A cube), "
+                    "6-31G* Cartesian, Schwarz 1e-12",
+        "molecules": int(plan.nmol), "ao_total": int(plan.mol_n.sum()), "quartets": int(plan.info["nquartets"]),
+        "plan_ms": plan_s * 1e3, "mixed_threshold": thr,
+        "scf_mixed": {"wall_s": mix.timings["scf_s"], "molecules_per_s": plan.nmol / mix.timings["scf_s"],
+                      "iterations": len(mix.iteration_s), "converged": int(mix.converged.sum()),
+                      "iteration_ms_p50": float(np.percentile(it_ms, 50)),
+                      "iteration_ms_p99": float(np.percentile(it_ms, 99)),
+                      "energy_err_vs_fp64_max": float(err.max()), "energy_err_vs_fp64_mean": float(err.mean())},
+        "scf_fp64": {"wall_s": ref.timings["scf_s"], "molecules_per_s": plan.nmol / ref.timings["scf_s"],
+                     "iterations": len(ref.iteration_s), "converged": int(ref.converged.sum())},
+        "jk": {"fp64_batched_ms": ms64, "mixed_batched_ms": msmx, "fp64_per_molecule_calls_ms": ms_single,
+               "mixed_batched_speedup_vs_fp64_per_molecule": ms_single / msmx,
+               "mixed_batched_speedup_vs_fp64_batched": ms64 / msmx,
+               "mixed_vs_fp64_max_abs_jk_diff": jk_err},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,8 +286,22 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-builds", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision builds and SCF")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-molecule batched workload")
     ap.add_argument("--mixed-threshold", type=float, default=1e-7)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the copy kernel that calibrates the HBM byte counters")
+    ap.add_argument("--summarize-profile", nargs="+", metavar="DIR",
+                    help="print a text summary of rocprofv3 CSV output directories and exit")
+    ap.add_argument("--traffic-table", nargs=4, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE"),
+                    help="HBM bytes per class-kernel launch from rocprofv3 counter CSV directories")
     args = ap.parse_args()
+    if args.summarize_profile:
+        return summarize_profile(args.summarize_profile)
+    if args.traffic_table:
+        return traffic_table(*args.traffic_table)
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the J/K engine runs on a single MI355X "
               "(multi-GPU sharding is a design note, not an implemented path).", file=sys.stderr)
@@ -127,20 +349,27 @@ def main() -> int:
 
     # mixed precision: same workload, FP32 for quartets with Q_ij Q_kl max|D| < threshold
     thr = args.mixed_threshold
-    out_mixed = torch.empty_like(out)
-    for _ in range(args.warmup):
-        plan.jk(dens[None], symmetric=True, validate=False, out=out_mixed, mixed_threshold=thr)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.jk(dens[None], symmetric=True, validate=False, out=out_mixed, mixed_threshold=thr)
-    torch.cuda.synchronize()
-    mixed_ms = (time.perf_counter() - t0) / args.steps * 1e3
-    mixed_err = float((out_mixed - out).abs().max())
-    scf_mixed = rhf(plan, spec.nelectron, spec.nuclear_repulsion(), conv_tol=1e-10,
-                    jk_flags={"mixed_threshold": thr})
-    log(f"mixed (thr {thr:g}): {mixed_ms:.3f} ms/build, max|dJK| {mixed_err:.2e}, "
-        f"E {scf_mixed.energy:.10f} (dE {scf_mixed.energy - scf.energy:.2e}), scf {scf_mixed.timings['scf_s']:.3f} s")
+    mixed = None
+    if not args.no_mixed:
+        out_mixed = torch.empty_like(out)
+        for _ in range(args.warmup):
+            plan.jk(dens[None], symmetric=True, validate=False, out=out_mixed, mixed_threshold=thr)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.jk(dens[None], symmetric=True, validate=False, out=out_mixed, mixed_threshold=thr)
+        torch.cuda.synchronize()
+        mixed_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        mixed_err = float((out_mixed - out).abs().max())
+        scf_mixed = rhf(plan, spec.nelectron, spec.nuclear_repulsion(), conv_tol=1e-10,
+                        jk_flags={"mixed_threshold": thr})
+        log(f"mixed (thr {thr:g}): {mixed_ms:.3f} ms/build, max|dJK| {mixed_err:.2e}, "
+            f"E {scf_mixed.energy:.10f} (dE {scf_mixed.energy - scf.energy:.2e}), "
+            f"scf {scf_mixed.timings['scf_s']:.3f} s")
+        mixed = {"threshold": thr, "ms_per_step": mixed_ms, "builds_per_s": 1e3 / mixed_ms,
+                 "speedup_vs_fp64": ms_per_step / mixed_ms, "max_abs_jk_diff_vs_fp64": mixed_err,
+                 "scf_energy": scf_mixed.energy, "scf_energy_err_vs_fp64": scf_mixed.energy - scf.energy,
+                 "scf_wall_s": scf_mixed.timings["scf_s"], "scf_converged": bool(scf_mixed.converged)}
 
     # per-class kernel durations (HIP events on the launch stream) over a timed pass
     plan.profile(True)
@@ -187,23 +416,22 @@ def main() -> int:
         "scf": {"energy": scf.energy, "converged": bool(scf.converged), "cycles": scf.cycles,
                 "wall_s": scf.timings["scf_s"], "integrals_s": scf.timings["integrals_s"],
                 "jk_builds": scf.timings["jk_builds"]},
-        "mixed": {"threshold": thr, "ms_per_step": mixed_ms, "builds_per_s": 1e3 / mixed_ms,
-                  "speedup_vs_fp64": ms_per_step / mixed_ms, "max_abs_jk_diff_vs_fp64": mixed_err,
-                  "scf_energy": scf_mixed.energy, "scf_energy_err_vs_fp64": scf_mixed.energy - scf.energy,
-                  "scf_wall_s": scf_mixed.timings["scf_s"], "scf_converged": bool(scf_mixed.converged)},
+        "mixed": mixed,
         "plan_ms": plan_s * 1e3,
         "ms_per_step_including_plan": ms_per_step + plan_s * 1e3,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": profiled_traffic(dom["class"]),
                      "kernel": f"jk class ({dom['class']}) quartet kernel",
-                     "note": "FP64 vector-ALU Rys quartet evaluation; algorithmic FLOP_eval of the class / "
-                             "avg HIP-event duration"},
+                     "note": "FP64 vector-ALU Rys quartet evaluation (no MFMA shape); algorithmic FLOP_eval "
+                             "of the class / avg HIP-event duration; traffic = HBM bytes per launch from the "
+                             "committed FETCH_SIZE/WRITE_SIZE summary"},
         "build_roofline": {"flop_eval": tot["flop_eval"], "flop_dig": tot["flop_dig"],
                            "bytes_dig": tot["bytes_dig"], "quartets": tot["quartets"],
                            "class_kernels_ms": total_kernel_ms,
                            "tflops_eval": tot["flop_eval"] / (total_kernel_ms * 1e-3) / 1e12},
         "classes": per_class,
     }
+    result["batch"] = None if args.no_batch else batch_section(thr)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(atm, bas, env, dens.cpu().numpy(), cutoff, args.cpu_builds,
                                               j_gpu, k_gpu)

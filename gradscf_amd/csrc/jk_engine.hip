@@ -172,6 +172,7 @@ struct PlanData {
   double* d_jp = nullptr;
   double* d_kp = nullptr;
   size_t ws_nbd = 0;
+  std::vector<double*> retired;  // outgrown workspaces, freed with the plan
   int* d_flag = nullptr;
   int* h_flag = nullptr;
   // one-electron: all segmented pairs sorted by class, nuclei (x, y, z, Z, molecule)
@@ -1039,6 +1040,7 @@ static void free_plan(PlanData& P) {
   f(P.d_pairs1e); f(P.d_nuclei);
   f(P.d_pair_q); f(P.d_dmax); f(P.d_seg_ao); f(P.d_seg_atom);
   f(P.d_molinfo); f(P.d_molao);
+  for (double* p : P.retired) f(p);
   for (auto& c : P.classes) {
     f(c.d_quartets);
     f(c.d_tasks);
@@ -1048,13 +1050,15 @@ static void free_plan(PlanData& P) {
   if (P.h_flag) (void)hipHostFree(P.h_flag);
 }
 
+// Workspace for nbd densities.  A grown workspace does not free the previous
+// buffers: a HIP graph captured from an earlier build still addresses them, so
+// they are retired and released with the plan.
 static void ensure_workspace(PlanData& P, size_t nbd) {
   if (P.ws_nbd >= nbd) return;
-  auto f = [](double*& p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-  };
-  f(P.d_dc); f(P.d_jp); f(P.d_kp);
+  for (double** p : {&P.d_dc, &P.d_jp, &P.d_kp}) {
+    if (*p) P.retired.push_back(*p);
+    *p = nullptr;
+  }
   const size_t nn = (size_t)P.ncc_tot;
   HIP_OK(hipMalloc(&P.d_dc, sizeof(double) * nn * nbd));
   HIP_OK(hipMalloc(&P.d_jp, sizeof(double) * nn * nbd));

@@ -111,3 +111,59 @@ def test_batch_geometry_vjp_matches_single():
         ref = p.jk_geometry_vjp(d, cj, ck)
         assert float((g[o:o + len(p.atm)] - ref).abs().max()) < 1e-11
         o += len(p.atm)
+
+
+def _batch_rhf_setup(basis="6-31g*"):
+    from gradscf_amd.jk import DirectJKPlan
+    # the last molecule is the geometry of the reference's published RHF energies
+    specs = _mols() + [M.parse_molecule_spec("O 0 0 0; H 0 1.43 1.11; H 0 -1.43 1.11", unit="Bohr")]
+    tabs = [systems.tables(s, basis, cart=True)[2:] for s in specs]
+    plan = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
+    return specs, tabs, plan
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_rhf_matches_single_molecule_rhf(use_graph):
+    """Batched SCF energies equal independent single-molecule runs (1e-9 Eh)."""
+    from gradscf_amd.batch_scf import rhf_batch
+    from gradscf_amd.jk import DirectJKPlan
+    from gradscf_amd.scf import rhf
+    specs, tabs, plan = _batch_rhf_setup()
+    res = rhf_batch(plan, [s.nelectron for s in specs], [s.nuclear_repulsion() for s in specs],
+                    use_graph=use_graph)
+    assert res.converged.all()
+    for spec, (a, b, e), energy in zip(specs, tabs, res.energies):
+        ref = rhf(DirectJKPlan(a, b, e, cart=True, cutoff=1e-12), spec.nelectron, spec.nuclear_repulsion())
+        assert ref.converged
+        assert abs(energy - ref.energy) < 1e-9
+    # reference-published RHF/6-31G* energy (examples/geometry/water_correlated_forces.py:84-87)
+    assert abs(res.energies[-1] - (-76.010492818245)) < 1e-9
+
+
+def test_batch_rhf_mixed_within_micro_hartree():
+    from gradscf_amd.batch_scf import rhf_batch
+    specs, tabs, plan = _batch_rhf_setup()
+    args = ([s.nelectron for s in specs], [s.nuclear_repulsion() for s in specs])
+    ref = rhf_batch(plan, *args, use_graph=False)
+    mixed = rhf_batch(plan, *args, mixed_threshold=1e-7, use_graph=True)
+    assert mixed.converged.all()
+    assert np.max(np.abs(mixed.energies - ref.energies)) < 1e-6
+
+
+def test_graph_replay_matches_eager_every_time():
+    """The captured J/K graph, replayed several times with eager work and
+    synchronisations in between, equals the eager build on every replay."""
+    import torch
+    from gradscf_amd.batch_scf import _GraphJK
+    specs, tabs, plan = _batch_rhf_setup()
+    g = _GraphJK(plan, None)
+    for i in range(4):
+        dens = plan.pack([torch.as_tensor(systems.random_density(int(n), seed=70 + 10 * i + m)[0], device="cuda")
+                          for m, n in enumerate(plan.mol_n)])
+        torch.cuda.synchronize()
+        j, k = g(dens)
+        eager = plan.jk(dens[None], symmetric=True)
+        torch.cuda.synchronize()
+        assert float((j - eager[0, 0]).abs().max()) < 1e-12
+        assert float((k - eager[1, 0]).abs().max()) < 1e-12
+        _ = (dens * 2.0).sum().item()  # eager work between replays
