@@ -15,7 +15,8 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgradscf_jk.so"
 
 EXPORTS = ("gscf_last_error", "gscf_plan_create", "gscf_plan_create_batch", "gscf_plan_get_molecules",
            "gscf_plan_destroy", "gscf_plan_get_info", "gscf_plan_get_schwarz", "gscf_plan_profile", "gscf_plan_class_times",
-           "gscf_jk_build", "gscf_direct_jk", "gscf_eri_blocks", "gscf_int1e", "gscf_jk_geometry_vjp")
+           "gscf_jk_build", "gscf_direct_jk", "gscf_eri_blocks", "gscf_int1e", "gscf_jk_geometry_vjp",
+           "gscf_workspace_bytes")
 
 GSCF_OK, GSCF_INVALID_ARGUMENT, GSCF_INTERNAL = 0, 1, 2
 FLAG_SYMMETRIC_DENSITY = 1
@@ -27,7 +28,7 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("n_cart", ctypes.c_int32), ("nbas", ctypes.c_int32),
                 ("nseg", ctypes.c_int32), ("npairs", ctypes.c_int64), ("nquartets", ctypes.c_int64),
                 ("cutoff", ctypes.c_double), ("plan_ms", ctypes.c_double), ("nmol", ctypes.c_int32),
-                ("nc_max", ctypes.c_int32), ("nn_total", ctypes.c_int64)]
+                ("nc_max", ctypes.c_int32), ("nn_total", ctypes.c_int64), ("device_bytes", ctypes.c_int64)]
 
 
 class NativeError(RuntimeError):
@@ -60,6 +61,7 @@ def lib() -> ctypes.CDLL:
     handle.gscf_eri_blocks.argtypes = [vp, vp, i64, vp, vp, vp]
     handle.gscf_int1e.argtypes = [vp, vp, vp, vp, vp]
     handle.gscf_jk_geometry_vjp.argtypes = [vp, vp, vp, vp, vp, vp]
+    handle.gscf_workspace_bytes.argtypes = [vp, i32, vp, i32, vp, i32, i32, dbl, i32, vp]
     return handle
 
 

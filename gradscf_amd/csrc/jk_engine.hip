@@ -1030,6 +1030,65 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
   HIP_OK(hipStreamSynchronize(s));
 }
 
+// Device bytes a plan holds, with the J/K workspace for nbd densities and
+// quartet-list / chunk sizes as given (exact after build_plan).
+static int64_t plan_bytes(const PlanData& P, int64_t nq, int64_t ntasks, int64_t nchunks, int64_t nbd) {
+  int64_t b = 0;
+  b += (int64_t)P.seg.size() * (sizeof(ShellXYZ) + 3 * sizeof(int));         // shells, seg_ao, seg_atom
+  b += (int64_t)std::max<size_t>(P.prims.size(), 1) * sizeof(PrimPair);
+  b += (int64_t)P.all_pairs.size() * 2 * sizeof(Pair);                        // all pairs, 1e pairs
+  b += (int64_t)P.pairs.size() * (sizeof(Pair) + sizeof(double));             // screened pairs, pair_q
+  b += (int64_t)P.n_cart * 2 * (sizeof(int) + sizeof(double)) + (int64_t)P.n * 3 * (sizeof(int) + sizeof(double));
+  b += (int64_t)P.nmol * sizeof(MolInfo) + (int64_t)(P.nmol + 1) * sizeof(MolAO);
+  b += std::max<int64_t>(P.nd_tot, 1) * sizeof(double);                       // shell-block density maxima
+  b += (int64_t)5 * sizeof(double) * std::max(P.natm, 1);                     // nuclei
+  b += nq * sizeof(int2) + ntasks * sizeof(SubTask) + nchunks * sizeof(BraChunk);
+  b += 3 * P.ncc_tot * (int64_t)sizeof(double) * nbd;                         // Cartesian D, Jp, Kp
+  return b;
+}
+
+static int64_t plan_bytes(const PlanData& P, int64_t nbd) {
+  int64_t nq = 0, nt = 0, nc = 0;
+  for (const QuartetClass& c : P.classes) {
+    nq += c.nq;
+    nt += (int64_t)c.tasks.size();
+    nc += (int64_t)c.chunks.size();
+  }
+  return plan_bytes(P, nq, nt, nc, nbd);
+}
+
+// Host-only upper bound of the device bytes of a plan plus one build of nb
+// densities (nonsymmetric K doubles the workspace): the exact size of every
+// table, with every intra-molecule canonical pair quartet retained (no
+// Schwarz screening is evaluated), at most one chunk per quartet and one
+// sub-task per pair of (class, bucket) runs.  Transient buffers (Schwarz blocks, one-electron staging) are freed
+// before a build and do not add to it.
+static int64_t forecast_bytes(PlanData& P, int nb) {
+  build_segments(P);
+  P.mol_pair0.assign(P.nmol + 1, 0);
+  int64_t nq = 0;
+  for (int m = 0; m < P.nmol; ++m) {
+    int64_t live = 0;
+    for (int S = P.mol_seg[m]; S < P.mol_seg[m + 1]; ++S)
+      for (int T = P.mol_seg[m]; T <= S; ++T) {
+        int a = S, c = T;
+        if (P.seg[c].l > P.seg[a].l) std::swap(a, c);
+        Pair pr;
+        make_pair(P, a, c, pr);
+        P.all_pairs.push_back(pr);
+        if (pr.np > 0) {
+          P.pairs.push_back(pr);
+          ++live;
+        }
+      }
+    nq += live * (live + 1) / 2;
+  }
+  // sub-tasks pair a bra run with a ket run of one molecule: at most
+  // (pair classes x primitive-count buckets)^2 per molecule
+  const int64_t runs = (int64_t)kNumPairClasses * 8;
+  return plan_bytes(P, nq, std::min<int64_t>(nq, runs * runs * P.nmol), nq, 2 * (int64_t)nb);
+}
+
 static void free_plan(PlanData& P) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
@@ -1364,6 +1423,18 @@ int gscf_plan_get_molecules(const gscf_plan* plan, int32_t max_mol, int32_t* n, 
   });
 }
 
+int gscf_workspace_bytes(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                         int32_t nenv, int32_t cart, double cutoff, int32_t nb, int64_t* bytes) {
+  return guarded([&] {
+    if (!bytes) throw InvalidArgument("NULL output");
+    if (nb < 1) throw InvalidArgument("Direct J/K shape mismatch");
+    if (!std::isfinite(cutoff) || cutoff < 0) throw InvalidArgument("Invalid Schwarz cutoff");
+    PlanData P;
+    validate_tables(P, atm, natm, bas, nbas, env, nenv, cart);
+    *bytes = forecast_bytes(P, nb);
+  });
+}
+
 int gscf_plan_destroy(gscf_plan* plan) {
   return guarded([&] {
     if (!plan) return;
@@ -1387,6 +1458,7 @@ int gscf_plan_get_info(const gscf_plan* plan, gscf_plan_info* info) {
     info->nmol = P.nmol;
     info->nc_max = P.nc_max;
     info->nn_total = P.nn_tot;
+    info->device_bytes = plan_bytes(P, (int64_t)P.ws_nbd);
   });
 }
 

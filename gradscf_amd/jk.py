@@ -369,3 +369,28 @@ class NativeDirectBasis:
     def get_jk(self, density, *, screening_threshold=0.0, workers=1, with_j=True, with_k=True, omega=0.0):
         return self.plan.get_jk(self.parameters, density, screening_threshold=screening_threshold,
                                 workers=workers, with_j=with_j, with_k=with_k, omega=omega)
+
+
+def native_jk_workspace_bytes(provider, workers, cutoff, nb: int = 1):
+    """Device-memory forecast of a direct J/K plan plus one build of nb
+    densities, before any GPU work (mirrors the reference's
+    native_jk_workspace_bytes, src/gradscf/integrals/backends/native/jk.py:14-68:
+    same arguments, None for a provider without a complete shell topology).
+    The bound counts every canonical pair quartet as retained; `workers` is
+    validated and otherwise unused (one GPU does the work)."""
+    if isinstance(workers, bool) or not isinstance(workers, Integral) or workers < 1:
+        raise ValueError("workers must be a positive integer.")
+    plan = getattr(provider, "plan", None)
+    params = getattr(provider, "parameters", None)
+    topology = getattr(plan, "topology", None)
+    if topology is None or params is None or not hasattr(topology, "nao"):
+        return None
+    atm, bas, env = pack(topology, params)
+    atm = np.ascontiguousarray(atm, dtype=np.int32)
+    bas = np.ascontiguousarray(bas, dtype=np.int32)
+    env = np.ascontiguousarray(env, dtype=np.float64)
+    out = ctypes.c_int64()
+    _native.check(_native.lib().gscf_workspace_bytes(
+        atm.ctypes.data, len(atm), bas.ctypes.data, len(bas), env.ctypes.data, len(env),
+        int(bool(topology.cart)), ctypes.c_double(float(cutoff)), int(nb), ctypes.byref(out)))
+    return int(out.value)

@@ -49,7 +49,19 @@ typedef struct {
   int32_t nmol;         /* molecules (1 unless built by gscf_plan_create_batch) */
   int32_t nc_max;       /* Cartesian AO count of the widest molecule */
   int64_t nn_total;     /* elements of one packed density: sum over molecules of n_m^2 */
+  int64_t device_bytes; /* device memory held by the plan and its current workspace */
 } gscf_plan_info;
+
+/* Device-memory forecast before any GPU work (the reference's
+ * native_jk_workspace_bytes, backends/native/jk.py:14-68, which the SCF input
+ * assembly uses to reject a direct SCF up front, scf/inputs/assembly.py:145-152):
+ * validates the tables like gscf_plan_create and returns in *bytes an upper
+ * bound of the device memory of a plan plus one gscf_jk_build of nb densities
+ * (every canonical pair quartet counted as retained; cutoff is only
+ * validated).  Host-only: callable without a GPU. */
+int gscf_workspace_bytes(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas,
+                         const double* env, int32_t nenv, int32_t cart, double cutoff, int32_t nb,
+                         int64_t* bytes);
 
 /* Last error message of the calling thread. */
 const char* gscf_last_error(void);

@@ -313,3 +313,21 @@ def test_screened_geometry_vjp_rejected():
     d = _cuda(np.eye(13))
     with pytest.raises(ValueError, match="cutoff=0"):
         plan.jk_geometry_vjp(d, d, None)
+
+
+@pytest.mark.parametrize("basis,cart,cutoff", [("6-31g*", True, 1e-12), ("cc-pvdz", False, 0.0)])
+def test_workspace_forecast_bounds_plan_memory(basis, cart, cutoff):
+    from gradscf_amd.basis import prepare_basis
+    from gradscf_amd.jk import DirectJKPlan, NativeDirectBasis, make_plan, native_jk_workspace_bytes
+    from gradscf_amd.packing import pack
+    topo, par = prepare_basis(systems.water_dimer(), basis, cart=cart)
+    atm, bas, env = pack(topo, par)
+    plan = DirectJKPlan(atm, bas, env, cart=cart, cutoff=cutoff)
+    d = _cuda(systems.random_density(plan.nao, nb=2, seed=5, symmetric=False))
+    plan.jk(d)
+    import ctypes
+    from gradscf_amd import _native
+    info = _native.PlanInfo()
+    _native.check(_native.lib().gscf_plan_get_info(plan._handle, ctypes.byref(info)))
+    forecast = native_jk_workspace_bytes(NativeDirectBasis(make_plan(topo), par), 1, cutoff, nb=2)
+    assert 0 < info.device_bytes <= forecast

@@ -220,3 +220,24 @@ def test_c_abi_rejects_bad_batch_offsets_without_gpu():
     assert create(atm, cross, mol_atm, mol_bas) == _native.GSCF_INVALID_ARGUMENT
     assert b"basis table" in L.gscf_last_error()
     assert create(atm, bas, mol_atm, mol_bas, nmol=0) == _native.GSCF_INVALID_ARGUMENT
+
+
+def test_workspace_forecast_without_gpu():
+    """The memory forecast (the reference's native_jk_workspace_bytes) runs on
+    a CPU-only host, validates like plan creation, grows with the density count
+    and is None without a complete topology."""
+    if not (ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so").is_file():
+        pytest.skip("engine not built")
+    from gradscf_amd.jk import NativeDirectBasis, make_plan, native_jk_workspace_bytes
+    topo, par = prepare_basis(M.water_cluster(4, seed=4, box=6.0), "6-31g*", cart=True)
+    prov = NativeDirectBasis(make_plan(topo), par)
+    one = native_jk_workspace_bytes(prov, 1, 1e-12)
+    four = native_jk_workspace_bytes(prov, 1, 1e-12, nb=4)
+    n = topo.nao()
+    assert one > 3 * 2 * n * n * 8
+    assert four - one == 3 * 2 * 3 * n * n * 8  # six more n x n FP64 workspace matrices per extra density
+    assert native_jk_workspace_bytes(object(), 1, 0.0) is None
+    with pytest.raises(ValueError):
+        native_jk_workspace_bytes(prov, 0, 0.0)
+    with pytest.raises(ValueError):
+        native_jk_workspace_bytes(prov, 1, -1.0)
