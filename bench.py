@@ -288,6 +288,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision builds and SCF")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-molecule batched workload")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default per-class schedules")
     ap.add_argument("--mixed-threshold", type=float, default=1e-7)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the copy kernel that calibrates the HBM byte counters")
@@ -330,9 +331,24 @@ def main() -> int:
     log(f"plan: {plan.info['npairs']} segmented pairs, {plan.info['nquartets']} quartets, {plan_s * 1e3:.1f} ms")
 
     scf = rhf(plan, spec.nelectron, spec.nuclear_repulsion(), conv_tol=1e-10)
+    dens = scf.density.contiguous()
+    tuning = None
+    if not args.no_autotune:
+        from gradscf_amd.autotune import autotune
+        t0 = time.perf_counter()
+        tuning = autotune(lambda: DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff),
+                          lambda p: p.jk(dens[None], symmetric=True, validate=False), reps=3, log=log)
+        tune_s = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan = DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff)
+        torch.cuda.synchronize()
+        plan_s = time.perf_counter() - t0
+        log(f"autotune {tune_s:.1f} s; tuned plan {plan_s * 1e3:.1f} ms")
+        scf = rhf(plan, spec.nelectron, spec.nuclear_repulsion(), conv_tol=1e-10)
+        dens = scf.density.contiguous()
     log(f"RHF E = {scf.energy:.10f} converged={scf.converged} cycles={scf.cycles} "
         f"scf {scf.timings['scf_s']:.3f} s ({scf.timings['jk_builds']} J/K builds)")
-    dens = scf.density.contiguous()
 
     out = torch.empty((2, 1, plan.nao, plan.nao), dtype=torch.float64, device="cuda")
     for _ in range(args.warmup):
@@ -430,6 +446,7 @@ def main() -> int:
                            "class_kernels_ms": total_kernel_ms,
                            "tflops_eval": tot["flop_eval"] / (total_kernel_ms * 1e-3) / 1e12},
         "classes": per_class,
+        "autotune": tuning,
     }
     result["batch"] = None if args.no_batch else batch_section(thr)
     if not args.no_cpu_baseline:

@@ -16,7 +16,8 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libgradscf_jk.so"
 EXPORTS = ("gscf_last_error", "gscf_plan_create", "gscf_plan_create_batch", "gscf_plan_get_molecules",
            "gscf_plan_destroy", "gscf_plan_get_info", "gscf_plan_get_schwarz", "gscf_plan_profile", "gscf_plan_class_times",
            "gscf_jk_build", "gscf_direct_jk", "gscf_eri_blocks", "gscf_int1e", "gscf_jk_geometry_vjp",
-           "gscf_workspace_bytes")
+           "gscf_workspace_bytes", "gscf_int1e_gradient", "gscf_tuning_set", "gscf_tuning_get",
+           "gscf_tuning_reset")
 
 GSCF_OK, GSCF_INVALID_ARGUMENT, GSCF_INTERNAL = 0, 1, 2
 FLAG_SYMMETRIC_DENSITY = 1
@@ -29,6 +30,11 @@ class PlanInfo(ctypes.Structure):
                 ("nseg", ctypes.c_int32), ("npairs", ctypes.c_int64), ("nquartets", ctypes.c_int64),
                 ("cutoff", ctypes.c_double), ("plan_ms", ctypes.c_double), ("nmol", ctypes.c_int32),
                 ("nc_max", ctypes.c_int32), ("nn_total", ctypes.c_int64), ("device_bytes", ctypes.c_int64)]
+
+
+class ClassTuning(ctypes.Structure):
+    _fields_ = [("chunk_items", ctypes.c_int32), ("bras_max", ctypes.c_int32), ("lds_pack", ctypes.c_int32),
+                ("rys_lds", ctypes.c_int32), ("target_prim", ctypes.c_int32)]
 
 
 class NativeError(RuntimeError):
@@ -61,6 +67,10 @@ def lib() -> ctypes.CDLL:
     handle.gscf_eri_blocks.argtypes = [vp, vp, i64, vp, vp, vp]
     handle.gscf_int1e.argtypes = [vp, vp, vp, vp, vp]
     handle.gscf_jk_geometry_vjp.argtypes = [vp, vp, vp, vp, vp, vp]
+    handle.gscf_int1e_gradient.argtypes = [vp, vp, vp, vp, vp]
+    handle.gscf_tuning_set.argtypes = [i32, vp]
+    handle.gscf_tuning_get.argtypes = [i32, vp]
+    handle.gscf_tuning_reset.argtypes = []
     handle.gscf_workspace_bytes.argtypes = [vp, i32, vp, i32, vp, i32, i32, dbl, i32, vp]
     return handle
 

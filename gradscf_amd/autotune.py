@@ -1,0 +1,114 @@
+"""Autotuner for the per-class J/K schedules.
+
+Each canonical quartet class runs as its own kernel launch, so its schedule
+(bra-chunk size, bras per chunk, LDS packing, Rys-table placement, primitive
+split) can be chosen independently of the other classes.  The tuner times
+every class under candidate schedules with HIP events on the launch stream
+(coordinate descent from the defaults, one parameter at a time), keeps the
+fastest schedule per class and installs the result in the engine's
+process-wide tuning table, which plans copy when they are created.
+
+Results never change J/K beyond the order of FP64 additions (tested).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import asdict, dataclass
+
+from . import _native
+
+PAIR_CLASSES = ((0, 0), (1, 0), (1, 1), (2, 0), (2, 1), (2, 2))
+LETTERS = "spdf"
+
+
+@dataclass(frozen=True)
+class Schedule:
+    chunk_items: int = 4096
+    bras_max: int = 16
+    lds_pack: int = 32 * 1024
+    rys_lds: int = 0
+    target_prim: int = 48
+
+
+SEARCH = {
+    "rys_lds": (0, 1),
+    "chunk_items": (2048, 4096, 8192),
+    "bras_max": (8, 16, 32),
+    "target_prim": (24, 48, 96),
+}
+
+
+def class_index(cls) -> int:
+    """(la, lb, lc, ld) -> index of the engine's tuning table."""
+    cb = PAIR_CLASSES.index((cls[0], cls[1]))
+    ck = PAIR_CLASSES.index((cls[2], cls[3]))
+    return cb * (cb + 1) // 2 + ck
+
+
+def class_name(cls) -> str:
+    return "".join(LETTERS[v] for v in cls)
+
+
+def set_schedule(schedule: Schedule, cls=None) -> None:
+    t = _native.ClassTuning(**asdict(schedule))
+    _native.check(_native.lib().gscf_tuning_set(-1 if cls is None else class_index(cls), ctypes.byref(t)))
+
+
+def get_schedule(cls) -> Schedule:
+    t = _native.ClassTuning()
+    _native.check(_native.lib().gscf_tuning_get(class_index(cls), ctypes.byref(t)))
+    return Schedule(*(getattr(t, f) for f, _ in _native.ClassTuning._fields_))
+
+
+def reset() -> None:
+    _native.check(_native.lib().gscf_tuning_reset())
+
+
+def _class_times(make_plan, run, reps):
+    import torch
+    plan = make_plan()
+    run(plan)  # warm-up (workspace, caches)
+    torch.cuda.synchronize()
+    plan.profile(True)
+    for _ in range(reps):
+        run(plan)
+    torch.cuda.synchronize()
+    times = {cls: ms / max(n, 1) for cls, ms, n in plan.class_times()}
+    plan.profile(False)
+    return times
+
+
+def autotune(make_plan, run, *, reps: int = 3, search=None, log=None) -> dict:
+    """Choose a schedule per class.
+
+    make_plan(): builds a DirectJKPlan under the currently installed tuning.
+    run(plan): one J/K build of the workload on that plan.
+    Returns {class name: {"schedule": dict, "ms": best avg ms, "default_ms": ms}}
+    and leaves the chosen schedules installed."""
+    search = SEARCH if search is None else search
+    reset()
+    base = Schedule()
+    best_t = _class_times(make_plan, run, reps)
+    default_t = dict(best_t)
+    best_s = {cls: base for cls in best_t}
+    for param, values in search.items():
+        current = {cls: best_s[cls] for cls in best_t}
+        for v in values:
+            if all(getattr(current[c], param) == v for c in current):
+                continue
+            # every class tries value v of this parameter on top of its current best
+            for cls in current:
+                set_schedule(Schedule(**{**asdict(current[cls]), param: v}), cls)
+            t = _class_times(make_plan, run, reps)
+            for cls, ms in t.items():
+                if cls in best_t and ms < best_t[cls]:
+                    best_t[cls] = ms
+                    best_s[cls] = Schedule(**{**asdict(current[cls]), param: v})
+            if log:
+                log(f"autotune {param}={v}: {sum(t.values()):.3f} ms over classes")
+        for cls in best_s:
+            set_schedule(best_s[cls], cls)
+    for cls in best_s:
+        set_schedule(best_s[cls], cls)
+    return {class_name(cls): {"schedule": asdict(best_s[cls]), "ms": best_t[cls], "default_ms": default_t[cls]}
+            for cls in best_s}

@@ -99,6 +99,7 @@ struct BraArgs {
   double thr;
   const double* pair_q;  // Schwarz bound per pair
   const double* dmax;    // per molecule [nseg][nseg] max |D| over shell blocks (all batches)
+  int tab_lds;           // stage the class's Rys table slice in LDS (after the slabs)
 };
 
 // Explicit quartet list writing contracted Cartesian blocks:

@@ -25,12 +25,16 @@ __device__ static const double rys_tab[] = {
 // ---------------------------------------------------------------------------
 // Rys roots (s = t^2) and weights for weight exp(-x t^2) on [0,1]
 // ---------------------------------------------------------------------------
+// `tab` is the N-root slice of the table (rys_tab + rys_table_offset[N - 1],
+// or a copy of it staged in LDS): RYS_NINT intervals of (RYS_DEG + 1) x 2N
+// Chebyshev coefficients, roots then weights.
 template <int N>
-__device__ __forceinline__ void rys_roots(double x, double* __restrict__ s, double* __restrict__ w) {
+__device__ __forceinline__ void rys_roots_tab(const double* __restrict__ tab, double x, double* __restrict__ s,
+                                              double* __restrict__ w) {
   if (x < RYS_XMAX) {
     const int k = (int)(x * (1.0 / RYS_H));
     const double y = x - (RYS_H * k + 0.5 * RYS_H);
-    const double* __restrict__ c = rys_tab + rys_table_offset[N - 1] + k * ((RYS_DEG + 1) * 2 * N);
+    const double* __restrict__ c = tab + k * ((RYS_DEG + 1) * 2 * N);
     double b1[2 * N], b2[2 * N];
 #pragma unroll
     for (int f = 0; f < 2 * N; ++f) { b1[f] = 0.0; b2[f] = 0.0; }
@@ -60,6 +64,14 @@ __device__ __forceinline__ void rys_roots(double x, double* __restrict__ s, doub
     }
   }
 }
+
+template <int N>
+__device__ __forceinline__ void rys_roots(double x, double* __restrict__ s, double* __restrict__ w) {
+  rys_roots_tab<N>(rys_tab + rys_table_offset[N - 1], x, s, w);
+}
+
+// Doubles in the N-root slice of the table.
+constexpr int rys_slice_size(int n) { return RYS_NINT * (RYS_DEG + 1) * 2 * n; }
 
 // ---------------------------------------------------------------------------
 // Cartesian components in libcint order (descending x, then descending y)

@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "_generated/rys_table.inc"
 #include "gradscf_jk.h"
 #include "gscf_args.h"
 
@@ -114,8 +115,37 @@ struct MolAO {
   int32_t n, nc, ao0, aoc0, seg0, nseg;
 };
 
+// Per-class schedule parameters (see gscf_class_tuning); plans copy the
+// process-wide table when they are created.
+constexpr int kNumQuartetClasses = kNumPairClasses * (kNumPairClasses + 1) / 2;
+static gscf_class_tuning g_tuning[kNumQuartetClasses] = {};
+static bool g_tuning_init = false;
+
+static gscf_class_tuning default_tuning() {
+  gscf_class_tuning t;
+  t.chunk_items = 4096;   // lane-items per workgroup before a chunk is closed
+  t.bras_max = 16;        // bra pairs (LDS slots) per chunk
+  t.lds_pack = 32 * 1024; // LDS bytes the slots of one workgroup may use
+  t.rys_lds = 0;          // stage the Rys table slice in LDS
+  t.target_prim = 48;     // primitive quartets per work item (primitive split)
+  return t;
+}
+
+static void ensure_tuning() {
+  if (g_tuning_init) return;
+  for (auto& t : g_tuning) t = default_tuning();
+  g_tuning_init = true;
+}
+
+static size_t rys_slice_bytes(int nr) {
+  return sizeof(double) * ((size_t)RYS_NINT * (RYS_DEG + 1) * 2 * nr + 2);  // + alignment pad
+}
+
 struct QuartetClass {
   int la, lb, lc, ld;
+  int tuning_index = 0;
+  int tab_lds = 0;          // Rys table slice staged in LDS
+  size_t tab_bytes = 0;
   int64_t nq = 0, nitems = 0;
   std::vector<SubTask> tasks;
   SubTask* d_tasks = nullptr;
@@ -126,14 +156,13 @@ struct QuartetClass {
   int nbra_max = 0;
 };
 
-// LDS of the bra-major kernel per workgroup: slots are packed up to 32 KB so
-// that several workgroups stay resident per CU; a single slot may use 64 KB.
-constexpr size_t kLdsPack = 32 * 1024;
+// LDS of the bra-major kernel per workgroup: slots are packed up to the
+// class's lds_pack so that several workgroups stay resident per CU; a single
+// slot (with the table, when staged) may use up to 64 KB.
 constexpr size_t kLdsBudget = 64 * 1024;
-constexpr int64_t kChunkItems = 4096;   // lane-items per workgroup before a chunk is closed
-constexpr int kChunkBrasMax = 16;
 
 struct PlanData {
+  gscf_class_tuning tuning[kNumQuartetClasses];
   int cart = 1, n = 0, n_cart = 0, nbas = 0, natm = 0;
   double cutoff = 0.0, plan_ms = 0.0;
   std::vector<int> atm, bas, ao, ao_cart;  // ao/ao_cart: global AO offsets of every shell
@@ -210,6 +239,10 @@ GSCF_CLASSES(X)
 void launch_int1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
                         const ShellXYZ* shells, const double* nuclei, int nnuc, const MolInfo* mols, double* s,
                         double* t, double* v, hipStream_t st);
+void launch_grad1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
+                         const ShellXYZ* shells, const int* seg_atom, const double* nuclei, int nnuc,
+                         const MolInfo* mols, const double* dens, const double* wdens, int natm, double* grad,
+                         hipStream_t st);
 
 #define X(A, B, C, D) void launch_jk_bra_c##A##B##C##D(const BraArgs& a, size_t lds, hipStream_t s);
 GSCF_CLASSES(X)
@@ -439,6 +472,8 @@ static void validate_tables(PlanData& P, const int32_t* atm, int natm, const int
                             const int32_t* mol_bas = nullptr) {
   if (!atm || !bas || !env || natm < 0 || nbas < 0 || nenv < 20 || (cart != 0 && cart != 1))
     throw InvalidArgument("Invalid libcint table dimensions or operator");
+  ensure_tuning();
+  std::copy(g_tuning, g_tuning + kNumQuartetClasses, P.tuning);
   if (nmol < 1 || (nmol > 1 && (!mol_atm || !mol_bas)))
     throw InvalidArgument("A batched plan needs per-molecule atm and bas offsets");
   P.nmol = nmol;
@@ -885,12 +920,15 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
   }
   // quartet classes: canonical (bra run, ket run) sub-tasks with materialised quartet lists
   static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
-  constexpr int kTargetPrim = 48;  // primitive quartets per work item
   P.nquartets = 0;
   for (int cb = 0; cb < kNumPairClasses; ++cb)
     for (int ck = 0; ck <= cb; ++ck) {
       QuartetClass qc;
       qc.la = pl[cb][0]; qc.lb = pl[cb][1]; qc.lc = pl[ck][0]; qc.ld = pl[ck][1];
+      qc.tuning_index = cb * (cb + 1) / 2 + ck;
+      const gscf_class_tuning& tu = P.tuning[qc.tuning_index];
+      const int64_t kTargetPrim = tu.target_prim, kChunkItems = tu.chunk_items;
+      const int kChunkBrasMax = tu.bras_max;
       qc.nq = 0;
       qc.nitems = 0;
       struct Pending { std::vector<int64_t> qoff; int bra0, ket0, nbra; int64_t q0, nq; };
@@ -942,8 +980,15 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
         const int rows = ncomp_a + ncomp_b;
         const int split = (ncomp_a * ncomp_b * ncomp_c * ncomp_d > 36) ? ncomp_c * ncomp_d : 1;
         const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)ncomp_a * ncomp_b);
+        const int nr = (qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1;
+        size_t pack = (size_t)tu.lds_pack;
+        if (tu.rys_lds && per_slot + rys_slice_bytes(nr) <= kLdsBudget) {
+          qc.tab_lds = 1;
+          qc.tab_bytes = rys_slice_bytes(nr);
+          pack = pack > qc.tab_bytes + per_slot ? pack - qc.tab_bytes : per_slot;
+        }
         qc.nbra_max = per_slot > kLdsBudget ? 0
-                      : (int)std::max<size_t>(1, std::min<size_t>(kChunkBrasMax, kLdsPack / per_slot));
+                      : (int)std::max<size_t>(1, std::min<size_t>((size_t)kChunkBrasMax, pack / per_slot));
         for (size_t ti = 0; qc.nbra_max >= 1 && ti < pend.size(); ++ti) {
           const Pending& pd = pend[ti];
           const SubTask& t = qc.tasks[ti];
@@ -1019,10 +1064,10 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
         const int z = P.atm[a * ATM_SLOTS + CHARGE_OF];
         if (z == 0) continue;
         const double* r = P.env.data() + P.atm[a * ATM_SLOTS + PTR_COORD];
-        nuc.insert(nuc.end(), {r[0], r[1], r[2], (double)z, (double)m});
+        nuc.insert(nuc.end(), {r[0], r[1], r[2], (double)z, (double)m, (double)a});
       }
-    P.nnuc = (int)(nuc.size() / 5);
-    if (nuc.empty()) nuc.assign(5, -1.0);
+    P.nnuc = (int)(nuc.size() / 6);
+    if (nuc.empty()) nuc.assign(6, -1.0);
     P.d_nuclei = dev_upload(nuc, s);
   }
   HIP_OK(hipMalloc(&P.d_flag, sizeof(int)));
@@ -1167,8 +1212,10 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     }
     const int nbd_eff = with_k ? nbd : nb;
     const int ncomp_a = (qc.la + 1) * (qc.la + 2) / 2, ncomp_b = (qc.lb + 1) * (qc.lb + 2) / 2;
-    const size_t lds = sizeof(double) * ((size_t)nbd_eff * qc.nbra_max * (ncomp_a + ncomp_b) * P.nc_max +
-                                         (size_t)nb * qc.nbra_max * ncomp_a * ncomp_b);
+    size_t lds = sizeof(double) * ((size_t)nbd_eff * qc.nbra_max * (ncomp_a + ncomp_b) * P.nc_max +
+                                   (size_t)nb * qc.nbra_max * ncomp_a * ncomp_b);
+    int tab_lds = qc.tab_lds;
+    if (tab_lds && lds + qc.tab_bytes <= kLdsBudget) lds += qc.tab_bytes; else tab_lds = 0;
     if (qc.d_chunks && lds <= kLdsBudget) {
       BraArgs b;
       b.pairs = P.d_pairs;
@@ -1192,6 +1239,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       b.thr = mixed_threshold;
       b.pair_q = P.d_pair_q;
       b.dmax = P.d_dmax;
+      b.tab_lds = tab_lds;
       launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, b, lds, s);
     } else {
       JKArgs a;
@@ -1244,6 +1292,25 @@ static void int1e(PlanData& P, double* s_out, double* t_out, double* v_out, hipS
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(st));
   HIP_OK(hipFree(tmp));
+}
+
+// One-electron gradient sum D d(T+V)/dR - W dS/dR into grad[natm][3] (+=).
+static void int1e_gradient(PlanData& P, const double* dens, const double* wdens, double* grad, hipStream_t s) {
+  if (!dens || !wdens || !grad) throw InvalidArgument("int1e gradient needs D, W and a gradient buffer");
+  if ((size_t)3 * P.natm * sizeof(double) > kLdsBudget) throw InvalidArgument("too many atoms for the gradient kernel");
+  const size_t ncc = (size_t)P.ncc_tot;
+  ensure_workspace(P, 2);
+  const double* src[2] = {dens, wdens};
+  for (int k = 0; k < 2; ++k)
+    hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc)), dim3(256), 0, s, src[k], P.nn_tot, 1,
+                       P.d_dc + k * ncc, P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, 1);
+  static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+  for (int c = 0; c < kNumPairClasses; ++c) {
+    const int b = P.p1e_begin[c], e = P.p1e_begin[c + 1];
+    launch_grad1e_class(pl[c][0], pl[c][1], P.d_pairs1e + b, e - b, P.d_prims, P.d_shells, P.d_seg_atom, P.d_nuclei,
+                        P.nnuc, P.d_molinfo, P.d_dc, P.d_dc + ncc, P.natm, grad, s);
+  }
+  HIP_OK(hipGetLastError());
 }
 
 static void geometry_vjp(PlanData& P, const double* dens, const double* cj, const double* ck, double* grad,
@@ -1435,6 +1502,35 @@ int gscf_workspace_bytes(const int32_t* atm, int32_t natm, const int32_t* bas, i
   });
 }
 
+int gscf_tuning_set(int32_t cls, const gscf_class_tuning* t) {
+  return guarded([&] {
+    if (!t) throw InvalidArgument("NULL tuning");
+    if (cls < -1 || cls >= kNumQuartetClasses) throw InvalidArgument("tuning class index out of range");
+    if (t->chunk_items < 64 || t->chunk_items > (1 << 20) || t->bras_max < 1 || t->bras_max > 256 ||
+        t->lds_pack < 1024 || t->lds_pack > (int)kLdsBudget || (t->rys_lds != 0 && t->rys_lds != 1) ||
+        t->target_prim < 1 || t->target_prim > 4096)
+      throw InvalidArgument("tuning parameter out of range");
+    ensure_tuning();
+    for (int c = 0; c < kNumQuartetClasses; ++c)
+      if (cls < 0 || c == cls) g_tuning[c] = *t;
+  });
+}
+
+int gscf_tuning_reset(void) {
+  return guarded([&] {
+    g_tuning_init = false;
+    ensure_tuning();
+  });
+}
+
+int gscf_tuning_get(int32_t cls, gscf_class_tuning* t) {
+  return guarded([&] {
+    if (!t || cls < 0 || cls >= kNumQuartetClasses) throw InvalidArgument("tuning class index out of range");
+    ensure_tuning();
+    *t = g_tuning[cls];
+  });
+}
+
 int gscf_plan_destroy(gscf_plan* plan) {
   return guarded([&] {
     if (!plan) return;
@@ -1549,6 +1645,14 @@ int gscf_int1e(gscf_plan* plan, double* s, double* t, double* v, void* stream) {
   return guarded([&] {
     if (!plan) throw InvalidArgument("NULL plan");
     int1e(plan->d, s, t, v, (hipStream_t)stream);
+  });
+}
+
+int gscf_int1e_gradient(gscf_plan* plan, const double* density, const double* wdensity, double* grad,
+                        void* stream) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("NULL plan");
+    int1e_gradient(plan->d, density, wdensity, grad, (hipStream_t)stream);
   });
 }
 

@@ -58,7 +58,7 @@ struct PrimQ {
 
 template <int NR>
 __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q, const QuartetGeom& g,
-                                           PrimQ<NR>& o) {
+                                           PrimQ<NR>& o, const double* __restrict__ tab) {
   const double pq = P.p + Q.p;
   o.ipq = 1.0 / pq;
   o.ip = P.ip;
@@ -77,7 +77,13 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
   const double rho = P.p * Q.p * o.ipq;
   const double T = rho * (o.PQ[0] * o.PQ[0] + o.PQ[1] * o.PQ[1] + o.PQ[2] * o.PQ[2]);
   o.pref = P.e * Q.e * sqrt(o.ipq);
-  rys_roots<NR>(T, o.s, o.w);
+  rys_roots_tab<NR>(tab, T, o.s, o.w);
+}
+
+template <int NR>
+__device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q, const QuartetGeom& g,
+                                           PrimQ<NR>& o) {
+  prim_setup<NR>(P, Q, g, o, rys_tab + rys_table_offset[NR - 1]);
 }
 
 // Full block of one primitive quartet accumulated into acc[NA][NB][NC][ND].
@@ -464,7 +470,7 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
 template <int LA, int LB, int LC, int LD, typename R>
 __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const Pair& kp, const int2 qk, int slot,
                                          int s0, int S, int kc, double* __restrict__ krow,
-                                         double* __restrict__ jrow) {
+                                         double* __restrict__ jrow, const double* __restrict__ tab) {
   using K = Class<LA, LB, LC, LD>;
   constexpr int ROWS = K::NA + K::NB;
   const int ns = a.n;  // LDS row length
@@ -496,7 +502,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
       for (int kq = 0; kq < kp.np; ++kq) {
         const PrimPair Q = a.prims[kp.poff + kq];
         PrimQ<K::NR> q;
-        prim_setup<K::NR>(P, Q, geo, q);
+        prim_setup<K::NR>(P, Q, geo, q, tab);
         accumulate_full<LA, LB, LC, LD, R>(q, AB, CD, acc);
       }
     }
@@ -619,7 +625,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
       for (int kq = 0; kq < kp.np; ++kq) {
         const PrimPair Q = a.prims[kp.poff + kq];
         PrimQ<K::NR> q;
-        prim_setup<K::NR>(P, Q, geo, q);
+        prim_setup<K::NR>(P, Q, geo, q, tab);
         accumulate_split<LA, LB, LC, LD, R>(q, AB, wk, acc);
       }
     }
@@ -694,7 +700,10 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
 // blocks it touches) is below a.thr is evaluated in FP32 (roots, weights and
 // prefactors stay FP64, digestion accumulates in FP64).
 // ----------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD>
+// With TLDS the Rys table slice of the class (rys_slice_size(NR) doubles) is
+// copied to LDS after the slabs, so the per-lane interval gathers of the root
+// evaluation read LDS instead of global memory.
+template <int LA, int LB, int LC, int LD, bool TLDS>
 __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
   using K = Class<LA, LB, LC, LD>;
   constexpr int ROWS = K::NA + K::NB;
@@ -706,6 +715,12 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
   double* jrow = lds + krow_size;          // [nbj][slot][NA][NB]
   const int jrow_size = a.nbj * a.nbra_max * K::NBRA;
   for (int i = threadIdx.x; i < krow_size + jrow_size; i += blockDim.x) lds[i] = 0.0;
+  const double* tab = rys_tab + rys_table_offset[K::NR - 1];
+  if constexpr (TLDS) {
+    double* ltab = lds + ((krow_size + jrow_size + 1) & ~1);  // 16-byte aligned
+    for (int i = threadIdx.x; i < rys_slice_size(K::NR); i += blockDim.x) ltab[i] = tab[i];
+    tab = ltab;
+  }
   __syncthreads();
   const int64_t total = (int64_t)ch.nq * ch.S * K::SPLIT;
   for (int64_t it = threadIdx.x; it < total; it += blockDim.x) {
@@ -730,9 +745,9 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
       lowp = a.pair_q[qk.x] * a.pair_q[qk.y] * d6 < a.thr;
     }
     if (lowp)
-      bra_item<LA, LB, LC, LD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow);
+      bra_item<LA, LB, LC, LD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, tab);
     else
-      bra_item<LA, LB, LC, LD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow);
+      bra_item<LA, LB, LC, LD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, tab);
   }
   __syncthreads();
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
@@ -838,7 +853,10 @@ void launch_blocks_t(const BlockArgs& a, hipStream_t s) {
 template <int LA, int LB, int LC, int LD>
 void launch_jk_bra_t(const BraArgs& a, size_t lds_bytes, hipStream_t s) {
   if (a.nchunks <= 0) return;
-  hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD>), dim3((unsigned)a.nchunks), dim3(256), lds_bytes, s, a);
+  if (a.tab_lds)
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, true>), dim3((unsigned)a.nchunks), dim3(256), lds_bytes, s, a);
+  else
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, false>), dim3((unsigned)a.nchunks), dim3(256), lds_bytes, s, a);
 }
 
 }  // namespace gscf

@@ -236,6 +236,21 @@ class DirectJKPlan:
             None if ck is None else ck.data_ptr(), grad.data_ptr(), _stream_handle()))
         return grad
 
+    def int1e_gradient(self, density, wdensity) -> torch.Tensor:
+        """sum D d(T+V)/dR - W dS/dR for every atm-row coordinate, as (natm, 3):
+        basis-centre rows carry the basis-function derivatives, nuclear rows
+        the Hellmann-Feynman term."""
+        dev = _device()
+        shape = (self.nao, self.nao) if self.nmol == 1 else (self.nn_total,)
+        d = torch.as_tensor(density).to(dev, torch.float64).contiguous()
+        w = torch.as_tensor(wdensity).to(dev, torch.float64).contiguous()
+        if tuple(d.shape) != shape or tuple(w.shape) != shape:
+            raise ValueError("int1e gradient inputs must be (n, n) (packed for a batched plan)")
+        grad = torch.zeros((len(self.atm), 3), dtype=torch.float64, device=dev)
+        _native.check(_native.lib().gscf_int1e_gradient(self._handle, d.data_ptr(), w.data_ptr(), grad.data_ptr(),
+                                                        _stream_handle()))
+        return grad
+
     def int1e(self):
         """(S, T, V) as (n, n) float64 GPU tensors in the plan's AO basis
         (packed (nn_total,) tensors for a batched plan)."""

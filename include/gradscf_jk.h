@@ -37,6 +37,18 @@ extern "C" {
 
 typedef struct gscf_plan gscf_plan;
 
+/* Schedule parameters of one canonical quartet class (no counterpart in the
+ * reference; set by the autotuner).  Class index c = cb*(cb+1)/2 + ck over
+ * pair classes ss, ps, pp, ds, dp, dd = 0..5 with ck <= cb (0 = ssss ...
+ * 20 = dddd).  Plans copy the process-wide table when they are created. */
+typedef struct {
+  int32_t chunk_items;  /* lane-items per workgroup before a bra chunk is closed (64..2^20) */
+  int32_t bras_max;     /* bra pairs (LDS exchange slots) per chunk (1..256) */
+  int32_t lds_pack;     /* LDS bytes the slots of one workgroup may use (1024..65536) */
+  int32_t rys_lds;      /* 1: stage the class's Rys table slice in LDS */
+  int32_t target_prim;  /* primitive quartets per work item (primitive split), 1..4096 */
+} gscf_class_tuning;
+
 typedef struct {
   int32_t n;            /* AO count (spherical or Cartesian, as requested) */
   int32_t n_cart;       /* Cartesian AO count used internally */
@@ -62,6 +74,13 @@ typedef struct {
 int gscf_workspace_bytes(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas,
                          const double* env, int32_t nenv, int32_t cart, double cutoff, int32_t nb,
                          int64_t* bytes);
+
+/* Process-wide tuning table: set one class (cls >= 0) or all (cls = -1),
+ * read one class, or restore the defaults.  Not thread-safe against
+ * concurrent plan creation. */
+int gscf_tuning_set(int32_t cls, const gscf_class_tuning* t);
+int gscf_tuning_get(int32_t cls, gscf_class_tuning* t);
+int gscf_tuning_reset(void);
 
 /* Last error message of the calling thread. */
 const char* gscf_last_error(void);
@@ -140,6 +159,16 @@ int gscf_eri_blocks(gscf_plan* plan, const int32_t* shls, int64_t nq, const int6
  * reference's int1e_ovlp / int1e_kin / int1e_nuc evaluation used for the SCF
  * core Hamiltonian (csrc/ffi.cc:11-86). */
 int gscf_int1e(gscf_plan* plan, double* s, double* t, double* v, void* stream);
+
+/* One-electron part of the SCF nuclear gradient (the reference's
+ * int1e_ipovlp / int1e_ipkin / int1e_ipnuc + int1e_iprinv contractions,
+ * csrc/geometry.cc via libcint autocode/grad1.c):
+ *   grad[natm,3] (device) += sum D d(T+V)/dR - W dS/dR
+ * for every atm row (basis-centre rows and nuclear rows, the latter carrying
+ * the Hellmann-Feynman term), with D, W F64[n,n] device matrices in the plan's
+ * AO basis (packed for a batched plan), symmetrised inside. */
+int gscf_int1e_gradient(gscf_plan* plan, const double* density, const double* wdensity, double* grad,
+                        void* stream);
 
 /* Coordinate VJP of J/K (direct_jk_geometry.cc:122-182, cutoff must be 0 in the
  * reference): grad[natm,3] (device) += d/dR sum(cot_j*J(D)) + sum(cot_k*K(D))

@@ -331,3 +331,25 @@ def test_workspace_forecast_bounds_plan_memory(basis, cart, cutoff):
     _native.check(_native.lib().gscf_plan_get_info(plan._handle, ctypes.byref(info)))
     forecast = native_jk_workspace_bytes(NativeDirectBasis(make_plan(topo), par), 1, cutoff, nb=2)
     assert 0 < info.device_bytes <= forecast
+
+
+@pytest.mark.parametrize("sched", [dict(rys_lds=1), dict(chunk_items=1024, bras_max=4, target_prim=12),
+                                   dict(rys_lds=1, chunk_items=8192, bras_max=32, target_prim=96)])
+def test_schedules_do_not_change_results(sched):
+    """Every tuning-table schedule (Rys table in LDS, chunk sizes, primitive
+    split) gives the default schedule's J/K up to FP64 summation order."""
+    from gradscf_amd import autotune as at
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "cc-pvdz", cart=False)
+    at.reset()
+    ref_plan = _plan(atm, bas, env, False, 1e-12)
+    d = _cuda(systems.random_density(ref_plan.nao, nb=2, seed=9, symmetric=False))
+    ref = ref_plan.jk(d)
+    ref_m = ref_plan.jk(d[:1] * 1e-3, symmetric=False, mixed_threshold=1e-7)
+    try:
+        at.set_schedule(at.Schedule(**sched))
+        plan = _plan(atm, bas, env, False, 1e-12)
+        assert float((plan.jk(d) - ref).abs().max()) < 1e-12
+        got_m = plan.jk(d[:1] * 1e-3, symmetric=False, mixed_threshold=1e-7)
+        assert float((got_m - ref_m).abs().max()) < 1e-12
+    finally:
+        at.reset()

@@ -241,3 +241,22 @@ def test_workspace_forecast_without_gpu():
         native_jk_workspace_bytes(prov, 0, 0.0)
     with pytest.raises(ValueError):
         native_jk_workspace_bytes(prov, 1, -1.0)
+
+
+def test_tuning_table_api_without_gpu():
+    if not (ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so").is_file():
+        pytest.skip("engine not built")
+    from gradscf_amd import autotune as at
+    at.reset()
+    assert at.get_schedule((0, 0, 0, 0)) == at.Schedule()
+    at.set_schedule(at.Schedule(chunk_items=2048, rys_lds=1), (2, 1, 1, 0))
+    assert at.get_schedule((2, 1, 1, 0)).chunk_items == 2048
+    assert at.get_schedule((2, 1, 1, 0)).rys_lds == 1
+    assert at.get_schedule((2, 1, 1, 1)) == at.Schedule()
+    assert [at.class_index(c) for c in [(0, 0, 0, 0), (1, 0, 0, 0), (2, 2, 2, 2)]] == [0, 1, 20]
+    with pytest.raises(ValueError):
+        at.set_schedule(at.Schedule(bras_max=0))
+    with pytest.raises(ValueError):
+        at.set_schedule(at.Schedule(rys_lds=2))
+    at.reset()
+    assert at.get_schedule((2, 1, 1, 0)) == at.Schedule()
