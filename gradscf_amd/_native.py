@@ -17,7 +17,7 @@ EXPORTS = ("gscf_last_error", "gscf_plan_create", "gscf_plan_create_batch", "gsc
            "gscf_plan_destroy", "gscf_plan_get_info", "gscf_plan_get_schwarz", "gscf_plan_profile", "gscf_plan_class_times",
            "gscf_jk_build", "gscf_direct_jk", "gscf_eri_blocks", "gscf_int1e", "gscf_jk_geometry_vjp",
            "gscf_workspace_bytes", "gscf_int1e_gradient", "gscf_tuning_set", "gscf_tuning_get",
-           "gscf_tuning_reset")
+           "gscf_tuning_reset", "gscf_jk_param_vjp", "gscf_int1e_param_vjp")
 
 GSCF_OK, GSCF_INVALID_ARGUMENT, GSCF_INTERNAL = 0, 1, 2
 FLAG_SYMMETRIC_DENSITY = 1
@@ -71,6 +71,8 @@ def lib() -> ctypes.CDLL:
     handle.gscf_tuning_set.argtypes = [i32, vp]
     handle.gscf_tuning_get.argtypes = [i32, vp]
     handle.gscf_tuning_reset.argtypes = []
+    handle.gscf_jk_param_vjp.argtypes = [vp, vp, vp, vp, vp, vp]
+    handle.gscf_int1e_param_vjp.argtypes = [vp, vp, vp, vp, vp]
     handle.gscf_workspace_bytes.argtypes = [vp, i32, vp, i32, vp, i32, i32, dbl, i32, vp]
     return handle
 

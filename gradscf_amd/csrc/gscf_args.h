@@ -17,6 +17,15 @@ struct PrimPair {
   double ip;       // 1 / p
 };
 
+// Per primitive pair, for the basis-parameter derivatives only: primitive
+// indices inside the two segmented shells, their coefficients (s/p factor
+// included) and the coefficient-free factor K_AB sqrt(2 pi^{5/2}) / p.
+struct PrimAux {
+  int ia, ib;
+  double ca, cb;
+  double ek;
+};
+
 struct Pair {
   int sa, sb;      // segmented shells, l(sa) >= l(sb)
   int poff, np;    // primitive-pair range
@@ -100,6 +109,25 @@ struct BraArgs {
   const double* pair_q;  // Schwarz bound per pair
   const double* dmax;    // per molecule [nseg][nseg] max |D| over shell blocks (all batches)
   int tab_lds;           // stage the class's Rys table slice in LDS (after the slabs)
+};
+
+struct ParamArgs {
+  const Pair* pairs;
+  const PrimPair* prims;
+  const PrimAux* aux;
+  const ShellXYZ* shells;
+  const int* seg_prim0;  // first primitive slot of each segmented shell
+  const int2* quartets;
+  const SubTask* tasks;
+  int ntasks;
+  int64_t nitems;
+  const double* dens;
+  const double* cj;
+  const double* ck;
+  const MolInfo* mols;
+  int nprim;             // primitive slots over all segmented shells
+  double* gcoef;         // [nprim] dE/d(coefficient incl. s/p factor)
+  double* gexp;          // [nprim] dE/d(exponent) at fixed coefficients
 };
 
 // Explicit quartet list writing contracted Cartesian blocks:

@@ -338,6 +338,199 @@ void launch_grad1e_class(int la, int lb, const Pair* pairs, int npairs, const Pr
 #undef G1E
 }
 
+// ---------------------------------------------------------------------------
+// Basis-parameter derivatives of E1 = sum D (T + V) - W S (symmetrised D, W):
+// per primitive pair (i, j) of a segmented shell pair,
+//   dE1/dc_i += [coefficient-free value] c_j,
+//   dE1/d(alpha_i) += -sum_x value(a + 2_x) c_i c_j   (d/d alpha g = -|r-A|^2 g),
+// and alike for j, with the raised overlaps/kinetic from 1D tables two units
+// higher and the raised nuclear attraction from Rys tables two units higher.
+// ---------------------------------------------------------------------------
+struct Param1eArgs {
+  const Pair* pairs;
+  const PrimPair* prims;
+  const PrimAux* aux;
+  const ShellXYZ* shells;
+  const int* seg_prim0;
+  const double* nuclei;
+  int nnuc;
+  int npairs;
+  const MolInfo* mols;
+  const double* dens;
+  const double* wdens;
+  int nprim;
+  double* gcoef;
+  double* gexp;
+};
+
+template <int LA, int LB>
+__global__ __launch_bounds__(256) void param1e_kernel(Param1eArgs a) {
+  constexpr int NA = ncart(LA), NB = ncart(LB);
+  constexpr int NR = (LA + LB + 2) / 2 + 1;
+  extern __shared__ double pl[];
+  double* ycoef = pl;
+  double* yexp = pl + a.nprim;
+  for (int i = threadIdx.x; i < 2 * a.nprim; i += blockDim.x) pl[i] = 0.0;
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.npairs) {
+    const Pair pr = a.pairs[i];
+    const ShellXYZ A = a.shells[pr.sa], B = a.shells[pr.sb];
+    const double Ar[3] = {A.x, A.y, A.z}, Br[3] = {B.x, B.y, B.z};
+    const MolInfo mi = a.mols[A.mol];
+    const int nc = mi.n;
+    const double* Dm = a.dens + mi.base;
+    const double* Wm = a.wdens + mi.base;
+    const double fpair = pr.sa == pr.sb ? 1.0 : 2.0;
+    double wd[NA][NB], ww[NA][NB];
+#pragma unroll
+    for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < NB; ++ib) {
+        const size_t x = (size_t)(A.ao + ia) * nc + B.ao + ib, y = (size_t)(B.ao + ib) * nc + A.ao + ia;
+        wd[ia][ib] = 0.5 * fpair * (Dm[x] + Dm[y]);
+        ww[ia][ib] = 0.5 * fpair * (Wm[x] + Wm[y]);
+      }
+    const double pi = 3.14159265358979323846;
+    const double s2p = sqrt(2.0 * pi * pi * sqrt(pi));
+    const int pA = a.seg_prim0[pr.sa], pB = a.seg_prim0[pr.sb];
+    for (int ip = 0; ip < pr.np; ++ip) {
+      const PrimPair P = a.prims[pr.poff + ip];
+      const PrimAux X = a.aux[pr.poff + ip];
+      const double p = P.p, ip2 = 0.5 / P.p;
+      const double Pr[3] = {P.px, P.py, P.pz};
+      const double kab = X.ek * p / s2p;  // K_AB (coefficient-free)
+      const double ai = P.aa, aj = P.ab;
+      double S1[3][LA + 3][LB + 5];
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        const double pa = Pr[x] - Ar[x], pb = Pr[x] - Br[x];
+#pragma unroll
+        for (int ia = 0; ia <= LA + 2; ++ia)
+#pragma unroll
+          for (int ib = 0; ib <= LB + 4; ++ib) {
+            double val;
+            if (ia == 0 && ib == 0) {
+              val = 1.0;
+            } else if (ia > 0) {
+              val = pa * S1[x][ia - 1][ib];
+              if (ia > 1) val += (ia - 1) * ip2 * S1[x][ia - 2][ib];
+              if (ib > 0) val += ib * ip2 * S1[x][ia - 1][ib - 1];
+            } else {
+              val = pb * S1[x][0][ib - 1];
+              if (ib > 1) val += (ib - 1) * ip2 * S1[x][0][ib - 2];
+            }
+            S1[x][ia][ib] = val;
+          }
+      }
+      const double sfac = kab * (pi / p) * sqrt(pi / p);
+      auto t1 = [&](int x, int ia, int ib) {
+        double t = -2.0 * aj * aj * S1[x][ia][ib + 2] + aj * (2 * ib + 1) * S1[x][ia][ib];
+        if (ib > 1) t -= 0.5 * ib * (ib - 1) * S1[x][ia][ib - 2];
+        return t;
+      };
+      double val = 0.0, rA = 0.0, rB = 0.0;
+#pragma unroll
+      for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib) {
+          const int ea[3] = {cart_comp(LA, ia, 0), cart_comp(LA, ia, 1), cart_comp(LA, ia, 2)};
+          const int eb[3] = {cart_comp(LB, ib, 0), cart_comp(LB, ib, 1), cart_comp(LB, ib, 2)};
+          double s[3], t[3];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) {
+            s[x] = S1[x][ea[x]][eb[x]];
+            t[x] = t1(x, ea[x], eb[x]);
+          }
+          const double sv = s[0] * s[1] * s[2];
+          const double tv = t[0] * s[1] * s[2] + s[0] * t[1] * s[2] + s[0] * s[1] * t[2];
+          val += sfac * (wd[ia][ib] * tv - ww[ia][ib] * sv);
+#pragma unroll
+          for (int x = 0; x < 3; ++x) {
+            const int y = (x + 1) % 3, z = (x + 2) % 3;
+            // a raised by two on axis x
+            const double sa2 = S1[x][ea[x] + 2][eb[x]], ta2 = t1(x, ea[x] + 2, eb[x]);
+            rA += sfac * (wd[ia][ib] * (ta2 * s[y] * s[z] + sa2 * (t[y] * s[z] + s[y] * t[z])) -
+                          ww[ia][ib] * sa2 * s[y] * s[z]);
+            // b raised by two on axis x
+            const double sb2 = S1[x][ea[x]][eb[x] + 2], tb2 = t1(x, ea[x], eb[x] + 2);
+            rB += sfac * (wd[ia][ib] * (tb2 * s[y] * s[z] + sb2 * (t[y] * s[z] + s[y] * t[z])) -
+                          ww[ia][ib] * sb2 * s[y] * s[z]);
+          }
+        }
+      for (int c = 0; c < a.nnuc; ++c) {
+        const double* C = a.nuclei + 6 * c;
+        if ((int)C[4] != A.mol) continue;
+        const double pc[3] = {Pr[0] - C[0], Pr[1] - C[1], Pr[2] - C[2]};
+        const double T = p * (pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2]);
+        double u[NR], w[NR];
+        rys_roots<NR>(T, u, w);
+        const double vf = -C[3] * 2.0 * pi / p * kab;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const double b10 = ip2 * (1.0 - u[r]);
+          double hA[3][LA + 3][LB + 1], hB[3][LA + 1][LB + 3];
+#pragma unroll
+          for (int x = 0; x < 3; ++x) {
+            double g[LA + LB + 3];
+            const double c00 = (Pr[x] - Ar[x]) - pc[x] * u[r];
+            g[0] = x == 2 ? vf * w[r] : 1.0;
+#pragma unroll
+            for (int m = 0; m < LA + LB + 2; ++m) g[m + 1] = c00 * g[m] + (m ? m * b10 * g[m - 1] : 0.0);
+            hrr_bra1<LA + 2, LB>(g, Ar[x] - Br[x], hA[x]);
+            hrr_bra1<LA, LB + 2>(g, Ar[x] - Br[x], hB[x]);
+          }
+#pragma unroll
+          for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < NB; ++ib) {
+              const int ea[3] = {cart_comp(LA, ia, 0), cart_comp(LA, ia, 1), cart_comp(LA, ia, 2)};
+              const int eb[3] = {cart_comp(LB, ib, 0), cart_comp(LB, ib, 1), cart_comp(LB, ib, 2)};
+              double v[3];
+#pragma unroll
+              for (int x = 0; x < 3; ++x) v[x] = hA[x][ea[x]][eb[x]];
+              const double o[3] = {v[1] * v[2], v[0] * v[2], v[0] * v[1]};
+              const double wv = wd[ia][ib];
+              val += wv * v[0] * o[0];
+#pragma unroll
+              for (int x = 0; x < 3; ++x) {
+                rA += wv * hA[x][ea[x] + 2][eb[x]] * o[x];
+                rB += wv * hB[x][ea[x]][eb[x] + 2] * o[x];
+              }
+            }
+        }
+      }
+      atomicAdd(ycoef + pA + X.ia, val * X.cb);
+      atomicAdd(ycoef + pB + X.ib, val * X.ca);
+      atomicAdd(yexp + pA + X.ia, -rA * X.ca * X.cb);
+      atomicAdd(yexp + pB + X.ib, -rB * X.ca * X.cb);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < a.nprim; k += blockDim.x) {
+    const double vc = ycoef[k], ve = yexp[k];
+    if (vc != 0.0) atomicAdd(a.gcoef + k, vc);
+    if (ve != 0.0) atomicAdd(a.gexp + k, ve);
+  }
+}
+
+void launch_param1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims, const PrimAux* aux,
+                          const ShellXYZ* shells, const int* seg_prim0, const double* nuclei, int nnuc,
+                          const MolInfo* mols, const double* dens, const double* wdens, int nprim, double* gcoef,
+                          double* gexp, hipStream_t st) {
+  if (npairs <= 0) return;
+  Param1eArgs a{pairs, prims, aux, shells, seg_prim0, nuclei, nnuc, npairs, mols, dens, wdens, nprim, gcoef, gexp};
+  const dim3 grid((npairs + 255) / 256), block(256);
+  const size_t lds = sizeof(double) * 2 * nprim;
+#define P1E(A, B)                                                          \
+  if (la == A && lb == B) {                                                \
+    hipLaunchKernelGGL((param1e_kernel<A, B>), grid, block, lds, st, a);   \
+    return;                                                                \
+  }
+  P1E(0, 0) P1E(1, 0) P1E(1, 1) P1E(2, 0) P1E(2, 1) P1E(2, 2)
+#undef P1E
+}
+
 void launch_int1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
                         const ShellXYZ* shells, const double* nuclei, int nnuc, const MolInfo* mols, double* s,
                         double* t, double* v, hipStream_t st) {

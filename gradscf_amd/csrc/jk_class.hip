@@ -2,6 +2,7 @@
 // (built once per class with -DGSCF_LA/LB/LC/LD so the build parallelises).
 #include "jk_kernels.h"
 #include "grad_kernels.h"
+#include "param_kernels.h"
 
 #define GSCF_CAT5(a, b, c, d, e) a##b##c##d##e
 #define GSCF_NAME(p, A, B, C, D) GSCF_CAT5(p, A, B, C, D)
@@ -15,6 +16,9 @@ void GSCF_NAME(launch_jk_bra_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const BraArg
 }
 void GSCF_NAME(launch_grad_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const GradArgs& a, hipStream_t s) {
   launch_grad_t<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, s);
+}
+void GSCF_NAME(launch_param_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const ParamArgs& a, hipStream_t s) {
+  launch_param_t<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, s);
 }
 void GSCF_NAME(launch_blocks_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const BlockArgs& a, hipStream_t s) {
   launch_blocks_t<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, s);

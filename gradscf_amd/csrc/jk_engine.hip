@@ -176,6 +176,8 @@ struct PlanData {
   int64_t nn_tot = 0, ncc_tot = 0, nd_tot = 0; // packed matrix sizes of one batch
   std::vector<Seg> seg;
   std::vector<PrimPair> prims;
+  std::vector<PrimAux> aux;          // aligned with prims
+  std::vector<int> seg_prim0;        // primitive slots of each segmented shell (nseg + 1)
   // all intra-molecule segmented pairs (unscreened) for block queries
   std::vector<Pair> all_pairs;
   std::vector<int> all_class;
@@ -217,6 +219,12 @@ struct PlanData {
   double* d_dmax = nullptr;
   int* d_seg_ao = nullptr;   // [nseg][2]: Cartesian AO offset, ncart
   int* d_seg_atom = nullptr; // atm row of each segmented shell
+  // basis-parameter derivatives
+  PrimAux* d_aux = nullptr;
+  int* d_seg_prim0 = nullptr;
+  int2* d_prim_slots = nullptr;  // per primitive slot: env index of its exponent, of its coefficient
+  double* d_prim_fsp = nullptr;  // s/p factor folded into the slot's coefficient
+  double* d_gparam = nullptr;    // [2][nprim] scratch sums
   // optional per-class kernel timing with HIP events on the launch stream
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
@@ -239,6 +247,10 @@ GSCF_CLASSES(X)
 void launch_int1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
                         const ShellXYZ* shells, const double* nuclei, int nnuc, const MolInfo* mols, double* s,
                         double* t, double* v, hipStream_t st);
+void launch_param1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims, const PrimAux* aux,
+                          const ShellXYZ* shells, const int* seg_prim0, const double* nuclei, int nnuc,
+                          const MolInfo* mols, const double* dens, const double* wdens, int nprim, double* gcoef,
+                          double* gexp, hipStream_t st);
 void launch_grad1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
                          const ShellXYZ* shells, const int* seg_atom, const double* nuclei, int nnuc,
                          const MolInfo* mols, const double* dens, const double* wdens, int natm, double* grad,
@@ -250,6 +262,21 @@ GSCF_CLASSES(X)
 #define X(A, B, C, D) void launch_grad_c##A##B##C##D(const GradArgs& a, hipStream_t s);
 GSCF_CLASSES(X)
 #undef X
+
+#define X(A, B, C, D) void launch_param_c##A##B##C##D(const ParamArgs& a, hipStream_t s);
+GSCF_CLASSES(X)
+#undef X
+
+void launch_param_class(int la, int lb, int lc, int ld, const ParamArgs& a, hipStream_t s) {
+#define X(A, B, C, D)                                      \
+  if (la == A && lb == B && lc == C && ld == D) {          \
+    launch_param_c##A##B##C##D(a, s);                      \
+    return;                                                \
+  }
+  GSCF_CLASSES(X)
+#undef X
+  throw Internal("no kernel for this angular-momentum class");
+}
 
 void launch_grad_class(int la, int lb, int lc, int ld, const GradArgs& a, hipStream_t s) {
 #define X(A, B, C, D)                                      \
@@ -461,6 +488,18 @@ __global__ void dmax_kernel(const double* __restrict__ D, int64_t ncc_tot, int n
 
 static unsigned nblocks(size_t count) { return (unsigned)((count + 255) / 256); }
 
+// env_grad[exponent slot] += gexp; env_grad[coefficient slot] += fsp * gcoef
+// (the exponent slot is shared by the contraction columns of a shell).
+__global__ void param_to_env_kernel(int nprim, const int2* __restrict__ slots, const double* __restrict__ fsp,
+                                    const double* __restrict__ gcoef, const double* __restrict__ gexp,
+                                    double* __restrict__ env_grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nprim) return;
+  const int2 sl = slots[i];
+  if (gexp[i] != 0.0) atomicAdd(env_grad + sl.x, gexp[i]);
+  env_grad[sl.y] += fsp[i] * gcoef[i];
+}
+
 // ----------------------------------------------------------------------------
 // plan construction
 // ----------------------------------------------------------------------------
@@ -629,6 +668,13 @@ static void make_pair(PlanData& P, int sa, int sb, Pair& out) {
       pp.ab = aj;
       pp.ip = 1.0 / p;
       P.prims.push_back(pp);
+      PrimAux x;
+      x.ia = i;
+      x.ib = j;
+      x.ca = A.c[i];
+      x.cb = B.c[j];
+      x.ek = std::exp(-mu) * s2p / p;
+      P.aux.push_back(x);
       ++out.np;
     }
 }
@@ -819,9 +865,33 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
       }
   }
   P.mol_pair0[P.nmol] = (int64_t)P.all_pairs.size();
-  if (P.prims.empty()) P.prims.push_back(PrimPair{});
+  if (P.prims.empty()) {
+    P.prims.push_back(PrimPair{});
+    P.aux.push_back(PrimAux{});
+  }
   P.d_prims = dev_upload(P.prims, s);
+  P.d_aux = dev_upload(P.aux, s);
   P.d_all_pairs = dev_upload(P.all_pairs, s);
+  {
+    // primitive slots of every segmented shell and where they live in env
+    const double pi = 3.14159265358979323846;
+    P.seg_prim0.assign(nseg + 1, 0);
+    std::vector<int2> slots;
+    std::vector<double> fsp;
+    for (int i = 0; i < nseg; ++i) {
+      const Seg& g = P.seg[i];
+      P.seg_prim0[i + 1] = P.seg_prim0[i] + g.np;
+      const int* row = P.bas.data() + g.parent * BAS_SLOTS;
+      const double f = g.l == 0 ? 0.5 / std::sqrt(pi) : (g.l == 1 ? std::sqrt(3.0 / (4.0 * pi)) : 1.0);
+      for (int p = 0; p < g.np; ++p) {
+        slots.push_back(make_int2(row[PTR_EXP] + p, row[PTR_COEFF] + g.ctr * g.np + p));
+        fsp.push_back(f);
+      }
+    }
+    P.d_seg_prim0 = dev_upload(P.seg_prim0, s);
+    P.d_prim_slots = dev_upload(slots, s);
+    P.d_prim_fsp = dev_upload(fsp, s);
+  }
   // Schwarz bounds (parent-shell maxima, as the reference defines them per shell pair)
   const int npair = (int)P.all_pairs.size();
   std::vector<double> q(npair, 1.0);
@@ -1080,7 +1150,9 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
 static int64_t plan_bytes(const PlanData& P, int64_t nq, int64_t ntasks, int64_t nchunks, int64_t nbd) {
   int64_t b = 0;
   b += (int64_t)P.seg.size() * (sizeof(ShellXYZ) + 3 * sizeof(int));         // shells, seg_ao, seg_atom
-  b += (int64_t)std::max<size_t>(P.prims.size(), 1) * sizeof(PrimPair);
+  b += (int64_t)std::max<size_t>(P.prims.size(), 1) * (sizeof(PrimPair) + sizeof(PrimAux));
+  b += (int64_t)(P.seg.size() + 1) * sizeof(int);
+  for (const Seg& g : P.seg) b += (int64_t)g.np * (sizeof(int2) + 3 * sizeof(double));  // slots, fsp, sums
   b += (int64_t)P.all_pairs.size() * 2 * sizeof(Pair);                        // all pairs, 1e pairs
   b += (int64_t)P.pairs.size() * (sizeof(Pair) + sizeof(double));             // screened pairs, pair_q
   b += (int64_t)P.n_cart * 2 * (sizeof(int) + sizeof(double)) + (int64_t)P.n * 3 * (sizeof(int) + sizeof(double));
@@ -1144,6 +1216,7 @@ static void free_plan(PlanData& P) {
   f(P.d_pairs1e); f(P.d_nuclei);
   f(P.d_pair_q); f(P.d_dmax); f(P.d_seg_ao); f(P.d_seg_atom);
   f(P.d_molinfo); f(P.d_molao);
+  f(P.d_aux); f(P.d_seg_prim0); f(P.d_prim_slots); f(P.d_prim_fsp); f(P.d_gparam);
   for (double* p : P.retired) f(p);
   for (auto& c : P.classes) {
     f(c.d_quartets);
@@ -1343,6 +1416,78 @@ static void geometry_vjp(PlanData& P, const double* dens, const double* cj, cons
     if ((size_t)3 * P.natm * sizeof(double) > kLdsBudget) throw InvalidArgument("too many atoms for the gradient kernel");
     launch_grad_class(qc.la, qc.lb, qc.lc, qc.ld, a, s);
   }
+  HIP_OK(hipGetLastError());
+}
+
+// Basis-parameter VJP of J/K into env_grad[nenv] (+=): exponent and
+// coefficient slots of every shell.
+static void jk_param_vjp(PlanData& P, const double* dens, const double* cj, const double* ck, double* env_grad,
+                         hipStream_t s) {
+  if (P.cutoff > 0) throw InvalidArgument("Screened direct J/K basis-parameter AD requires cutoff=0");
+  if (!dens || !env_grad || (!cj && !ck))
+    throw InvalidArgument("basis-parameter VJP needs a density, an env gradient buffer and a cotangent");
+  const int nprim = P.seg_prim0.back();
+  if ((size_t)2 * nprim * sizeof(double) > kLdsBudget)
+    throw InvalidArgument("too many primitives for the basis-parameter kernel");
+  const size_t ncc = (size_t)P.ncc_tot;
+  ensure_workspace(P, 3);
+  if (!P.d_gparam) HIP_OK(hipMalloc(&P.d_gparam, sizeof(double) * 2 * std::max(nprim, 1)));
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(2 * (size_t)nprim)), dim3(256), 0, s, P.d_gparam,
+                     2 * (size_t)nprim, 0.0);
+  const double* src[3] = {dens, cj, ck};
+  for (int k = 0; k < 3; ++k)
+    if (src[k])
+      hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc)), dim3(256), 0, s, src[k], P.nn_tot, 1,
+                         P.d_dc + k * ncc, P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, 1);
+  for (const QuartetClass& qc : P.classes) {
+    ParamArgs a;
+    a.pairs = P.d_pairs;
+    a.prims = P.d_prims;
+    a.aux = P.d_aux;
+    a.shells = P.d_shells;
+    a.seg_prim0 = P.d_seg_prim0;
+    a.quartets = qc.d_quartets;
+    a.tasks = qc.d_tasks;
+    a.ntasks = (int)qc.tasks.size();
+    a.nitems = qc.nitems;
+    a.dens = P.d_dc;
+    a.cj = cj ? P.d_dc + ncc : nullptr;
+    a.ck = ck ? P.d_dc + 2 * ncc : nullptr;
+    a.mols = P.d_molinfo;
+    a.nprim = nprim;
+    a.gcoef = P.d_gparam;
+    a.gexp = P.d_gparam + nprim;
+    launch_param_class(qc.la, qc.lb, qc.lc, qc.ld, a, s);
+  }
+  hipLaunchKernelGGL(param_to_env_kernel, dim3(nblocks((size_t)nprim)), dim3(256), 0, s, nprim, P.d_prim_slots,
+                     P.d_prim_fsp, P.d_gparam, P.d_gparam + nprim, env_grad);
+  HIP_OK(hipGetLastError());
+}
+
+// Basis-parameter derivatives of sum D (T + V) - W S into env_grad (+=).
+static void int1e_param_vjp(PlanData& P, const double* dens, const double* wdens, double* env_grad, hipStream_t s) {
+  if (!dens || !wdens || !env_grad) throw InvalidArgument("int1e parameter VJP needs D, W and an env gradient buffer");
+  const int nprim = P.seg_prim0.back();
+  if ((size_t)2 * nprim * sizeof(double) > kLdsBudget)
+    throw InvalidArgument("too many primitives for the basis-parameter kernel");
+  const size_t ncc = (size_t)P.ncc_tot;
+  ensure_workspace(P, 2);
+  if (!P.d_gparam) HIP_OK(hipMalloc(&P.d_gparam, sizeof(double) * 2 * std::max(nprim, 1)));
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(2 * (size_t)nprim)), dim3(256), 0, s, P.d_gparam,
+                     2 * (size_t)nprim, 0.0);
+  const double* src[2] = {dens, wdens};
+  for (int k = 0; k < 2; ++k)
+    hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc)), dim3(256), 0, s, src[k], P.nn_tot, 1,
+                       P.d_dc + k * ncc, P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, 1);
+  static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+  for (int c = 0; c < kNumPairClasses; ++c) {
+    const int b = P.p1e_begin[c], e = P.p1e_begin[c + 1];
+    launch_param1e_class(pl[c][0], pl[c][1], P.d_pairs1e + b, e - b, P.d_prims, P.d_aux, P.d_shells,
+                         P.d_seg_prim0, P.d_nuclei, P.nnuc, P.d_molinfo, P.d_dc, P.d_dc + ncc, nprim, P.d_gparam,
+                         P.d_gparam + nprim, s);
+  }
+  hipLaunchKernelGGL(param_to_env_kernel, dim3(nblocks((size_t)nprim)), dim3(256), 0, s, nprim, P.d_prim_slots,
+                     P.d_prim_fsp, P.d_gparam, P.d_gparam + nprim, env_grad);
   HIP_OK(hipGetLastError());
 }
 
@@ -1653,6 +1798,22 @@ int gscf_int1e_gradient(gscf_plan* plan, const double* density, const double* wd
   return guarded([&] {
     if (!plan) throw InvalidArgument("NULL plan");
     int1e_gradient(plan->d, density, wdensity, grad, (hipStream_t)stream);
+  });
+}
+
+int gscf_jk_param_vjp(gscf_plan* plan, const double* density, const double* cot_j, const double* cot_k,
+                      double* env_grad, void* stream) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("NULL plan");
+    jk_param_vjp(plan->d, density, cot_j, cot_k, env_grad, (hipStream_t)stream);
+  });
+}
+
+int gscf_int1e_param_vjp(gscf_plan* plan, const double* density, const double* wdensity, double* env_grad,
+                         void* stream) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("NULL plan");
+    int1e_param_vjp(plan->d, density, wdensity, env_grad, (hipStream_t)stream);
   });
 }
 

@@ -67,3 +67,50 @@ def rhf_gradient(plan: DirectJKPlan, density, mo_coeff, mo_energy, nocc: int, ch
     rows = (g1 + g2).cpu().numpy()
     nnuc = len(charges)
     return fold_rows(rows, nnuc, shell_atom) + nuclear_repulsion_gradient(charges, coords)
+
+
+def _normalized_torch(l: int, a, c):
+    """torch version of basis.normalized_shell_coefficients (for autograd)."""
+    import math
+    n = l + 1.5
+    prim = 1.0 / torch.sqrt(math.gamma(n) / (2.0 * (2.0 * a) ** n))
+    w = c * prim[:, None]
+    metric = math.gamma(n) / (2.0 * (a[:, None] + a[None, :]) ** n)
+    norm2 = torch.einsum("pi,pq,qi->i", w, metric, w)
+    return w / torch.sqrt(norm2)[None, :]
+
+
+def env_to_basis_gradient(env_grad, bas, topology, parameters):
+    """Chain an env-slot gradient (exponent slots and normalised-coefficient
+    slots) through the basis normalisation: returns (dE/d exponents,
+    dE/d raw coefficients) per shell, shaped like parameters.exponents and
+    parameters.coefficients (the reference's JAX normalisation chain,
+    src/gradscf/integrals/basis/normalization.py:57-63)."""
+    g = np.asarray(env_grad, dtype=np.float64)
+    d_exp, d_coef = [], []
+    for s, (l, a, c) in enumerate(zip(topology.angular_momenta, parameters.exponents, parameters.coefficients)):
+        row = bas[s]
+        npr, nct = int(row[2]), int(row[3])
+        ge = torch.as_tensor(g[int(row[5]):int(row[5]) + npr])
+        gc = torch.as_tensor(g[int(row[6]):int(row[6]) + npr * nct].reshape(nct, npr).T.copy())
+        at = torch.tensor(np.asarray(a, dtype=np.float64), requires_grad=True)
+        ct = torch.tensor(np.asarray(c, dtype=np.float64).reshape(npr, nct), requires_grad=True)
+        e = _normalized_torch(int(l), at, ct)
+        loss = (gc * e).sum() + (ge * at).sum()
+        loss.backward()
+        d_exp.append(at.grad.numpy().copy())
+        d_coef.append(ct.grad.numpy().reshape(np.asarray(c).shape).copy())
+    return d_exp, d_coef
+
+
+def rhf_basis_gradient(plan: DirectJKPlan, density, mo_coeff, mo_energy, nocc: int, bas, topology, parameters):
+    """dE/d(exponents) and dE/d(raw contraction coefficients) of a converged
+    RHF energy (variational: D and W held fixed), per shell.  `plan` must be
+    unscreened and built from the packed tables of (topology, parameters)."""
+    if plan.cutoff > 0:
+        raise ValueError("Screened direct J/K basis-parameter AD requires cutoff=0")
+    d = density.to(torch.float64).contiguous()
+    co = mo_coeff[:, :nocc]
+    w = (2.0 * (co * mo_energy[:nocc]) @ co.T).contiguous()
+    env_grad = plan.int1e_param_vjp(d, w) + plan.jk_param_vjp(d, 0.5 * d, -0.25 * d)
+    return env_to_basis_gradient(env_grad.cpu().numpy(), bas, topology, parameters)

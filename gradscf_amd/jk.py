@@ -251,6 +251,33 @@ class DirectJKPlan:
                                                         _stream_handle()))
         return grad
 
+    def _env_vjp(self, fn, *mats) -> torch.Tensor:
+        dev = _device()
+        shape = (self.nao, self.nao) if self.nmol == 1 else (self.nn_total,)
+        ptrs = []
+        keep = []
+        for m in mats:
+            if m is None:
+                ptrs.append(None)
+                continue
+            t = torch.as_tensor(m).to(dev, torch.float64).contiguous()
+            if tuple(t.shape) != shape:
+                raise ValueError("basis-parameter VJP inputs must be (n, n) (packed for a batched plan)")
+            keep.append(t)
+            ptrs.append(t.data_ptr())
+        out = torch.zeros(len(self.env), dtype=torch.float64, device=dev)
+        _native.check(fn(self._handle, *ptrs, out.data_ptr(), _stream_handle()))
+        return out
+
+    def jk_param_vjp(self, density, cot_j=None, cot_k=None) -> torch.Tensor:
+        """d/d env [sum(cot_j * J(D)) + sum(cot_k * K(D))] at every exponent and
+        (normalised) coefficient slot, as an env-sized vector (cutoff must be 0)."""
+        return self._env_vjp(_native.lib().gscf_jk_param_vjp, density, cot_j, cot_k)
+
+    def int1e_param_vjp(self, density, wdensity) -> torch.Tensor:
+        """d/d env [sum D (T + V) - W S] at every exponent and coefficient slot."""
+        return self._env_vjp(_native.lib().gscf_int1e_param_vjp, density, wdensity)
+
     def int1e(self):
         """(S, T, V) as (n, n) float64 GPU tensors in the plan's AO basis
         (packed (nn_total,) tensors for a batched plan)."""

@@ -170,6 +170,19 @@ int gscf_int1e(gscf_plan* plan, double* s, double* t, double* v, void* stream);
 int gscf_int1e_gradient(gscf_plan* plan, const double* density, const double* wdensity, double* grad,
                         void* stream);
 
+/* Basis-parameter VJPs (exponents and contraction coefficients): env_grad
+ * F64[nenv] (device) += dE/d env at the exponent and coefficient slots of
+ * every shell (PTR_EXP + p, PTR_COEFF + c*nprim + p), coefficients taken as
+ * the normalised values stored in env (the caller chains through its
+ * normalisation).  gscf_jk_param_vjp: E = sum(cot_j*J(D)) + sum(cot_k*K(D))
+ * (plan cutoff must be 0; cot_j / cot_k may be NULL); gscf_int1e_param_vjp:
+ * E = sum D (T+V) - W S.  Exponent derivatives use d/d alpha g = -|r-A|^2 g
+ * (the reference's exponent route, csrc/exponents.cc:1-3). */
+int gscf_jk_param_vjp(gscf_plan* plan, const double* density, const double* cot_j, const double* cot_k,
+                      double* env_grad, void* stream);
+int gscf_int1e_param_vjp(gscf_plan* plan, const double* density, const double* wdensity, double* env_grad,
+                         void* stream);
+
 /* Coordinate VJP of J/K (direct_jk_geometry.cc:122-182, cutoff must be 0 in the
  * reference): grad[natm,3] (device) += d/dR sum(cot_j*J(D)) + sum(cot_k*K(D))
  * for the atm-row coordinates shells sit on.  cot_j / cot_k may be NULL. */

@@ -93,3 +93,115 @@ def test_rhf_gradient_matches_finite_difference(basis, cart):
             fd += (_enuc(spec.charges, coords_p) - _enuc(spec.charges, coords_m)) / (2 * h)
             assert abs(fd - g[a, ax]) < 1e-6, (a, ax, fd, g[a, ax])
     assert np.allclose(nuclear_repulsion_gradient(spec.charges, spec.coords_bohr).sum(axis=0), 0.0, atol=1e-12)
+
+
+def _slots(bas, shells):
+    """(env index, relative step) of every exponent and coefficient slot of the given shells."""
+    out = []
+    for s in shells:
+        row = bas[s]
+        for p in range(int(row[2])):
+            out.append((int(row[5]) + p, "exp"))
+        for k in range(int(row[2]) * int(row[3])):
+            out.append((int(row[6]) + k, "coef"))
+    return out
+
+
+def _fd_env(fun, env, idx, kind):
+    h = 1e-5 * abs(env[idx]) if kind == "exp" else 1e-5
+    ep, em = np.array(env, copy=True), np.array(env, copy=True)
+    ep[idx] += h
+    em[idx] -= h
+    return (fun(ep) - fun(em)) / (2 * h)
+
+
+@pytest.mark.parametrize("basis,cart", [("6-31g*", True), ("cc-pvdz", False)])
+def test_jk_param_vjp_matches_fd(basis, cart):
+    """d/d env [sum(cJ*J(D)) + sum(cK*K(D))] at exponent and coefficient slots
+    (general contraction included for cc-pVDZ) vs central differences of the
+    GPU J/K built from displaced env."""
+    import torch
+    from gradscf_amd.jk import DirectJKPlan
+    spec = M.parse_molecule_spec(WATER, unit="Bohr")
+    _, _, atm, bas, env = systems.tables(spec, basis, cart=cart)
+    plan = DirectJKPlan(atm, bas, env, cart=cart)
+    n = plan.nao
+    d = torch.as_tensor(systems.random_density(n, seed=91)[0], device="cuda")
+    cj = torch.as_tensor(systems.random_density(n, seed=92)[0], device="cuda")
+    ck = torch.as_tensor(systems.random_density(n, seed=93)[0], device="cuda")
+    g = plan.jk_param_vjp(d, cj, ck).cpu().numpy()
+
+    def e(env_):
+        out = DirectJKPlan(atm, bas, env_, cart=cart).jk(d)
+        return float((cj * out[0, 0]).sum() + (ck * out[1, 0]).sum())
+
+    d_shell = int(np.nonzero(bas[:, 1] == 2)[0][0])
+    for idx, kind in _slots(bas, [0, 3, d_shell, len(bas) - 1]):
+        fd = _fd_env(e, env, idx, kind)
+        assert abs(fd - g[idx]) < 1e-6 * max(1.0, abs(fd)), (idx, kind, fd, g[idx])
+
+
+@pytest.mark.parametrize("basis,cart", [("6-31g*", True), ("cc-pvdz", False)])
+def test_int1e_param_vjp_matches_fd(basis, cart):
+    import torch
+    from gradscf_amd.jk import DirectJKPlan
+    spec = M.parse_molecule_spec(WATER, unit="Bohr")
+    _, _, atm, bas, env = systems.tables(spec, basis, cart=cart)
+    plan = DirectJKPlan(atm, bas, env, cart=cart)
+    n = plan.nao
+    d = torch.as_tensor(systems.random_density(n, seed=94)[0], device="cuda")
+    w = torch.as_tensor(systems.random_density(n, seed=95)[0], device="cuda")
+    g = plan.int1e_param_vjp(d, w).cpu().numpy()
+
+    def e(env_):
+        s, t, v = DirectJKPlan(atm, bas, env_, cart=cart).int1e()
+        return float((d * (t + v)).sum() - (w * s).sum())
+
+    d_shell = int(np.nonzero(bas[:, 1] == 2)[0][0])
+    for idx, kind in _slots(bas, [0, 3, d_shell, len(bas) - 1]):
+        fd = _fd_env(e, env, idx, kind)
+        assert abs(fd - g[idx]) < 1e-6 * max(1.0, abs(fd)), (idx, kind, fd, g[idx])
+
+
+def test_rhf_basis_gradient_matches_finite_difference():
+    """dE/d(exponent) and dE/d(raw coefficient) of the converged RHF energy
+    (water 6-31G*, Cartesian) against central differences of SCF energies
+    (relative step 1e-5 for exponents, 1e-5 for coefficients)."""
+    from gradscf_amd.basis import BasisParameters, prepare_basis
+    from gradscf_amd.grad import rhf_basis_gradient
+    from gradscf_amd.jk import DirectJKPlan
+    from gradscf_amd.packing import pack
+    spec = M.parse_molecule_spec(WATER, unit="Bohr")
+    topo, par = prepare_basis(spec, "6-31g*", cart=True)
+    atm, bas, env = pack(topo, par)
+    res = _scf_energy(atm, bas, env, spec, True)
+    plan = DirectJKPlan(atm, bas, env, cart=True, cutoff=0.0)
+    d_exp, d_coef = rhf_basis_gradient(plan, res.density, res.mo_coeff, res.mo_energy, spec.nelectron // 2,
+                                       bas, topo, par)
+
+    def energy(p):
+        a, b, e = pack(topo, p)
+        return _scf_energy(a, b, e, spec, True).energy
+
+    for s in (0, 3, 5, 6):  # O core s, O p, O d, H s
+        for p in range(len(par.exponents[s])):
+            h = 1e-5 * par.exponents[s][p]
+            vals = []
+            for sgn in (1, -1):
+                ex = list(par.exponents)
+                a = np.array(ex[s], dtype=float)
+                a[p] += sgn * h
+                ex[s] = a
+                vals.append(energy(BasisParameters(tuple(ex), par.coefficients, par.centers, par.nuclear_coords)))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            assert abs(fd - d_exp[s][p]) < 1e-6 * max(1.0, abs(fd)), ("exp", s, p, fd, d_exp[s][p])
+            vals = []
+            for sgn in (1, -1):
+                cs = list(par.coefficients)
+                c = np.array(cs[s], dtype=float)
+                c.reshape(len(par.exponents[s]), -1)[p, 0] += sgn * 1e-5
+                cs[s] = c
+                vals.append(energy(BasisParameters(par.exponents, tuple(cs), par.centers, par.nuclear_coords)))
+            fd = (vals[0] - vals[1]) / 2e-5
+            ref = np.asarray(d_coef[s]).reshape(len(par.exponents[s]), -1)[p, 0]
+            assert abs(fd - ref) < 1e-6 * max(1.0, abs(fd)), ("coef", s, p, fd, ref)

@@ -260,3 +260,44 @@ def test_tuning_table_api_without_gpu():
         at.set_schedule(at.Schedule(rys_lds=2))
     at.reset()
     assert at.get_schedule((2, 1, 1, 0)) == at.Schedule()
+
+
+def test_basis_gradient_chain_rule_matches_fd():
+    """The normalisation chain (env-slot gradient -> raw exponents and raw
+    coefficients) against central differences of the numpy normalisation."""
+    from gradscf_amd.basis import BasisParameters
+    from gradscf_amd.grad import env_to_basis_gradient
+    topo, par = prepare_basis(M.water_monomer(), "cc-pvdz")
+    atm, bas, env = pack(topo, par)
+    rng = np.random.default_rng(3)
+    g = np.zeros(len(env))
+    for row in bas:  # cotangent on the normalised-coefficient slots; exponent slots enter linearly
+        g[row[6]:row[6] + row[2] * row[3]] = rng.normal(size=row[2] * row[3])
+
+    def f(p):
+        _, _, e = pack(topo, p)
+        return float(g @ e)
+
+    d_exp, d_coef = env_to_basis_gradient(g, bas, topo, par)
+    for s in (0, 2, 4, 5):
+        for p in range(len(par.exponents[s])):
+            h = 1e-5 * par.exponents[s][p]
+            vals = []
+            for sgn in (1, -1):
+                ex = list(par.exponents)
+                a = np.array(ex[s], dtype=float)
+                a[p] += sgn * h
+                ex[s] = a
+                vals.append(f(BasisParameters(tuple(ex), par.coefficients, par.centers, par.nuclear_coords)))
+            assert abs((vals[0] - vals[1]) / (2 * h) - d_exp[s][p]) < 1e-7 * max(1.0, abs(d_exp[s][p]))
+            c0 = np.asarray(par.coefficients[s], dtype=float)
+            for col in range(c0.reshape(len(par.exponents[s]), -1).shape[1]):
+                vals = []
+                for sgn in (1, -1):
+                    cs = list(par.coefficients)
+                    c = c0.reshape(len(par.exponents[s]), -1).copy()
+                    c[p, col] += sgn * 1e-6
+                    cs[s] = c.reshape(c0.shape)
+                    vals.append(f(BasisParameters(par.exponents, tuple(cs), par.centers, par.nuclear_coords)))
+                ref = np.asarray(d_coef[s]).reshape(len(par.exponents[s]), -1)[p, col]
+                assert abs((vals[0] - vals[1]) / 2e-6 - ref) < 1e-7 * max(1.0, abs(ref))
