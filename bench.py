@@ -73,15 +73,16 @@ def cpu_baseline(atm, bas, env, dens, cutoff, builds, j_gpu, k_gpu):
 
 CLASS_LETTERS = "spdf"
 HBM_PROFILE = os.path.join(ROOT, "profiles", "jk_class_kernels_water16_ccpvdz_hbm.txt")
+TUNING_DIR = os.path.join(ROOT, "gradscf_amd", "tuning")
 
 
 def _kernel_class(name: str):
     """'void gscf::jk_bra_kernel<2, 1, 1, 0>(gscf::BraArgs)' -> ('jk_bra_kernel', 'dpps')."""
     import re
-    m = re.search(r"(\w+)<(\d), (\d), (\d), (\d)>", name)
+    m = re.search(r"(\w+)<(\d), (\d), (\d), (\d)(?:, \w+)*>", name)
     if not m:
         return None, None
-    return m.group(1), "".join(CLASS_LETTERS[int(v)] for v in m.groups()[1:])
+    return m.group(1), "".join(CLASS_LETTERS[int(v)] for v in m.groups()[1:5])
 
 
 def copy_calibration() -> int:
@@ -195,7 +196,7 @@ def profiled_traffic(cls: str):
     return None
 
 
-def batch_section(thr: float, reps: int = 5) -> dict:
+def batch_section(thr: float, reps: int = 5, autotune: bool = True, retune: bool = False, save_dir=None) -> dict:
     """256 seeded water clusters (H2O)_n, n ~ U{1..8} (default_rng(256), cluster
     i seeded 1000+i, 8 A cube), 6-31G* Cartesian, Schwarz cutoff 1e-12: batched
     FP64 and mixed SCF with every iteration's J/K for all molecules in one
@@ -213,6 +214,26 @@ def batch_section(thr: float, reps: int = 5) -> dict:
     tabs = [pack(*prepare_basis(s, "6-31g*", cart=True)) for s in specs]
     nel = [s.nelectron for s in specs]
     enuc = [s.nuclear_repulsion() for s in specs]
+    from gradscf_amd import autotune as at
+    tuning = None
+    if autotune:
+        table_path = os.path.join(TUNING_DIR, "batch256_631gs_cart.json")
+        if os.path.isfile(table_path) and not retune:
+            tuning = {"source": os.path.relpath(table_path, ROOT), "table": at.load_table(table_path)}
+        else:
+            at.reset()
+            p0 = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
+            d0 = rhf_batch(p0, nel, enuc, use_graph=False, max_cycle=3).density
+            del p0
+            result = at.autotune(lambda: DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12),
+                                 lambda p: p.jk(d0[None], symmetric=True, validate=False), reps=2,
+                                 search={"rys_lds": (0, 1), "chunk_items": (2048, 8192), "bras_max": (8, 32)},
+                                 log=log)
+            tuning = {"source": "autotuned in this run", "classes": result}
+            if save_dir:
+                at.save_table(os.path.join(save_dir, "batch256_631gs_cart.json"), result)
+    else:
+        at.reset()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plan = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
@@ -264,6 +285,7 @@ def batch_section(thr: float, reps: int = 5) -> dict:
         "workload": "256 water clusters (H2O)_n, n~U{1..8} (default_rng(256), cluster i seed 1000+i, 8 A cube), "
                     "6-31G* Cartesian, Schwarz 1e-12",
         "molecules": int(plan.nmol), "ao_total": int(plan.mol_n.sum()), "quartets": int(plan.info["nquartets"]),
+        "tuning": tuning,
         "plan_ms": plan_s * 1e3, "mixed_threshold": thr,
         "scf_mixed": {"wall_s": mix.timings["scf_s"], "molecules_per_s": plan.nmol / mix.timings["scf_s"],
                       "iterations": len(mix.iteration_s), "converged": int(mix.converged.sum()),
@@ -289,6 +311,9 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision builds and SCF")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-molecule batched workload")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-class schedules")
+    ap.add_argument("--retune", action="store_true",
+                    help="run the autotuner even where a committed schedule table exists")
+    ap.add_argument("--save-tuning", metavar="DIR", help="write autotuned schedule tables into DIR")
     ap.add_argument("--mixed-threshold", type=float, default=1e-7)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the copy kernel that calibrates the HBM byte counters")
@@ -334,10 +359,17 @@ def main() -> int:
     dens = scf.density.contiguous()
     tuning = None
     if not args.no_autotune:
-        from gradscf_amd.autotune import autotune
+        from gradscf_amd import autotune as at
         t0 = time.perf_counter()
-        tuning = autotune(lambda: DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff),
-                          lambda p: p.jk(dens[None], symmetric=True, validate=False), reps=3, log=log)
+        table_path = os.path.join(TUNING_DIR, "water16_ccpvdz_sph.json")
+        if os.path.isfile(table_path) and not args.retune:
+            tuning = {"source": os.path.relpath(table_path, ROOT), "table": at.load_table(table_path)}
+        else:
+            result = at.autotune(lambda: DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff),
+                                 lambda p: p.jk(dens[None], symmetric=True, validate=False), reps=3, log=log)
+            tuning = {"source": "autotuned in this run", "classes": result}
+            if args.save_tuning:
+                at.save_table(os.path.join(args.save_tuning, "water16_ccpvdz_sph.json"), result)
         tune_s = time.perf_counter() - t0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -448,7 +480,8 @@ def main() -> int:
         "classes": per_class,
         "autotune": tuning,
     }
-    result["batch"] = None if args.no_batch else batch_section(thr)
+    result["batch"] = None if args.no_batch else batch_section(thr, autotune=not args.no_autotune,
+                                                               retune=args.retune, save_dir=args.save_tuning)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(atm, bas, env, dens.cpu().numpy(), cutoff, args.cpu_builds,
                                               j_gpu, k_gpu)

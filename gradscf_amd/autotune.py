@@ -112,3 +112,24 @@ def autotune(make_plan, run, *, reps: int = 3, search=None, log=None) -> dict:
         set_schedule(best_s[cls], cls)
     return {class_name(cls): {"schedule": asdict(best_s[cls]), "ms": best_t[cls], "default_ms": default_t[cls]}
             for cls in best_s}
+
+
+def save_table(path, result: dict) -> None:
+    """Write the chosen per-class schedules ({class name: {"schedule": ...}})."""
+    import json
+    with open(path, "w") as f:
+        json.dump({k: v["schedule"] for k, v in sorted(result.items())}, f, indent=1, sort_keys=True)
+        f.write("\n")
+
+
+def load_table(path) -> dict:
+    """Install per-class schedules from a file written by save_table (classes
+    not listed keep the defaults); returns the table."""
+    import json
+    reset()
+    with open(path) as f:
+        table = json.load(f)
+    for name, sched in table.items():
+        cls = tuple(LETTERS.index(ch) for ch in name)
+        set_schedule(Schedule(**sched), cls)
+    return table
