@@ -301,6 +301,209 @@ def batch_section(thr: float, reps: int = 5, autotune: bool = True, retune: bool
     }
 
 
+def water_section() -> dict:
+    """Water (O 0 0 0; H 0 +-0.757 0.587 A), 6-31G Cartesian, core guess, DIIS,
+    conv_tol 1e-10: one J/K build and the full SCF on the GPU (the CPU
+    reference for the same workload is timed in the cpu_baseline leg)."""
+    import torch
+    from gradscf_amd import molecule as M
+    from gradscf_amd.basis import prepare_basis
+    from gradscf_amd.jk import DirectJKPlan
+    from gradscf_amd.packing import pack
+    from gradscf_amd.scf import rhf
+
+    spec = M.water_monomer()
+    atm, bas, env = pack(*prepare_basis(spec, "6-31g", cart=True))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan = DirectJKPlan(atm, bas, env, cart=True, cutoff=0.0)
+    torch.cuda.synchronize()
+    plan_ms = (time.perf_counter() - t0) * 1e3
+    res = rhf(plan, spec.nelectron, spec.nuclear_repulsion(), conv_tol=1e-10)
+    d = res.density.contiguous()
+    out = torch.empty((2, 1, plan.nao, plan.nao), dtype=torch.float64, device="cuda")
+    for _ in range(5):
+        plan.jk(d[None], symmetric=True, validate=False, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(100):
+        plan.jk(d[None], symmetric=True, validate=False, out=out)
+    torch.cuda.synchronize()
+    gpu_ms = (time.perf_counter() - t0) / 100 * 1e3
+    log(f"water 6-31G: GPU J/K {gpu_ms * 1e3:.1f} us/build, SCF {res.timings['scf_s']:.3f} s, E {res.energy:.10f}")
+    return {"workload": "water (O 0 0 0; H 0 +-0.757 0.587 A), 6-31G Cartesian, 13 AO",
+            "tables": (atm, bas, env), "spec": spec, "density": d.cpu().numpy(), "j": out[0, 0].cpu().numpy(),
+            "k": out[1, 0].cpu().numpy(),
+            "record": {"energy": res.energy, "gpu_jk_ms": gpu_ms, "gpu_plan_ms": plan_ms,
+                       "gpu_scf_s": res.timings["scf_s"], "gpu_scf_cycles": res.cycles}}
+
+
+def cpu_baseline_water(w: dict) -> dict:
+    """The reference CPU path on the water/6-31G workload: libcint J/K through
+    the restated DirectJK (one worker) and the restated RHF loop."""
+    import numpy as np
+    import oracle
+    from oracle import scf as oscf
+    atm, bas, env = w["tables"]
+    spec, dh = w["spec"], w["density"]
+    backend = "libcint" if oracle.have_ref() else "oracle"
+    with oracle.backend(backend):
+        oracle.direct_jk(atm, bas, env, dh, cart=True, cutoff=0.0, workers=1)
+        t0 = time.perf_counter()
+        for _ in range(20):
+            ref = oracle.direct_jk(atm, bas, env, dh, cart=True, cutoff=0.0, workers=1)
+        cpu_ms = (time.perf_counter() - t0) / 20 * 1e3
+        s, t, v = oracle.int1e(atm, bas, env, True)
+
+        def jk(dm):
+            o = oracle.direct_jk(atm, bas, env, dm, cart=True, cutoff=0.0, workers=1)
+            return o[0, 0], o[1, 0]
+
+        t0 = time.perf_counter()
+        ores = oscf.rhf(t + v, s, spec.nelectron, spec.nuclear_repulsion(), jk)
+        cpu_scf_s = time.perf_counter() - t0
+    err = float(max(np.max(np.abs(ref[0, 0] - w["j"])), np.max(np.abs(ref[1, 0] - w["k"]))))
+    log(f"water 6-31G CPU reference: J/K {cpu_ms:.3f} ms/build, SCF {cpu_scf_s:.3f} s, "
+        f"dE(GPU-CPU) {w['record']['energy'] - ores['energy']:.2e}")
+    return {"kind": "reference" if backend == "libcint" else "port", "cores": 1, "jk_ms": cpu_ms,
+            "scf_s": cpu_scf_s, "scf_cycles": ores.get("cycles"), "energy": ores["energy"],
+            "energy_diff_gpu_minus_cpu": w["record"]["energy"] - ores["energy"],
+            "jk_max_abs_diff_vs_gpu": err}
+
+
+def mixed_section(thr: float) -> dict:
+    """C20H42 (all-trans) and (H2O)32 (seed 32, 15 A cube), 6-31G* Cartesian,
+    Schwarz 1e-12: full RHF in FP64 and in mixed precision (FP32 for quartets
+    with Q_ij Q_kl max|D| < thr), energy error and J/K build times."""
+    import torch
+    from gradscf_amd import molecule as M
+    from gradscf_amd.basis import prepare_basis
+    from gradscf_amd.jk import DirectJKPlan
+    from gradscf_amd.packing import pack
+    from gradscf_amd.scf import rhf
+
+    out = {}
+    for name, spec in (("C20H42", M.linear_alkane(20)), ("(H2O)32", M.water_cluster(32, seed=32, box=15.0))):
+        atm, bas, env = pack(*prepare_basis(spec, "6-31g*", cart=True))
+        plan = DirectJKPlan(atm, bas, env, cart=True, cutoff=1e-12)
+        ref = rhf(plan, spec.nelectron, spec.nuclear_repulsion(), conv_tol=1e-10)
+        mix = rhf(plan, spec.nelectron, spec.nuclear_repulsion(), conv_tol=1e-10, jk_flags={"mixed_threshold": thr})
+        d = ref.density.contiguous()[None]
+
+        def timed(**kw):
+            plan.jk(d, symmetric=True, validate=False, **kw)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(10):
+                plan.jk(d, symmetric=True, validate=False, **kw)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / 10 * 1e3
+
+        ms64, msmx = timed(), timed(mixed_threshold=thr)
+        log(f"{name} 6-31G*: n={plan.nao} E={ref.energy:.10f} mixed dE={mix.energy - ref.energy:.2e} "
+            f"J/K {ms64:.2f} ms FP64 / {msmx:.2f} ms mixed; SCF {ref.timings['scf_s']:.3f} / "
+            f"{mix.timings['scf_s']:.3f} s")
+        out[name] = {"n_ao": int(plan.nao), "quartets": int(plan.info["nquartets"]), "energy_fp64": ref.energy,
+                     "energy_mixed": mix.energy, "energy_err": mix.energy - ref.energy,
+                     "converged": bool(ref.converged and mix.converged), "jk_ms_fp64": ms64, "jk_ms_mixed": msmx,
+                     "scf_s_fp64": ref.timings["scf_s"], "scf_s_mixed": mix.timings["scf_s"]}
+    out["threshold"] = thr
+    return out
+
+
+def gradient_section(thr: float) -> dict:
+    """(H2O)16 cc-pVDZ spherical: RHF nuclear gradient (all 48 nuclei) and
+    dE/d(exponent, raw coefficient) for every primitive, in FP64 and from the
+    mixed-precision SCF, with central finite-difference spot checks (1e-4 bohr,
+    1e-5 relative) on the FP64 gradient."""
+    import numpy as np
+    import torch
+    from gradscf_amd import molecule as M
+    from gradscf_amd.basis import BasisParameters, prepare_basis
+    from gradscf_amd.grad import rhf_basis_gradient, rhf_gradient, shell_nuclei
+    from gradscf_amd.jk import DirectJKPlan
+    from gradscf_amd.packing import pack
+    from gradscf_amd.scf import rhf
+
+    spec = M.water_cluster(16, seed=16, box=12.0)
+    topo, par = prepare_basis(spec, "cc-pvdz", cart=False)
+    atm, bas, env = pack(topo, par)
+    sh = shell_nuclei(par.centers, par.nuclear_coords)
+    nocc = spec.nelectron // 2
+
+    def scf(env_, mixed=None, tol=1e-10):
+        p = DirectJKPlan(atm, bas, env_, cart=False, cutoff=1e-12)
+        flags = {} if mixed is None else {"mixed_threshold": mixed}
+        return rhf(p, spec.nelectron, 0.0, conv_tol=tol, conv_tol_density=tol * 100, conv_tol_grad=tol * 10,
+                   jk_flags=flags)
+
+    res = scf(env, tol=1e-12)
+    mix = scf(env, mixed=thr, tol=1e-12)
+    gplan = DirectJKPlan(atm, bas, env, cart=False, cutoff=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = rhf_gradient(gplan, res.density, res.mo_coeff, res.mo_energy, nocc, spec.charges, spec.coords_bohr, sh)
+    torch.cuda.synchronize()
+    t_nuc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dexp, dcoef = rhf_basis_gradient(gplan, res.density, res.mo_coeff, res.mo_energy, nocc, bas, topo, par)
+    torch.cuda.synchronize()
+    t_par = time.perf_counter() - t0
+    gm = rhf_gradient(gplan, mix.density, mix.mo_coeff, mix.mo_energy, nocc, spec.charges, spec.coords_bohr, sh)
+    dexp_m, dcoef_m = rhf_basis_gradient(gplan, mix.density, mix.mo_coeff, mix.mo_energy, nocc, bas, topo, par)
+    checks = []
+    h = 1e-4
+    for a, ax in ((0, 0), (17, 2)):
+        rows = [a] + [len(spec.charges) + i for i in np.nonzero(sh == a)[0]]
+        e = []
+        for sgn in (1, -1):
+            env_ = np.array(env, copy=True)
+            for r in rows:
+                env_[int(atm[r, 1]) + ax] += sgn * h
+            e.append(scf(env_, tol=1e-12).energy)
+        cp, cm = spec.coords_bohr.copy(), spec.coords_bohr.copy()
+        cp[a, ax] += h
+        cm[a, ax] -= h
+        z = spec.charges.astype(float)
+
+        def enuc(r):
+            return sum(z[i] * z[j] / np.linalg.norm(r[i] - r[j]) for i in range(len(z)) for j in range(i))
+
+        fd = (e[0] - e[1]) / (2 * h) + (enuc(cp) - enuc(cm)) / (2 * h)
+        checks.append({"param": f"R[{a},{'xyz'[ax]}]", "analytic": float(g[a, ax]), "fd": fd,
+                       "abs_err": abs(fd - float(g[a, ax]))})
+    for s, p in ((2, 0), (4, 0)):
+        hh = 1e-5 * par.exponents[s][p]
+        e = []
+        for sgn in (1, -1):
+            ex = list(par.exponents)
+            arr = np.array(ex[s], dtype=float)
+            arr[p] += sgn * hh
+            ex[s] = arr
+            _, _, env_ = pack(topo, BasisParameters(tuple(ex), par.coefficients, par.centers, par.nuclear_coords))
+            e.append(scf(env_, tol=1e-12).energy)
+        fd = (e[0] - e[1]) / (2 * hh)
+        checks.append({"param": f"alpha[shell {s}, prim {p}]", "analytic": float(dexp[s][p]), "fd": fd,
+                       "abs_err": abs(fd - float(dexp[s][p]))})
+    max_fd = max(c["abs_err"] for c in checks)
+    dm = max(float(np.max(np.abs(gm - g))),
+             max(float(np.max(np.abs(np.asarray(x) - np.asarray(y)))) for x, y in zip(dexp_m, dexp)),
+             max(float(np.max(np.abs(np.asarray(x) - np.asarray(y)))) for x, y in zip(dcoef_m, dcoef)))
+    log(f"gradient (H2O)16 cc-pVDZ: nuclear {t_nuc:.3f} s, basis parameters {t_par:.3f} s, "
+        f"max |FD - analytic| {max_fd:.2e}, mixed vs FP64 {dm:.2e}")
+    nexp = sum(len(a) for a in par.exponents)
+    ncoef = sum(np.asarray(c).size for c in par.coefficients)
+    return {"workload": "(H2O)16 cc-pVDZ spherical RHF; nuclear gradient of 48 nuclei and dE/d(exponent, "
+                        "raw coefficient) of every primitive",
+            "n_exponents": int(nexp), "n_coefficients": int(ncoef),
+            "nuclear_gradient_s": t_nuc, "basis_gradient_s": t_par,
+            "max_abs_nuclear_gradient": float(np.max(np.abs(g))),
+            "translational_sum": float(np.max(np.abs(g.sum(axis=0)))),
+            "fd_checks": checks, "max_fd_abs_err": max_fd, "mixed_threshold": thr,
+            "mixed_vs_fp64_max_abs_gradient_diff": dm,
+            "scf_energy_fp64": res.energy, "scf_energy_mixed": mix.energy}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +514,8 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision builds and SCF")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-molecule batched workload")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-class schedules")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the water/6-31G, mixed C20H42/(H2O)32 and gradient sections")
     ap.add_argument("--retune", action="store_true",
                     help="run the autotuner even where a committed schedule table exists")
     ap.add_argument("--save-tuning", metavar="DIR", help="write autotuned schedule tables into DIR")
@@ -480,11 +685,19 @@ def main() -> int:
         "classes": per_class,
         "autotune": tuning,
     }
+    water = None
+    if not args.no_extra:
+        water = water_section()
+        result["water_631g"] = water["record"]
+        result["mixed_configs"] = mixed_section(thr)
+        result["gradients"] = gradient_section(thr)
     result["batch"] = None if args.no_batch else batch_section(thr, autotune=not args.no_autotune,
                                                                retune=args.retune, save_dir=args.save_tuning)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(atm, bas, env, dens.cpu().numpy(), cutoff, args.cpu_builds,
                                               j_gpu, k_gpu)
+        if water is not None:
+            result["cpu_baseline"]["water_631g"] = cpu_baseline_water(water)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

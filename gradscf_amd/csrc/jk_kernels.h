@@ -697,8 +697,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
 // global atomics.  Global FP64 atomics are executed at the memory side on this
 // platform, so their count, not FLOPs, is what this layout minimises.
 // With a.mixed set, a quartet whose Q_ij Q_kl max|D| (max over the six density
-// blocks it touches) is below a.thr is evaluated in FP32 (roots, weights and
-// prefactors stay FP64, digestion accumulates in FP64).
+// blocks it touches) is below a.thr may be evaluated in FP32 (roots, weights
+// and prefactors stay FP64, digestion accumulates in FP64); the choice is made
+// per wavefront (all lanes below the threshold), never per lane.
 // ----------------------------------------------------------------------------
 // With TLDS the Rys table slice of the class (rys_slice_size(NR) doubles) is
 // copied to LDS after the slabs, so the per-lane interval gathers of the root
@@ -743,6 +744,10 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
                                   fmax(dm[la * nsg + lc], dm[la * nsg + ld])),
                              fmax(dm[lb * nsg + lc], dm[lb * nsg + ld]));
       lowp = a.pair_q[qk.x] * a.pair_q[qk.y] * d6 < a.thr;
+      // wave-uniform: FP32 only when every active lane of the wavefront
+      // qualifies, so a wave never executes both precisions (neighbouring
+      // lanes hold Schwarz-adjacent quartets and mostly agree)
+      lowp = __all(lowp);
     }
     if (lowp)
       bra_item<LA, LB, LC, LD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, tab);

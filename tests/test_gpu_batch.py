@@ -75,7 +75,10 @@ def test_batch_mixed_precision_matches_single():
     got = plan.unpack(plan.jk(plan.pack(dens), symmetric=True, mixed_threshold=1e-7))
     for p, d, g in zip(singles, dens, got):
         ref = p.jk(d, symmetric=True, mixed_threshold=1e-7)
-        assert float((g - ref).abs().max()) < 1e-12
+        # precision is chosen per wavefront, and the two plans group quartets into
+        # waves differently: they may differ by FP32 rounding (2^-24 relative) of
+        # quartets whose bound is below 1e-7, a few hundred of them per element
+        assert float((g - ref).abs().max()) < 1e-11
 
 
 @pytest.mark.parametrize("cart", [True, False])

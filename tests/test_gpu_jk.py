@@ -350,6 +350,7 @@ def test_schedules_do_not_change_results(sched):
         plan = _plan(atm, bas, env, False, 1e-12)
         assert float((plan.jk(d) - ref).abs().max()) < 1e-12
         got_m = plan.jk(d[:1] * 1e-3, symmetric=False, mixed_threshold=1e-7)
-        assert float((got_m - ref_m).abs().max()) < 1e-12
+        # per-wavefront precision choice: FP32 rounding of sub-threshold quartets
+        assert float((got_m - ref_m).abs().max()) < 1e-11
     finally:
         at.reset()
