@@ -34,7 +34,8 @@ class PlanInfo(ctypes.Structure):
 
 class ClassTuning(ctypes.Structure):
     _fields_ = [("chunk_items", ctypes.c_int32), ("bras_max", ctypes.c_int32), ("lds_pack", ctypes.c_int32),
-                ("rys_lds", ctypes.c_int32), ("target_prim", ctypes.c_int32)]
+                ("rys_lds", ctypes.c_int32), ("target_prim", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("ket_lds", ctypes.c_int32)]
 
 
 class NativeError(RuntimeError):

@@ -28,10 +28,14 @@ class Schedule:
     lds_pack: int = 32 * 1024
     rys_lds: int = 0
     target_prim: int = 48
+    threads: int = 256
+    ket_lds: int = 0
 
 
 SEARCH = {
     "rys_lds": (0, 1),
+    "ket_lds": (0, 1),
+    "threads": (256, 512),
     "chunk_items": (2048, 4096, 8192),
     "bras_max": (8, 16, 32),
     "target_prim": (24, 48, 96),

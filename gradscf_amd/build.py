@@ -11,6 +11,7 @@ from __future__ import annotations
 import argparse
 import os
 import subprocess
+import sys
 from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
@@ -40,11 +41,12 @@ def _compile(src: Path, obj: Path, defines=()) -> None:
 
 
 def build(jobs: int | None = None) -> Path:
-    import importlib.util
-    spec = importlib.util.spec_from_file_location("gen_rys_tables", CSRC / "gen_rys_tables.py")
-    gen = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(gen)
-    gen.ensure(CSRC / "_generated" / "rys_table.inc")
+    # the generator fans out over processes, so it runs as its own program
+    # (it regenerates only when its source hash changed)
+    r = subprocess.run([sys.executable, str(CSRC / "gen_rys_tables.py"), "--out",
+                        str(CSRC / "_generated" / "rys_table.inc")], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Rys table generation failed:\n{r.stderr[-4000:]}")
     BUILD.mkdir(exist_ok=True)
     LIB.parent.mkdir(exist_ok=True)
     units = [(CSRC / "jk_engine.hip", BUILD / "jk_engine.o", ()),

@@ -87,6 +87,8 @@ struct BraChunk {
   int32_t S;
   int32_t bra0;
   int32_t nbra;
+  int32_t ket0;   // first ket pair of the chunk's ket window
+  int32_t nket;   // window length when the ket-side J is accumulated in LDS (0: global atomics)
 };
 
 struct BraArgs {
@@ -109,6 +111,8 @@ struct BraArgs {
   const double* pair_q;  // Schwarz bound per pair
   const double* dmax;    // per molecule [nseg][nseg] max |D| over shell blocks (all batches)
   int tab_lds;           // stage the class's Rys table slice in LDS (after the slabs)
+  int nket_max;          // LDS ket-J window capacity in ket pairs (0: ket-side J by global atomics)
+  int threads;           // workgroup size (256 or 512)
 };
 
 struct ParamArgs {

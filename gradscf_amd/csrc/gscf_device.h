@@ -25,9 +25,35 @@ __device__ static const double rys_tab[] = {
 // ---------------------------------------------------------------------------
 // Rys roots (s = t^2) and weights for weight exp(-x t^2) on [0,1]
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Reciprocal and reciprocal square root of finite, positive, normal FP64
+// arguments (exponent sums, Boys arguments): the hardware estimates refined
+// by two Newton-Raphson steps, r <- r + r (1 - x r) and
+// y <- y + y (1/2 - x y^2 / 2); each step squares the relative error, so two
+// steps from the estimate reach FP64 rounding (about 1 ulp).  The general
+// IEEE sequences handle zeros, denormals and infinities these arguments
+// never take, at three to four times the instruction count.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rcp_pos(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+__device__ __forceinline__ double rsqrt_pos(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  double t = fma(-hx * y, y, 0.5);
+  y = fma(y, t, y);
+  t = fma(-hx * y, y, 0.5);
+  return fma(y, t, y);
+}
+
 // `tab` is the N-root slice of the table (rys_tab + rys_table_offset[N - 1],
 // or a copy of it staged in LDS): RYS_NINT intervals of (RYS_DEG + 1) x 2N
-// Chebyshev coefficients, roots then weights.
+// monomial coefficients in y (degree-major), roots then weights.
 template <int N>
 __device__ __forceinline__ void rys_roots_tab(const double* __restrict__ tab, double x, double* __restrict__ s,
                                               double* __restrict__ w) {
@@ -35,27 +61,22 @@ __device__ __forceinline__ void rys_roots_tab(const double* __restrict__ tab, do
     const int k = (int)(x * (1.0 / RYS_H));
     const double y = x - (RYS_H * k + 0.5 * RYS_H);
     const double* __restrict__ c = tab + k * ((RYS_DEG + 1) * 2 * N);
-    double b1[2 * N], b2[2 * N];
+    // Horner in y on [-1, 1): one FMA per degree and function
+    double acc[2 * N];
 #pragma unroll
-    for (int f = 0; f < 2 * N; ++f) { b1[f] = 0.0; b2[f] = 0.0; }
-    const double y2 = 2.0 * y;
+    for (int f = 0; f < 2 * N; ++f) acc[f] = c[RYS_DEG * 2 * N + f];
 #pragma unroll
-    for (int j = RYS_DEG; j >= 1; --j) {
+    for (int j = RYS_DEG - 1; j >= 0; --j)
 #pragma unroll
-      for (int f = 0; f < 2 * N; ++f) {
-        const double b0 = fma(y2, b1[f], c[j * 2 * N + f] - b2[f]);
-        b2[f] = b1[f];
-        b1[f] = b0;
-      }
-    }
+      for (int f = 0; f < 2 * N; ++f) acc[f] = fma(acc[f], y, c[j * 2 * N + f]);
 #pragma unroll
     for (int f = 0; f < N; ++f) {
-      s[f] = fma(y, b1[f], c[f] - b2[f]);
-      w[f] = fma(y, b1[N + f], c[N + f] - b2[N + f]);
+      s[f] = acc[f];
+      w[f] = acc[N + f];
     }
   } else {
-    const double ix = 1.0 / x;
-    const double sq = sqrt(ix);
+    const double sq = rsqrt_pos(x);
+    const double ix = sq * sq;
     constexpr int base = N * (N - 1) / 2;
 #pragma unroll
     for (int f = 0; f < N; ++f) {

@@ -128,6 +128,8 @@ static gscf_class_tuning default_tuning() {
   t.lds_pack = 32 * 1024; // LDS bytes the slots of one workgroup may use
   t.rys_lds = 0;          // stage the Rys table slice in LDS
   t.target_prim = 48;     // primitive quartets per work item (primitive split)
+  t.threads = 256;        // workgroup size
+  t.ket_lds = 0;          // accumulate the ket-side J of a chunk's ket window in LDS
   return t;
 }
 
@@ -146,6 +148,8 @@ struct QuartetClass {
   int tuning_index = 0;
   int tab_lds = 0;          // Rys table slice staged in LDS
   size_t tab_bytes = 0;
+  int nket_max = 0;         // LDS ket-J window capacity (ket pairs), 0 = global atomics
+  int threads = 256;
   int64_t nq = 0, nitems = 0;
   std::vector<SubTask> tasks;
   SubTask* d_tasks = nullptr;
@@ -160,6 +164,7 @@ struct QuartetClass {
 // class's lds_pack so that several workgroups stay resident per CU; a single
 // slot (with the table, when staged) may use up to 64 KB.
 constexpr size_t kLdsBudget = 64 * 1024;
+constexpr size_t kKetLdsBytes = 16 * 1024;  // LDS ket-J window per density
 
 struct PlanData {
   gscf_class_tuning tuning[kNumQuartetClasses];
@@ -1077,6 +1082,8 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
                 c.S = t.S;
                 c.bra0 = pd.bra0 + b;
                 c.nbra = 1;
+                c.ket0 = pd.ket0 + (int32_t)k0;
+                c.nket = c.nq;
                 qc.chunks.push_back(c);
               }
               ++b;
@@ -1096,9 +1103,24 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
             }
             c.nbra = e - b;
             c.nq = (int32_t)(pd.qoff[e] - pd.qoff[b]);
+            c.ket0 = pd.ket0;
+            c.nket = 0;
+            for (int bb = b; bb < e; ++bb) c.nket = std::max<int32_t>(c.nket, (int32_t)(pd.qoff[bb + 1] - pd.qoff[bb]));
             qc.chunks.push_back(c);
             b = e;
           }
+        }
+        // ket-side J in LDS for chunks whose ket window fits kKetLdsBytes per density
+        qc.threads = tu.threads;
+        if (tu.ket_lds) {
+          const int nket_comp = ncomp_c * ncomp_d;
+          const int cap = (int)(kKetLdsBytes / (sizeof(double) * nket_comp));
+          for (BraChunk& c : qc.chunks) {
+            if (c.nket > cap) c.nket = 0;
+            qc.nket_max = std::max(qc.nket_max, c.nket);
+          }
+        } else {
+          for (BraChunk& c : qc.chunks) c.nket = 0;
         }
         if (qc.nbra_max >= 1) qc.d_chunks = dev_upload(qc.chunks, s);
       }
@@ -1287,6 +1309,10 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     const int ncomp_a = (qc.la + 1) * (qc.la + 2) / 2, ncomp_b = (qc.lb + 1) * (qc.lb + 2) / 2;
     size_t lds = sizeof(double) * ((size_t)nbd_eff * qc.nbra_max * (ncomp_a + ncomp_b) * P.nc_max +
                                    (size_t)nb * qc.nbra_max * ncomp_a * ncomp_b);
+    const int ncomp_c = (qc.lc + 1) * (qc.lc + 2) / 2, ncomp_d = (qc.ld + 1) * (qc.ld + 2) / 2;
+    int nket_max = with_j ? qc.nket_max : 0;
+    const size_t jket_bytes = sizeof(double) * (size_t)nb * nket_max * ncomp_c * ncomp_d;
+    if (nket_max && lds + jket_bytes <= kLdsBudget) lds += jket_bytes; else nket_max = 0;
     int tab_lds = qc.tab_lds;
     if (tab_lds && lds + qc.tab_bytes <= kLdsBudget) lds += qc.tab_bytes; else tab_lds = 0;
     if (qc.d_chunks && lds <= kLdsBudget) {
@@ -1313,6 +1339,8 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       b.pair_q = P.d_pair_q;
       b.dmax = P.d_dmax;
       b.tab_lds = tab_lds;
+      b.nket_max = nket_max;
+      b.threads = qc.threads;
       launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, b, lds, s);
     } else {
       JKArgs a;
@@ -1653,7 +1681,8 @@ int gscf_tuning_set(int32_t cls, const gscf_class_tuning* t) {
     if (cls < -1 || cls >= kNumQuartetClasses) throw InvalidArgument("tuning class index out of range");
     if (t->chunk_items < 64 || t->chunk_items > (1 << 20) || t->bras_max < 1 || t->bras_max > 256 ||
         t->lds_pack < 1024 || t->lds_pack > (int)kLdsBudget || (t->rys_lds != 0 && t->rys_lds != 1) ||
-        t->target_prim < 1 || t->target_prim > 4096)
+        t->target_prim < 1 || t->target_prim > 4096 || (t->threads != 256 && t->threads != 512) ||
+        (t->ket_lds != 0 && t->ket_lds != 1))
       throw InvalidArgument("tuning parameter out of range");
     ensure_tuning();
     for (int c = 0; c < kNumQuartetClasses; ++c)

@@ -60,7 +60,8 @@ template <int NR>
 __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q, const QuartetGeom& g,
                                            PrimQ<NR>& o, const double* __restrict__ tab) {
   const double pq = P.p + Q.p;
-  o.ipq = 1.0 / pq;
+  const double rs = rsqrt_pos(pq);  // 1/sqrt(p+q)
+  o.ipq = rs * rs;
   o.ip = P.ip;
   o.iq = Q.ip;
   o.PQ[0] = P.px - Q.px;
@@ -76,7 +77,7 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
   o.pf = P.p * o.ipq;  // rho / q
   const double rho = P.p * Q.p * o.ipq;
   const double T = rho * (o.PQ[0] * o.PQ[0] + o.PQ[1] * o.PQ[1] + o.PQ[2] * o.PQ[2]);
-  o.pref = P.e * Q.e * sqrt(o.ipq);
+  o.pref = P.e * Q.e * rs;
   rys_roots_tab<NR>(tab, T, o.s, o.w);
 }
 
@@ -470,7 +471,8 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
 template <int LA, int LB, int LC, int LD, typename R>
 __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const Pair& kp, const int2 qk, int slot,
                                          int s0, int S, int kc, double* __restrict__ krow,
-                                         double* __restrict__ jrow, const double* __restrict__ tab) {
+                                         double* __restrict__ jrow, double* __restrict__ jket, int kwin0,
+                                         const double* __restrict__ tab) {
   using K = Class<LA, LB, LC, LD>;
   constexpr int ROWS = K::NA + K::NB;
   const int ns = a.n;  // LDS row length
@@ -543,7 +545,10 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ia = 0; ia < K::NA; ++ia)
 #pragma unroll
               for (int ib = 0; ib < K::NB; ++ib) sj += (double)acc[ia][ib][ic][id] * dab[ia][ib];
-            atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * sj);
+            if (jket)
+              atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * K::NKET + ic * K::ND + id, f * sj);
+            else
+              atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * sj);
           }
       }
       if (a.with_k) {
@@ -644,7 +649,10 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             atomicAdd(jr + ia * K::NB + ib, f * (double)acc[ia][ib] * dcd);
             s2 += (double)acc[ia][ib] * (Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia]);
           }
-        atomicAdd(J + (size_t)c * n + d, f * s2);
+        if (jket)
+          atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * K::NKET + kc, f * s2);
+        else
+          atomicAdd(J + (size_t)c * n + d, f * s2);
       }
       if (a.with_k) {
         double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
@@ -705,7 +713,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
 // copied to LDS after the slabs, so the per-lane interval gathers of the root
 // evaluation read LDS instead of global memory.
 template <int LA, int LB, int LC, int LD, bool TLDS>
-__global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
+__global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   using K = Class<LA, LB, LC, LD>;
   constexpr int ROWS = K::NA + K::NB;
   extern __shared__ double lds[];
@@ -715,10 +723,14 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
   double* krow = lds;                      // [nbd][slot][ROWS][n]
   double* jrow = lds + krow_size;          // [nbj][slot][NA][NB]
   const int jrow_size = a.nbj * a.nbra_max * K::NBRA;
-  for (int i = threadIdx.x; i < krow_size + jrow_size; i += blockDim.x) lds[i] = 0.0;
+  // ket-side J of the chunk's ket window: [nbj][nket_max][NC*ND]
+  double* jket = (a.nket_max > 0 && ch.nket > 0) ? jrow + jrow_size : nullptr;
+  const int jket_size = a.nbj * a.nket_max * K::NKET;
+  const int zero_size = krow_size + jrow_size + (jket ? jket_size : 0);
+  for (int i = threadIdx.x; i < zero_size; i += blockDim.x) lds[i] = 0.0;
   const double* tab = rys_tab + rys_table_offset[K::NR - 1];
   if constexpr (TLDS) {
-    double* ltab = lds + ((krow_size + jrow_size + 1) & ~1);  // 16-byte aligned
+    double* ltab = lds + ((krow_size + jrow_size + jket_size + 1) & ~1);  // 16-byte aligned
     for (int i = threadIdx.x; i < rys_slice_size(K::NR); i += blockDim.x) ltab[i] = tab[i];
     tab = ltab;
   }
@@ -750,9 +762,9 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
       lowp = __all(lowp);
     }
     if (lowp)
-      bra_item<LA, LB, LC, LD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, tab);
+      bra_item<LA, LB, LC, LD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
     else
-      bra_item<LA, LB, LC, LD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, tab);
+      bra_item<LA, LB, LC, LD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
   }
   __syncthreads();
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
@@ -780,6 +792,20 @@ __global__ __launch_bounds__(256) void jk_bra_kernel(BraArgs a) {
     const MolInfo mi = a.mols[a.shells[bp.sa].mol];
     const int row = a.shells[bp.sa].ao + e / K::NB, col = a.shells[bp.sb].ao + e % K::NB;
     atomicAdd(a.jp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
+  }
+  if (jket) {
+    const int nk = a.nbj * ch.nket * K::NKET;
+    for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+      const int e = i % K::NKET;
+      const int w = (i / K::NKET) % ch.nket;
+      const int bt = i / (K::NKET * ch.nket);
+      const double v = jket[((size_t)bt * a.nket_max + w) * K::NKET + e];
+      if (v == 0.0) continue;
+      const Pair kp = a.pairs[ch.ket0 + w];
+      const MolInfo mi = a.mols[a.shells[kp.sa].mol];
+      const int row = a.shells[kp.sa].ao + e / K::ND, col = a.shells[kp.sb].ao + e % K::ND;
+      atomicAdd(a.jp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
+    }
   }
 }
 
@@ -858,10 +884,11 @@ void launch_blocks_t(const BlockArgs& a, hipStream_t s) {
 template <int LA, int LB, int LC, int LD>
 void launch_jk_bra_t(const BraArgs& a, size_t lds_bytes, hipStream_t s) {
   if (a.nchunks <= 0) return;
+  const dim3 block(a.threads);
   if (a.tab_lds)
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, true>), dim3((unsigned)a.nchunks), dim3(256), lds_bytes, s, a);
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, true>), dim3((unsigned)a.nchunks), block, lds_bytes, s, a);
   else
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, false>), dim3((unsigned)a.nchunks), dim3(256), lds_bytes, s, a);
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, false>), dim3((unsigned)a.nchunks), block, lds_bytes, s, a);
 }
 
 }  // namespace gscf

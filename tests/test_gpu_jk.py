@@ -334,7 +334,9 @@ def test_workspace_forecast_bounds_plan_memory(basis, cart, cutoff):
 
 
 @pytest.mark.parametrize("sched", [dict(rys_lds=1), dict(chunk_items=1024, bras_max=4, target_prim=12),
-                                   dict(rys_lds=1, chunk_items=8192, bras_max=32, target_prim=96)])
+                                   dict(rys_lds=1, chunk_items=8192, bras_max=32, target_prim=96),
+                                   dict(ket_lds=1), dict(ket_lds=1, threads=512, rys_lds=1, chunk_items=1024),
+                                   dict(threads=512, bras_max=1)])
 def test_schedules_do_not_change_results(sched):
     """Every tuning-table schedule (Rys table in LDS, chunk sizes, primitive
     split) gives the default schedule's J/K up to FP64 summation order."""

@@ -124,11 +124,9 @@ def test_rys_table_against_mpmath():
             k = int(x / 2)
             y = x - (2 * k + 1)
             c = tab[off[n - 1] + k * 28 * n: off[n - 1] + (k + 1) * 28 * n].reshape(14, 2 * n)
-            b1 = np.zeros(2 * n)
-            b2 = np.zeros(2 * n)
-            for jj in range(13, 0, -1):
-                b1, b2 = c[jj] + 2 * y * b1 - b2, b1
-            f = c[0] + y * b1 - b2
+            f = c[13].copy()
+            for jj in range(12, -1, -1):   # Horner in y, as the device evaluates it
+                f = f * y + c[jj]
             r, w = gen._rys_mp(n, float(x))
             assert np.max(np.abs(f[:n] - r) / np.array(r)) < 5e-15
             assert np.max(np.abs(f[n:] - w) / np.array(w)) < 5e-15
