@@ -712,7 +712,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
 // With TLDS the Rys table slice of the class (rys_slice_size(NR) doubles) is
 // copied to LDS after the slabs, so the per-lane interval gathers of the root
 // evaluation read LDS instead of global memory.
-template <int LA, int LB, int LC, int LD, bool TLDS>
+template <int LA, int LB, int LC, int LD, bool TLDS, bool MIX>
 __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   using K = Class<LA, LB, LC, LD>;
   constexpr int ROWS = K::NA + K::NB;
@@ -746,7 +746,7 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     const Pair bp = a.pairs[qk.x];
     const Pair kp = a.pairs[qk.y];
     bool lowp = false;
-    if (a.mixed) {
+    if constexpr (MIX) {
       const int la = a.shells[bp.sa].lseg, lb = a.shells[bp.sb].lseg;
       const int lc = a.shells[kp.sa].lseg, ld = a.shells[kp.sb].lseg;
       const MolInfo mi = a.mols[a.shells[bp.sa].mol];
@@ -761,10 +761,13 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
       // lanes hold Schwarz-adjacent quartets and mostly agree)
       lowp = __all(lowp);
     }
-    if (lowp)
-      bra_item<LA, LB, LC, LD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
-    else
-      bra_item<LA, LB, LC, LD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
+    if constexpr (MIX) {
+      if (lowp) {
+        bra_item<LA, LB, LC, LD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
+        continue;
+      }
+    }
+    bra_item<LA, LB, LC, LD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
   }
   __syncthreads();
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
@@ -884,11 +887,16 @@ void launch_blocks_t(const BlockArgs& a, hipStream_t s) {
 template <int LA, int LB, int LC, int LD>
 void launch_jk_bra_t(const BraArgs& a, size_t lds_bytes, hipStream_t s) {
   if (a.nchunks <= 0) return;
-  const dim3 block(a.threads);
-  if (a.tab_lds)
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, true>), dim3((unsigned)a.nchunks), block, lds_bytes, s, a);
+  const dim3 block(a.threads), grid((unsigned)a.nchunks);
+  // variants: Rys table in LDS or global; FP64 only, or with the FP32 quartet path
+  if (a.tab_lds && a.mixed)
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, true, true>), grid, block, lds_bytes, s, a);
+  else if (a.tab_lds)
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, true, false>), grid, block, lds_bytes, s, a);
+  else if (a.mixed)
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, false, true>), grid, block, lds_bytes, s, a);
   else
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, false>), dim3((unsigned)a.nchunks), block, lds_bytes, s, a);
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, false, false>), grid, block, lds_bytes, s, a);
 }
 
 }  // namespace gscf
