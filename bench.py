@@ -575,11 +575,13 @@ def main() -> int:
             tuning = {"source": os.path.relpath(table_path, ROOT), "table": json.load(f)}
     elif not args.no_autotune:
         t0 = time.perf_counter()
-        result = at.autotune(lambda: DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff),
-                             lambda p: p.jk(dens[None], symmetric=True, validate=False), reps=3, log=log)
-        tuning = {"source": "autotuned in this run", "classes": result}
+        result, opts = at.autotune_with_options(lambda: DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff),
+                                                lambda p: p.jk(dens[None], symmetric=True, validate=False),
+                                                reps=3, log=log)
+        tuning = {"source": "autotuned in this run", "classes": result, "options": opts}
         if args.save_tuning:
-            at.save_table(os.path.join(args.save_tuning, "water16_ccpvdz_sph.json"), result)
+            at.save_table(os.path.join(args.save_tuning, "water16_ccpvdz_sph.json"), result,
+                          {"general_contraction": opts["general_contraction"]})
         tune_s = time.perf_counter() - t0
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -17,7 +17,8 @@ EXPORTS = ("gscf_last_error", "gscf_plan_create", "gscf_plan_create_batch", "gsc
            "gscf_plan_destroy", "gscf_plan_get_info", "gscf_plan_get_schwarz", "gscf_plan_profile", "gscf_plan_class_times",
            "gscf_jk_build", "gscf_direct_jk", "gscf_eri_blocks", "gscf_int1e", "gscf_jk_geometry_vjp",
            "gscf_workspace_bytes", "gscf_int1e_gradient", "gscf_tuning_set", "gscf_tuning_get",
-           "gscf_tuning_reset", "gscf_jk_param_vjp", "gscf_int1e_param_vjp")
+           "gscf_tuning_reset", "gscf_jk_param_vjp", "gscf_int1e_param_vjp", "gscf_options_set",
+           "gscf_options_get")
 
 GSCF_OK, GSCF_INVALID_ARGUMENT, GSCF_INTERNAL = 0, 1, 2
 FLAG_SYMMETRIC_DENSITY = 1
@@ -36,6 +37,10 @@ class ClassTuning(ctypes.Structure):
     _fields_ = [("chunk_items", ctypes.c_int32), ("bras_max", ctypes.c_int32), ("lds_pack", ctypes.c_int32),
                 ("rys_lds", ctypes.c_int32), ("target_prim", ctypes.c_int32), ("threads", ctypes.c_int32),
                 ("ket_lds", ctypes.c_int32)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("general_contraction", ctypes.c_int32), ("streams", ctypes.c_int32)]
 
 
 class NativeError(RuntimeError):
@@ -72,6 +77,8 @@ def lib() -> ctypes.CDLL:
     handle.gscf_tuning_set.argtypes = [i32, vp]
     handle.gscf_tuning_get.argtypes = [i32, vp]
     handle.gscf_tuning_reset.argtypes = []
+    handle.gscf_options_set.argtypes = [vp]
+    handle.gscf_options_get.argtypes = [vp]
     handle.gscf_jk_param_vjp.argtypes = [vp, vp, vp, vp, vp, vp]
     handle.gscf_int1e_param_vjp.argtypes = [vp, vp, vp, vp, vp]
     handle.gscf_workspace_bytes.argtypes = [vp, i32, vp, i32, vp, i32, i32, dbl, i32, vp]

@@ -68,6 +68,54 @@ def reset() -> None:
     _native.check(_native.lib().gscf_tuning_reset())
 
 
+def get_options() -> dict:
+    """Process-wide engine options (general contraction, launch streams)."""
+    o = _native.Options()
+    _native.check(_native.lib().gscf_options_get(ctypes.byref(o)))
+    return {f: getattr(o, f) for f, _ in _native.Options._fields_}
+
+
+def set_options(**kw) -> None:
+    """Change engine options; plans created afterwards use them."""
+    cur = get_options()
+    unknown = set(kw) - set(cur)
+    if unknown:
+        raise ValueError(f"unknown engine options {sorted(unknown)}")
+    cur.update(kw)
+    o = _native.Options(**cur)
+    _native.check(_native.lib().gscf_options_set(ctypes.byref(o)))
+
+
+def build_ms(make_plan, run, reps: int = 5) -> float:
+    """Average wall time of run(plan) on the current stream (HIP events), with
+    class launches spread over streams as in production."""
+    import torch
+    plan = make_plan()
+    run(plan)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        run(plan)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def choose_options(make_plan, run, *, reps: int = 5, log=None) -> dict:
+    """Time whole builds with and without native general contraction and keep
+    the faster; returns {"general_contraction": 0|1, "ms": {...}}."""
+    ms = {}
+    for gc in (1, 0):
+        set_options(general_contraction=gc)
+        ms[gc] = build_ms(make_plan, run, reps)
+        if log:
+            log(f"options general_contraction={gc}: {ms[gc]:.3f} ms per build")
+    best = min(ms, key=ms.get)
+    set_options(general_contraction=best)
+    return {"general_contraction": best, "ms": ms}
+
+
 def _class_times(make_plan, run, reps):
     import torch
     plan = make_plan()
@@ -118,22 +166,51 @@ def autotune(make_plan, run, *, reps: int = 3, search=None, log=None) -> dict:
             for cls in best_s}
 
 
-def save_table(path, result: dict) -> None:
-    """Write the chosen per-class schedules ({class name: {"schedule": ...}})."""
+def autotune_with_options(make_plan, run, *, reps: int = 3, log=None):
+    """Tune the per-class schedules once with native general contraction and
+    once without, keep whichever gives the faster whole build, and install it.
+    Returns (per-class result as from autotune, {"general_contraction": ...,
+    "build_ms": {0: ms, 1: ms}})."""
+    runs = {}
+    for gc in (1, 0):
+        set_options(general_contraction=gc)
+        res = autotune(make_plan, run, reps=reps, log=log)
+        ms = build_ms(make_plan, run)
+        runs[gc] = (ms, res)
+        if log:
+            log(f"autotune general_contraction={gc}: {ms:.3f} ms per build")
+    gc = min(runs, key=lambda k: runs[k][0])
+    set_options(general_contraction=gc)
+    reset()
+    for name, v in runs[gc][1].items():
+        set_schedule(Schedule(**v["schedule"]), tuple(LETTERS.index(ch) for ch in name))
+    return runs[gc][1], {"general_contraction": gc, "build_ms": {k: v[0] for k, v in runs.items()}}
+
+
+def save_table(path, result: dict, options: dict | None = None) -> None:
+    """Write the chosen per-class schedules ({class name: {"schedule": ...}})
+    and, when given, the engine options under the key "_options"."""
     import json
+    table = {k: v["schedule"] for k, v in sorted(result.items())}
+    if options:
+        table["_options"] = dict(options)
     with open(path, "w") as f:
-        json.dump({k: v["schedule"] for k, v in sorted(result.items())}, f, indent=1, sort_keys=True)
+        json.dump(table, f, indent=1, sort_keys=True)
         f.write("\n")
 
 
 def load_table(path) -> dict:
-    """Install per-class schedules from a file written by save_table (classes
-    not listed keep the defaults); returns the table."""
+    """Install per-class schedules (and engine options, if recorded) from a
+    file written by save_table; classes not listed keep the defaults.
+    Returns the table."""
     import json
     reset()
     with open(path) as f:
         table = json.load(f)
     for name, sched in table.items():
+        if name == "_options":
+            set_options(**sched)
+            continue
         cls = tuple(LETTERS.index(ch) for ch in name)
         set_schedule(Schedule(**sched), cls)
     return table

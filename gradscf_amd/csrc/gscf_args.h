@@ -26,6 +26,20 @@ struct PrimAux {
   double ek;
 };
 
+// Column coefficients (s/p factor included) of a primitive pair of a
+// contracted pair: primitive i of shell A in columns 0/1, primitive j of shell
+// B in columns 0/1 (zero where a shell has one column).  The primitive pair's
+// e then carries no coefficients.
+struct PrimCoef {
+  double ca[2];
+  double cb[2];
+};
+
+// Column pattern of a pair: columns of (A, B) = (1 + (p & 1), 1 + (p >> 1)).
+__host__ __device__ constexpr bool pattern_valid(int la, int lb, int p) {
+  return p == 0 || (la == 0 && lb == 0 && (p == 1 || p == 3)) || (la > 0 && lb == 0 && p == 2);
+}
+
 struct Pair {
   int sa, sb;      // segmented shells, l(sa) >= l(sb)
   int poff, np;    // primitive-pair range
@@ -110,6 +124,7 @@ struct BraArgs {
   double thr;
   const double* pair_q;  // Schwarz bound per pair
   const double* dmax;    // per molecule [nseg][nseg] max |D| over shell blocks (all batches)
+  const PrimCoef* coef;  // column coefficients of a contracted pair set (general patterns)
   int tab_lds;           // stage the class's Rys table slice in LDS (after the slabs)
   int nket_max;          // LDS ket-J window capacity in ket pairs (0: ket-side J by global atomics)
   int threads;           // workgroup size (256 or 512)

@@ -11,8 +11,9 @@ namespace gscf {
 void GSCF_NAME(launch_jk_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const JKArgs& a, hipStream_t s) {
   launch_jk_t<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, s);
 }
-void GSCF_NAME(launch_jk_bra_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const BraArgs& a, size_t lds, hipStream_t s) {
-  launch_jk_bra_t<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, lds, s);
+bool GSCF_NAME(launch_jk_bra_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const BraArgs& a, int pb, int pk, size_t lds,
+                                                                   hipStream_t s) {
+  return launch_jk_bra_patterns<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, pb, pk, lds, s);
 }
 void GSCF_NAME(launch_grad_c, GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD)(const GradArgs& a, hipStream_t s) {
   launch_grad_t<GSCF_LA, GSCF_LB, GSCF_LC, GSCF_LD>(a, s);

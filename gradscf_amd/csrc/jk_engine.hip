@@ -139,18 +139,24 @@ static void ensure_tuning() {
   g_tuning_init = true;
 }
 
+static gscf_options g_options = {1, 4};
+constexpr int kMaxStreams = 4;
+
 static size_t rys_slice_bytes(int nr) {
   return sizeof(double) * ((size_t)RYS_NINT * (RYS_DEG + 1) * 2 * nr + 2);  // + alignment pad
 }
 
 struct QuartetClass {
   int la, lb, lc, ld;
+  int pb = 0, pk = 0;       // column patterns of bra and ket pairs (contracted sets)
   int tuning_index = 0;
   int tab_lds = 0;          // Rys table slice staged in LDS
   size_t tab_bytes = 0;
   int nket_max = 0;         // LDS ket-J window capacity (ket pairs), 0 = global atomics
   int threads = 256;
+  size_t per_slot = 0;      // LDS bytes of one bra slot for one density
   int64_t nq = 0, nitems = 0;
+  double cost = 0.0;        // estimated work, for spreading launches over streams
   std::vector<SubTask> tasks;
   SubTask* d_tasks = nullptr;
   int2* d_quartets = nullptr;
@@ -166,8 +172,30 @@ struct QuartetClass {
 constexpr size_t kLdsBudget = 64 * 1024;
 constexpr size_t kKetLdsBytes = 16 * 1024;  // LDS ket-J window per density
 
+// The shell pairs quartet kernels run over.  Segmented sets hold one
+// contraction column per shell; contracted sets (J/K only) hold parent shells
+// whose general-contraction columns are evaluated from one primitive
+// integral each (column pattern per pair: (1,1), (2,1), (2,2) for ss pairs,
+// (1,2) for ps/ds pairs with a general s shell).
+struct PairSet {
+  bool contracted = false;
+  bool built = false;
+  ShellXYZ* d_shells = nullptr;   // owned only by contracted sets
+  PrimPair* d_prims = nullptr;    // owned only by contracted sets
+  PrimCoef* d_coef = nullptr;     // contracted sets: column coefficients per primitive pair
+  int* d_blk = nullptr;           // [nshell][2]: AO offset and width of each shell's density block
+  std::vector<Pair> pairs;        // retained, class/pattern/molecule/bucket sorted
+  std::vector<double> pair_q;
+  Pair* d_pairs = nullptr;
+  double* d_pair_q = nullptr;
+  std::vector<QuartetClass> classes;
+  int64_t nquartets = 0;
+  int64_t nprims = 0;             // primitive pairs owned by a contracted set
+};
+
 struct PlanData {
   gscf_class_tuning tuning[kNumQuartetClasses];
+  gscf_options options;
   int cart = 1, n = 0, n_cart = 0, nbas = 0, natm = 0;
   double cutoff = 0.0, plan_ms = 0.0;
   std::vector<int> atm, bas, ao, ao_cart;  // ao/ao_cart: global AO offsets of every shell
@@ -186,16 +214,16 @@ struct PlanData {
   // all intra-molecule segmented pairs (unscreened) for block queries
   std::vector<Pair> all_pairs;
   std::vector<int> all_class;
-  // screened, class-sorted pairs
-  std::vector<Pair> pairs;
-  std::vector<double> pair_q;
-  int class_begin[kNumPairClasses + 1] = {0};
-  std::vector<QuartetClass> classes;
-  int64_t nquartets = 0;
+  // J/K pair set and the segmented set of the derivative kernels (the same
+  // object when J/K runs on segmented pairs; built on first use otherwise)
+  PairSet jk;
+  PairSet seg_set;
+  std::vector<double> seg_pair_q;  // parent-shell Schwarz bound of every all_pairs entry
+  std::vector<double> qmax_mol;    // largest bound per molecule
+  PairSet& segq() { return jk.contracted ? seg_set : jk; }
   // device
   ShellXYZ* d_shells = nullptr;
   PrimPair* d_prims = nullptr;
-  Pair* d_pairs = nullptr;
   Pair* d_all_pairs = nullptr;
   int* d_c2s_idx = nullptr;     // [n_cart][2] spherical AO feeding a Cartesian AO
   double* d_c2s_coef = nullptr;
@@ -219,10 +247,9 @@ struct PlanData {
   // parent-shell Schwarz bounds, one nbas_m x nbas_m block per molecule
   // (lower triangle, local shell indices; empty when cutoff == 0)
   std::vector<double> parent_q;
-  // mixed precision: per-pair bounds on device, per-build density block maxima
-  double* d_pair_q = nullptr;
+  std::vector<int64_t> qbase;      // start of each molecule's parent_q block
+  // mixed precision: per-build density block maxima over the J/K shell blocks
   double* d_dmax = nullptr;
-  int* d_seg_ao = nullptr;   // [nseg][2]: Cartesian AO offset, ncart
   int* d_seg_atom = nullptr; // atm row of each segmented shell
   // basis-parameter derivatives
   PrimAux* d_aux = nullptr;
@@ -234,7 +261,15 @@ struct PlanData {
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
-  std::vector<std::pair<int, size_t>> prof_recs;  // (class index, event pair offset)
+  struct ProfRec { int cls; size_t ev; int64_t build; };
+  std::vector<ProfRec> prof_recs;  // (canonical class, event pair offset, build number)
+  int64_t prof_builds = 0;
+  // class launches of one build spread over the caller's stream and side
+  // streams (default priority, non-blocking), joined back by events
+  std::vector<hipStream_t> side;
+  hipEvent_t fork_ev = nullptr;
+  std::vector<hipEvent_t> join_ev;
+  std::vector<std::vector<int>> lane_classes;  // class indices per stream (0 = caller's)
 };
 
 // Per-class launchers, one translation unit each (jk_class.hip).
@@ -261,7 +296,7 @@ void launch_grad1e_class(int la, int lb, const Pair* pairs, int npairs, const Pr
                          const MolInfo* mols, const double* dens, const double* wdens, int natm, double* grad,
                          hipStream_t st);
 
-#define X(A, B, C, D) void launch_jk_bra_c##A##B##C##D(const BraArgs& a, size_t lds, hipStream_t s);
+#define X(A, B, C, D) bool launch_jk_bra_c##A##B##C##D(const BraArgs& a, int pb, int pk, size_t lds, hipStream_t s);
 GSCF_CLASSES(X)
 #undef X
 #define X(A, B, C, D) void launch_grad_c##A##B##C##D(const GradArgs& a, hipStream_t s);
@@ -294,11 +329,13 @@ void launch_grad_class(int la, int lb, int lc, int ld, const GradArgs& a, hipStr
   throw Internal("no kernel for this angular-momentum class");
 }
 
-void launch_jk_bra_class(int la, int lb, int lc, int ld, const BraArgs& a, size_t lds, hipStream_t s) {
-#define X(A, B, C, D)                                      \
-  if (la == A && lb == B && lc == C && ld == D) {          \
-    launch_jk_bra_c##A##B##C##D(a, lds, s);                \
-    return;                                                \
+void launch_jk_bra_class(int la, int lb, int lc, int ld, int pb, int pk, const BraArgs& a, size_t lds,
+                         hipStream_t s) {
+#define X(A, B, C, D)                                                                 \
+  if (la == A && lb == B && lc == C && ld == D) {                                     \
+    if (!launch_jk_bra_c##A##B##C##D(a, pb, pk, lds, s))                              \
+      throw Internal("no kernel for this column pattern");                            \
+    return;                                                                           \
   }
   GSCF_CLASSES(X)
 #undef X
@@ -518,6 +555,7 @@ static void validate_tables(PlanData& P, const int32_t* atm, int natm, const int
     throw InvalidArgument("Invalid libcint table dimensions or operator");
   ensure_tuning();
   std::copy(g_tuning, g_tuning + kNumQuartetClasses, P.tuning);
+  P.options = g_options;
   if (nmol < 1 || (nmol > 1 && (!mol_atm || !mol_bas)))
     throw InvalidArgument("A batched plan needs per-molecule atm and bas offsets");
   P.nmol = nmol;
@@ -826,6 +864,440 @@ static void build_c2s_maps(PlanData& P, hipStream_t s) {
   P.d_s2c_coef = dev_upload(scoef, s);
 }
 
+static int bucket_of(int np) {
+  int b = 0;
+  while ((2 << b) <= np) ++b;
+  return b;
+}
+
+struct PairCand {
+  Pair pr;
+  int cls, pat, mol;
+  double q;
+};
+
+// Screen candidate pairs (pair prune Q_ij Qmax < cutoff), sort them by (pair
+// class, molecule, column pattern, primitive-count bucket, Schwarz
+// descending) and organise canonical quartets: for every (bra run, ket run)
+// of one molecule a sub-task whose surviving kets form a prefix of the ket
+// run; one QuartetClass (one launch) per (bra class, ket class, bra pattern,
+// ket pattern); bra chunks for the LDS kernel when with_chunks.
+static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand, bool with_chunks, hipStream_t s) {
+  const double cutoff = P.cutoff;
+  std::vector<int> keep;
+  for (int i = 0; i < (int)cand.size(); ++i) {
+    if (cand[i].pr.np == 0) continue;
+    if (cutoff > 0 && cand[i].q * P.qmax_mol[cand[i].mol] < cutoff) continue;
+    keep.push_back(i);
+  }
+  std::stable_sort(keep.begin(), keep.end(), [&](int x, int y) {
+    const PairCand& a = cand[x];
+    const PairCand& b = cand[y];
+    if (a.cls != b.cls) return a.cls < b.cls;
+    if (a.mol != b.mol) return a.mol < b.mol;
+    if (a.pat != b.pat) return a.pat < b.pat;
+    const int bx = bucket_of(a.pr.np), by = bucket_of(b.pr.np);
+    if (bx != by) return bx > by;
+    return a.q > b.q;
+  });
+  S.pairs.clear();
+  S.pair_q.clear();
+  for (int i : keep) {
+    S.pairs.push_back(cand[i].pr);
+    S.pair_q.push_back(cand[i].q);
+  }
+  struct Run { int cls, mol, pat, bkt, beg, end, npmin, npmax; };
+  std::vector<Run> runs;
+  for (int k = 0; k < (int)keep.size();) {
+    const PairCand& c0 = cand[keep[k]];
+    Run r{c0.cls, c0.mol, c0.pat, bucket_of(c0.pr.np), k, k, 1 << 30, 0};
+    while (k < (int)keep.size()) {
+      const PairCand& c = cand[keep[k]];
+      if (c.cls != r.cls || c.mol != r.mol || c.pat != r.pat || bucket_of(c.pr.np) != r.bkt) break;
+      r.npmin = std::min(r.npmin, c.pr.np);
+      r.npmax = std::max(r.npmax, c.pr.np);
+      ++k;
+    }
+    r.end = k;
+    runs.push_back(r);
+  }
+  // runs of each (class, molecule) in list order, for the intra-molecule pairing
+  std::map<std::pair<int, int>, std::vector<int>> runs_of;
+  for (int i = 0; i < (int)runs.size(); ++i) runs_of[{runs[i].cls, runs[i].mol}].push_back(i);
+  S.d_pairs = dev_upload(S.pairs, s);
+  {
+    std::vector<double> pq = S.pair_q;
+    if (pq.empty()) pq.push_back(0.0);
+    S.d_pair_q = dev_upload(pq, s);
+  }
+  static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
+  S.nquartets = 0;
+  for (int cb = 0; cb < kNumPairClasses; ++cb)
+    for (int ck = 0; ck <= cb; ++ck)
+      for (int pb = 0; pb < 4; ++pb)
+        for (int pk = 0; pk < 4; ++pk) {
+          if (!pattern_valid(pl[cb][0], pl[cb][1], pb) || !pattern_valid(pl[ck][0], pl[ck][1], pk)) continue;
+          if (!S.contracted && (pb || pk)) continue;
+          QuartetClass qc;
+          qc.la = pl[cb][0]; qc.lb = pl[cb][1]; qc.lc = pl[ck][0]; qc.ld = pl[ck][1];
+          qc.pb = pb;
+          qc.pk = pk;
+          qc.tuning_index = cb * (cb + 1) / 2 + ck;
+          const gscf_class_tuning& tu = P.tuning[qc.tuning_index];
+          const int64_t kTargetPrim = tu.target_prim, kChunkItems = tu.chunk_items;
+          const int kChunkBrasMax = tu.bras_max;
+          qc.nq = 0;
+          qc.nitems = 0;
+          struct Pending { std::vector<int64_t> qoff; int bra0, ket0, nbra; int64_t q0, nq; };
+          std::vector<Pending> pend;
+          for (int rbi = 0; rbi < (int)runs.size(); ++rbi) {
+            const Run& rb = runs[rbi];
+            if (rb.cls != cb || rb.pat != pb) continue;
+            auto it = runs_of.find({ck, rb.mol});
+            if (it == runs_of.end()) continue;
+            for (int rki : it->second) {
+              const Run& rk = runs[rki];
+              if (rk.pat != pk) continue;
+              // canonical: within one class each unordered pair of runs once (bra run
+              // not after the ket run in list order), a run with itself as a triangle
+              if (cb == ck && rbi > rki) continue;
+              const bool same_list = rbi == rki;
+              const int nbra = rb.end - rb.beg, nket = rk.end - rk.beg;
+              std::vector<int64_t> qoff(nbra + 1, 0);
+              for (int b = 0; b < nbra; ++b) {
+                int cnt = nket;
+                if (cutoff > 0) {
+                  const double qb = S.pair_q[rb.beg + b];
+                  int lo = 0, hi = nket;
+                  while (lo < hi) {
+                    const int mid = (lo + hi) / 2;
+                    if (qb * S.pair_q[rk.beg + mid] < cutoff) hi = mid; else lo = mid + 1;
+                  }
+                  cnt = lo;
+                }
+                if (same_list) cnt = std::min(cnt, b + 1);
+                qoff[b + 1] = qoff[b] + cnt;
+              }
+              const int64_t nq = qoff[nbra];
+              if (nq == 0) continue;
+              SubTask t;
+              t.q0 = qc.nq;
+              t.nq = nq;
+              const int64_t work = (int64_t)rb.npmax * rk.npmax;
+              t.S = (int)std::max<int64_t>(1, std::min<int64_t>(rb.npmin, (work + kTargetPrim - 1) / kTargetPrim));
+              t.pad = 0;
+              t.item0 = qc.nitems;
+              qc.nitems += (int64_t)t.S * nq;
+              {
+                // rough work model: primitive quartets weighted by the class's
+                // Cartesian block size, plus a per-quartet digestion term
+                const double nblk = (double)ncart_h(qc.la) * ncart_h(qc.lb) * ncart_h(qc.lc) * ncart_h(qc.ld);
+                const double cols = (double)(1 + (pb & 1)) * (1 + (pb >> 1)) * (1 + (pk & 1)) * (1 + (pk >> 1));
+                const double npq = 0.25 * (double)(rb.npmin + rb.npmax) * (rk.npmin + rk.npmax);
+                qc.cost += (double)nq * (npq * (8.0 + 0.5 * nblk + 0.25 * nblk * (cols - 1.0)) + 4.0 * nblk * cols + 64.0);
+              }
+              qc.nq += nq;
+              qc.tasks.push_back(t);
+              pend.push_back({std::move(qoff), rb.beg, rk.beg, nbra, t.q0, nq});
+            }
+          }
+          if (qc.nq == 0) continue;
+          if (with_chunks) {
+            // bra-major chunks: consecutive bras of one sub-task, bounded LDS slots and lane-items;
+            // widths count every contraction column of a shell
+            const int wa = ncart_h(qc.la) * (1 + (pb & 1)), wb = ncart_h(qc.lb) * (1 + (pb >> 1));
+            const int wc = ncart_h(qc.lc) * (1 + (pk & 1)), wd = ncart_h(qc.ld) * (1 + (pk >> 1));
+            const int rows = wa + wb;
+            const int split = (wa * wb * wc * wd > 36) ? wc * wd : 1;
+            const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb);
+            const int nr = (qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1;
+            size_t pack = (size_t)tu.lds_pack;
+            if (tu.rys_lds && per_slot + rys_slice_bytes(nr) <= kLdsBudget) {
+              qc.tab_lds = 1;
+              qc.tab_bytes = rys_slice_bytes(nr);
+              pack = pack > qc.tab_bytes + per_slot ? pack - qc.tab_bytes : per_slot;
+            }
+            qc.per_slot = per_slot;
+            qc.nbra_max = per_slot > kLdsBudget ? 0
+                          : (int)std::max<size_t>(1, std::min<size_t>((size_t)kChunkBrasMax, pack / per_slot));
+            for (size_t ti = 0; qc.nbra_max >= 1 && ti < pend.size(); ++ti) {
+              const Pending& pd = pend[ti];
+              const SubTask& t = qc.tasks[ti];
+              const int64_t per_q = (int64_t)t.S * split;
+              int b = 0;
+              while (b < pd.nbra) {
+                const int64_t cnt = pd.qoff[b + 1] - pd.qoff[b];
+                if (cnt == 0) { ++b; continue; }
+                if (cnt * per_q > kChunkItems) {
+                  // one long bra: several single-bra chunks
+                  const int64_t step = std::max<int64_t>(1, kChunkItems / per_q);
+                  for (int64_t k0 = 0; k0 < cnt; k0 += step) {
+                    BraChunk c;
+                    c.q0 = t.q0 + pd.qoff[b] + k0;
+                    c.nq = (int32_t)std::min<int64_t>(step, cnt - k0);
+                    c.S = t.S;
+                    c.bra0 = pd.bra0 + b;
+                    c.nbra = 1;
+                    c.ket0 = pd.ket0 + (int32_t)k0;
+                    c.nket = c.nq;
+                    qc.chunks.push_back(c);
+                  }
+                  ++b;
+                  continue;
+                }
+                BraChunk c;
+                c.q0 = t.q0 + pd.qoff[b];
+                c.S = t.S;
+                c.bra0 = pd.bra0 + b;
+                int64_t items = 0;
+                int e = b;
+                while (e < pd.nbra && e - b < qc.nbra_max) {
+                  const int64_t ce = pd.qoff[e + 1] - pd.qoff[e];
+                  if (items > 0 && (items + ce * per_q > kChunkItems || ce * per_q > kChunkItems)) break;
+                  items += ce * per_q;
+                  ++e;
+                }
+                c.nbra = e - b;
+                c.nq = (int32_t)(pd.qoff[e] - pd.qoff[b]);
+                c.ket0 = pd.ket0;
+                c.nket = 0;
+                for (int bb = b; bb < e; ++bb)
+                  c.nket = std::max<int32_t>(c.nket, (int32_t)(pd.qoff[bb + 1] - pd.qoff[bb]));
+                qc.chunks.push_back(c);
+                b = e;
+              }
+            }
+            // ket-side J in LDS for chunks whose ket window fits kKetLdsBytes per density
+            qc.threads = tu.threads;
+            if (tu.ket_lds) {
+              const int cap = (int)(kKetLdsBytes / (sizeof(double) * wc * wd));
+              for (BraChunk& c : qc.chunks) {
+                if (c.nket > cap) c.nket = 0;
+                qc.nket_max = std::max(qc.nket_max, c.nket);
+              }
+            } else {
+              for (BraChunk& c : qc.chunks) c.nket = 0;
+            }
+            if (qc.nbra_max >= 1) qc.d_chunks = dev_upload(qc.chunks, s);
+          }
+          HIP_OK(hipMalloc(&qc.d_quartets, sizeof(int2) * qc.nq));
+          for (const Pending& pd : pend) {
+            int64_t* d_qoff = dev_upload(pd.qoff, s);
+            hipLaunchKernelGGL(expand_kernel, dim3(nblocks((size_t)pd.nq)), dim3(256), 0, s, d_qoff, pd.nbra,
+                               pd.bra0, pd.ket0, pd.nq, qc.d_quartets + pd.q0);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipStreamSynchronize(s));
+            HIP_OK(hipFree(d_qoff));
+          }
+          qc.d_tasks = dev_upload(qc.tasks, s);
+          S.nquartets += qc.nq;
+          S.classes.push_back(std::move(qc));
+        }
+  S.built = true;
+}
+
+static void free_pair_set(PairSet& S) {
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  if (S.contracted) {
+    f(S.d_shells);
+    f(S.d_prims);
+  }
+  f(S.d_coef); f(S.d_blk); f(S.d_pairs); f(S.d_pair_q);
+  for (auto& c : S.classes) {
+    f(c.d_quartets);
+    f(c.d_tasks);
+    f(c.d_chunks);
+  }
+  S = PairSet();
+}
+
+// Segmented pair set over every intra-molecule segment pair; its density
+// blocks are the segments.
+static void build_segmented_set(PlanData& P, PairSet& S, bool with_chunks, hipStream_t s) {
+  S.contracted = false;
+  S.d_shells = P.d_shells;
+  S.d_prims = P.d_prims;
+  std::vector<PairCand> cand;
+  cand.reserve(P.all_pairs.size());
+  for (size_t i = 0; i < P.all_pairs.size(); ++i)
+    cand.push_back({P.all_pairs[i], P.all_class[i], 0, P.seg[P.all_pairs[i].sa].mol, P.seg_pair_q[i]});
+  if (&S == &P.jk) {
+    std::vector<int> blk;
+    for (const Seg& g : P.seg) {
+      blk.push_back(g.ao_cart);
+      blk.push_back(ncart_h(g.l));
+    }
+    S.d_blk = dev_upload(blk, s);
+  }
+  build_pair_set(P, S, cand, with_chunks, s);
+}
+
+// The derivative kernels run over segmented pairs (no chunks needed).
+static PairSet& ensure_seg_set(PlanData& P, hipStream_t s) {
+  PairSet& S = P.segq();
+  if (!S.built) {
+    build_segmented_set(P, S, false, s);
+    HIP_OK(hipStreamSynchronize(s));
+  }
+  return S;
+}
+
+// General contraction is evaluated natively for s shells with two columns
+// (cc-pVXZ-type 1s/2s); other generally contracted shells use segments.
+static bool contracted_eligible(const PlanData& P) {
+  bool any = false;
+  for (int sh = 0; sh < P.nbas; ++sh) {
+    const int l = P.bas[sh * BAS_SLOTS + ANG_OF], nc = P.bas[sh * BAS_SLOTS + NCTR_OF];
+    if (nc > 1) {
+      if (nc > 2 || l != 0) return false;
+      any = true;
+    }
+  }
+  return any;
+}
+
+// Contracted J/K pair set over parent shells.  Returns false (and leaves the
+// set empty) when a general-pattern class cannot hold one bra's exchange rows
+// for one density in LDS; the caller then uses segmented pairs.
+static bool build_contracted_set(PlanData& P, hipStream_t s) {
+  PairSet& S = P.jk;
+  S.contracted = true;
+  const double pi = 3.14159265358979323846;
+  const double s2p = std::sqrt(2.0 * std::pow(pi, 2.5));
+  std::vector<ShellXYZ> sh(P.nbas);
+  std::vector<int> blk;
+  std::vector<MolAO> mols = P.mols;
+  int64_t db = 0;
+  for (int m = 0; m < P.nmol; ++m) {
+    mols[m].seg0 = P.mol_bas[m];
+    mols[m].nseg = P.mol_bas[m + 1] - P.mol_bas[m];
+    mols[m].dbase = db;
+    db += (int64_t)mols[m].nseg * mols[m].nseg;
+    for (int b = P.mol_bas[m]; b < P.mol_bas[m + 1]; ++b) {
+      const int* row = P.bas.data() + b * BAS_SLOTS;
+      const double* r = P.env.data() + P.atm[row[ATOM_OF] * ATM_SLOTS + PTR_COORD];
+      ShellXYZ& x = sh[b];
+      x.x = r[0]; x.y = r[1]; x.z = r[2];
+      x.ao = P.ao_cart[b] - P.mols[m].aoc0;
+      x.l = row[ANG_OF];
+      x.mol = m;
+      x.lseg = b - P.mol_bas[m];
+      blk.push_back(x.ao);
+      blk.push_back(row[NCTR_OF] * ncart_h(x.l));
+    }
+  }
+  mols[P.nmol].seg0 = P.nbas;
+  mols[P.nmol].dbase = db;
+  // parent pairs with their primitive pairs and column coefficients
+  std::vector<PrimPair> prims;
+  std::vector<PrimCoef> coef;
+  std::vector<PairCand> cand;
+  auto fsp = [&](int l) { return l == 0 ? 0.5 / std::sqrt(pi) : (l == 1 ? std::sqrt(3.0 / (4.0 * pi)) : 1.0); };
+  for (int m = 0; m < P.nmol; ++m) {
+    const int nb = P.mol_bas[m + 1] - P.mol_bas[m];
+    for (int Sb = P.mol_bas[m]; Sb < P.mol_bas[m + 1]; ++Sb)
+      for (int Tb = P.mol_bas[m]; Tb <= Sb; ++Tb) {
+        int a = Sb, b = Tb;
+        const int* ra = P.bas.data() + a * BAS_SLOTS;
+        const int* rb = P.bas.data() + b * BAS_SLOTS;
+        // l(A) >= l(B); for equal l the generally contracted shell first
+        if (rb[ANG_OF] > ra[ANG_OF] || (rb[ANG_OF] == ra[ANG_OF] && rb[NCTR_OF] > ra[NCTR_OF])) {
+          std::swap(a, b);
+          std::swap(ra, rb);
+        }
+        const int la = ra[ANG_OF], lb = rb[ANG_OF], nca = ra[NCTR_OF], ncb = rb[NCTR_OF];
+        const int pat = (nca - 1) + 2 * (ncb - 1);
+        if (!pattern_valid(la, lb, pat)) throw Internal("unsupported general-contraction pattern");
+        const double* A = P.env.data() + P.atm[ra[ATOM_OF] * ATM_SLOTS + PTR_COORD];
+        const double* B = P.env.data() + P.atm[rb[ATOM_OF] * ATM_SLOTS + PTR_COORD];
+        double ab2 = 0;
+        for (int x = 0; x < 3; ++x) ab2 += (A[x] - B[x]) * (A[x] - B[x]);
+        Pair pr;
+        pr.sa = a;
+        pr.sb = b;
+        pr.poff = (int)prims.size();
+        pr.np = 0;
+        const int npa = ra[NPRIM_OF], npb = rb[NPRIM_OF];
+        const double fa = fsp(la), fb = fsp(lb);
+        for (int i = 0; i < npa; ++i)
+          for (int j = 0; j < npb; ++j) {
+            const double ai = P.env[ra[PTR_EXP] + i], aj = P.env[rb[PTR_EXP] + j], p = ai + aj;
+            const double mu = ai * aj / p * ab2;
+            if (mu > 60.0) continue;
+            PrimCoef c;
+            for (int k = 0; k < 2; ++k) {
+              c.ca[k] = k < nca ? P.env[ra[PTR_COEFF] + k * npa + i] * fa : 0.0;
+              c.cb[k] = k < ncb ? P.env[rb[PTR_COEFF] + k * npb + j] * fb : 0.0;
+            }
+            PrimPair pp;
+            pp.p = p;
+            pp.px = (ai * A[0] + aj * B[0]) / p;
+            pp.py = (ai * A[1] + aj * B[1]) / p;
+            pp.pz = (ai * A[2] + aj * B[2]) / p;
+            const double ek = std::exp(-mu) * s2p / p;
+            pp.e = pat == 0 ? c.ca[0] * c.cb[0] * ek : ek;
+            pp.aa = ai;
+            pp.ab = aj;
+            pp.ip = 1.0 / p;
+            prims.push_back(pp);
+            coef.push_back(c);
+            ++pr.np;
+          }
+        const int ps = Sb - P.mol_bas[m], pt = Tb - P.mol_bas[m];
+        const double qv = P.parent_q.empty() ? 1.0
+                          : P.parent_q[(size_t)P.qbase[m] + (size_t)std::max(ps, pt) * nb + std::min(ps, pt)];
+        cand.push_back({pr, pair_class(la, lb), pat, m, qv});
+      }
+  }
+  if (prims.empty()) {
+    prims.push_back(PrimPair{});
+    coef.push_back(PrimCoef{});
+  }
+  S.d_shells = dev_upload(sh, s);
+  S.d_prims = dev_upload(prims, s);
+  S.d_coef = dev_upload(coef, s);
+  S.nprims = (int64_t)prims.size();
+  S.d_blk = dev_upload(blk, s);
+  build_pair_set(P, S, cand, true, s);
+  for (const QuartetClass& qc : S.classes)
+    if ((qc.pb || qc.pk) && qc.nbra_max < 1) {
+      free_pair_set(S);
+      return false;
+    }
+  P.mols = mols;
+  P.nd_tot = db;
+  return true;
+}
+
+// Spread the class launches of a build over the plan's streams: longest
+// estimated class first onto the least-loaded stream (stream 0 is the
+// caller's).  Side streams and the fork/join events are created here.
+static void assign_streams(PlanData& P) {
+  const int ns = std::max(1, std::min<int>(kMaxStreams, P.options.streams));
+  P.lane_classes.assign(ns, {});
+  std::vector<int> order(P.jk.classes.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int x, int y) { return P.jk.classes[x].cost > P.jk.classes[y].cost; });
+  std::vector<double> load(ns, 0.0);
+  for (int ci : order) {
+    const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[l] += P.jk.classes[ci].cost;
+    P.lane_classes[l].push_back(ci);
+  }
+  for (int l = 1; l < ns; ++l) {
+    hipStream_t st;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    P.side.push_back(st);
+    hipEvent_t ev;
+    HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    P.join_ev.push_back(ev);
+  }
+  if (ns > 1) HIP_OK(hipEventCreateWithFlags(&P.fork_ev, hipEventDisableTiming));
+}
+
 static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
   if (!std::isfinite(cutoff) || cutoff < 0) throw InvalidArgument("Invalid Schwarz cutoff");
   P.cutoff = cutoff;
@@ -843,18 +1315,6 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
     sh[i].lseg = P.seg[i].lseg;
   }
   P.d_shells = dev_upload(sh, s);
-  // molecule table for the kernels
-  {
-    std::vector<MolInfo> mi(P.nmol);
-    for (int m = 0; m < P.nmol; ++m) {
-      mi[m].base = P.mols[m].cbase;
-      mi[m].dbase = P.mols[m].dbase;
-      mi[m].n = P.mols[m].nc;
-      mi[m].nseg = P.mols[m].nseg;
-    }
-    P.d_molinfo = dev_upload(mi, s);
-    P.d_molao = dev_upload(P.mols, s);
-  }
   // all intra-molecule segmented pairs S >= T, molecule by molecule
   P.mol_pair0.assign(P.nmol + 1, 0);
   for (int m = 0; m < P.nmol; ++m) {
@@ -899,23 +1359,24 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
   }
   // Schwarz bounds (parent-shell maxima, as the reference defines them per shell pair)
   const int npair = (int)P.all_pairs.size();
-  std::vector<double> q(npair, 1.0);
-  std::vector<double> qmax(P.nmol, 0.0);
+  std::vector<double>& q = P.seg_pair_q;
+  q.assign(npair, 1.0);
+  P.qmax_mol.assign(P.nmol, 0.0);
+  P.qbase.assign(P.nmol + 1, 0);
+  for (int m = 0; m < P.nmol; ++m) {
+    const int64_t nb = P.mol_bas[m + 1] - P.mol_bas[m];
+    P.qbase[m + 1] = P.qbase[m] + nb * nb;
+  }
   if (cutoff > 0) {
     std::vector<double> seg_q;
     compute_schwarz(P, seg_q, s);
-    std::vector<int64_t> qbase(P.nmol + 1, 0);
-    for (int m = 0; m < P.nmol; ++m) {
-      const int64_t nb = P.mol_bas[m + 1] - P.mol_bas[m];
-      qbase[m + 1] = qbase[m] + nb * nb;
-    }
     // parent_q[qbase[m] + i*nb + j], i >= j local parent shells: max over segment pairs
-    P.parent_q.assign((size_t)qbase[P.nmol], 0.0);
+    P.parent_q.assign((size_t)P.qbase[P.nmol], 0.0);
     auto pslot = [&](int S, int T) -> double& {
       const int m = P.seg[S].mol;
       const int nb = P.mol_bas[m + 1] - P.mol_bas[m];
       const int ps = P.seg[S].parent - P.mol_bas[m], pt = P.seg[T].parent - P.mol_bas[m];
-      return P.parent_q[(size_t)qbase[m] + (size_t)std::max(ps, pt) * nb + std::min(ps, pt)];
+      return P.parent_q[(size_t)P.qbase[m] + (size_t)std::max(ps, pt) * nb + std::min(ps, pt)];
     };
     for (int m = 0; m < P.nmol; ++m)
       for (int S = P.mol_seg[m]; S < P.mol_seg[m + 1]; ++S)
@@ -930,213 +1391,32 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
   }
   for (int i = 0; i < npair; ++i) {
     const int m = P.seg[P.all_pairs[i].sa].mol;
-    qmax[m] = std::max(qmax[m], q[i]);
+    P.qmax_mol[m] = std::max(P.qmax_mol[m], q[i]);
   }
-  // retained pairs, grouped by class, molecule and primitive-pair count
-  // (power-of-two buckets, so lanes of one wavefront run similar primitive
-  // loops), Schwarz-descending inside a (class, molecule, bucket) list
-  auto bucket = [](int np) {
-    int b = 0;
-    while ((2 << b) <= np) ++b;
-    return b;
-  };
-  auto mol_of = [&](int i) { return P.seg[P.all_pairs[i].sa].mol; };
-  std::vector<int> keep;
-  for (int i = 0; i < npair; ++i) {
-    if (P.all_pairs[i].np == 0) continue;
-    if (cutoff > 0 && q[i] * qmax[mol_of(i)] < cutoff) continue;
-    keep.push_back(i);
-  }
-  std::stable_sort(keep.begin(), keep.end(), [&](int x, int y) {
-    if (P.all_class[x] != P.all_class[y]) return P.all_class[x] < P.all_class[y];
-    if (mol_of(x) != mol_of(y)) return mol_of(x) < mol_of(y);
-    const int bx = bucket(P.all_pairs[x].np), by = bucket(P.all_pairs[y].np);
-    if (bx != by) return bx > by;
-    return q[x] > q[y];
-  });
-  for (int i : keep) {
-    P.pairs.push_back(P.all_pairs[i]);
-    P.pair_q.push_back(q[i]);
-  }
-  struct Run { int cls, mol, bkt, beg, end, npmin, npmax; };
-  std::vector<Run> runs;
-  for (int k = 0; k < (int)keep.size();) {
-    Run r{P.all_class[keep[k]], mol_of(keep[k]), bucket(P.all_pairs[keep[k]].np), k, k, 1 << 30, 0};
-    while (k < (int)keep.size() && P.all_class[keep[k]] == r.cls && mol_of(keep[k]) == r.mol &&
-           bucket(P.all_pairs[keep[k]].np) == r.bkt) {
-      r.npmin = std::min(r.npmin, P.all_pairs[keep[k]].np);
-      r.npmax = std::max(r.npmax, P.all_pairs[keep[k]].np);
-      ++k;
-    }
-    r.end = k;
-    runs.push_back(r);
-  }
-  // runs of each (class, molecule), for the intra-molecule quartet pairing below
-  std::map<std::pair<int, int>, std::vector<int>> runs_of;
-  for (int i = 0; i < (int)runs.size(); ++i) runs_of[{runs[i].cls, runs[i].mol}].push_back(i);
-  for (int c = 0; c <= kNumPairClasses; ++c) P.class_begin[c] = (int)keep.size();
-  for (int i = (int)runs.size() - 1; i >= 0; --i) P.class_begin[runs[i].cls] = runs[i].beg;
-  for (int c = kNumPairClasses - 1; c >= 0; --c) P.class_begin[c] = std::min(P.class_begin[c], P.class_begin[c + 1]);
-  P.d_pairs = dev_upload(P.pairs, s);
   {
-    std::vector<double> pq = P.pair_q;
-    if (pq.empty()) pq.push_back(0.0);
-    P.d_pair_q = dev_upload(pq, s);
-    std::vector<int> sa;
-    for (const Seg& g : P.seg) {
-      sa.push_back(g.ao_cart);
-      sa.push_back(ncart_h(g.l));
-    }
-    P.d_seg_ao = dev_upload(sa, s);
     std::vector<int> at;
     for (const Seg& g : P.seg) at.push_back(P.bas[g.parent * BAS_SLOTS + ATOM_OF]);
     P.d_seg_atom = dev_upload(at, s);
+  }
+  // J/K pair set: parent shells with their general-contraction columns where
+  // the basis has them (and the exchange slabs fit), segmented shells otherwise
+  bool done = false;
+  if (P.options.general_contraction && contracted_eligible(P)) done = build_contracted_set(P, s);
+  if (!done) build_segmented_set(P, P.jk, true, s);
+  assign_streams(P);
+  // molecule tables (the density-maximum blocks follow the J/K shell set)
+  {
+    std::vector<MolInfo> mi(P.nmol);
+    for (int m = 0; m < P.nmol; ++m) {
+      mi[m].base = P.mols[m].cbase;
+      mi[m].dbase = P.mols[m].dbase;
+      mi[m].n = P.mols[m].nc;
+      mi[m].nseg = P.mols[m].nseg;
+    }
+    P.d_molinfo = dev_upload(mi, s);
+    P.d_molao = dev_upload(P.mols, s);
     HIP_OK(hipMalloc(&P.d_dmax, sizeof(double) * std::max<int64_t>(P.nd_tot, 1)));
   }
-  // quartet classes: canonical (bra run, ket run) sub-tasks with materialised quartet lists
-  static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
-  P.nquartets = 0;
-  for (int cb = 0; cb < kNumPairClasses; ++cb)
-    for (int ck = 0; ck <= cb; ++ck) {
-      QuartetClass qc;
-      qc.la = pl[cb][0]; qc.lb = pl[cb][1]; qc.lc = pl[ck][0]; qc.ld = pl[ck][1];
-      qc.tuning_index = cb * (cb + 1) / 2 + ck;
-      const gscf_class_tuning& tu = P.tuning[qc.tuning_index];
-      const int64_t kTargetPrim = tu.target_prim, kChunkItems = tu.chunk_items;
-      const int kChunkBrasMax = tu.bras_max;
-      qc.nq = 0;
-      qc.nitems = 0;
-      struct Pending { std::vector<int64_t> qoff; int bra0, ket0, nbra; int64_t q0, nq; };
-      std::vector<Pending> pend;
-      for (const Run& rb : runs) {
-        if (rb.cls != cb) continue;
-        auto it = runs_of.find({ck, rb.mol});
-        if (it == runs_of.end()) continue;
-        for (int rki : it->second) {
-          const Run& rk = runs[rki];
-          const bool same_list = cb == ck && rb.bkt == rk.bkt;
-          if (cb == ck && rb.bkt < rk.bkt) continue;  // canonical: each unordered pair-pair once
-          const int nbra = rb.end - rb.beg, nket = rk.end - rk.beg;
-          std::vector<int64_t> qoff(nbra + 1, 0);
-          for (int b = 0; b < nbra; ++b) {
-            int cnt = nket;
-            if (cutoff > 0) {
-              const double qb = P.pair_q[rb.beg + b];
-              int lo = 0, hi = nket;
-              while (lo < hi) {
-                const int mid = (lo + hi) / 2;
-                if (qb * P.pair_q[rk.beg + mid] < cutoff) hi = mid; else lo = mid + 1;
-              }
-              cnt = lo;
-            }
-            if (same_list) cnt = std::min(cnt, b + 1);
-            qoff[b + 1] = qoff[b] + cnt;
-          }
-          const int64_t nq = qoff[nbra];
-          if (nq == 0) continue;
-          SubTask t;
-          t.q0 = qc.nq;
-          t.nq = nq;
-          const int64_t work = (int64_t)rb.npmax * rk.npmax;
-          t.S = (int)std::max<int64_t>(1, std::min<int64_t>(rb.npmin, (work + kTargetPrim - 1) / kTargetPrim));
-          t.pad = 0;
-          t.item0 = qc.nitems;
-          qc.nitems += (int64_t)t.S * nq;
-          qc.nq += nq;
-          qc.tasks.push_back(t);
-          pend.push_back({std::move(qoff), rb.beg, rk.beg, nbra, t.q0, nq});
-        }
-      }
-      if (qc.nq == 0) continue;
-      {
-        // bra-major chunks: consecutive bras of one sub-task, bounded LDS slots and lane-items
-        const int ncomp_a = (qc.la + 1) * (qc.la + 2) / 2, ncomp_b = (qc.lb + 1) * (qc.lb + 2) / 2;
-        const int ncomp_c = (qc.lc + 1) * (qc.lc + 2) / 2, ncomp_d = (qc.ld + 1) * (qc.ld + 2) / 2;
-        const int rows = ncomp_a + ncomp_b;
-        const int split = (ncomp_a * ncomp_b * ncomp_c * ncomp_d > 36) ? ncomp_c * ncomp_d : 1;
-        const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)ncomp_a * ncomp_b);
-        const int nr = (qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1;
-        size_t pack = (size_t)tu.lds_pack;
-        if (tu.rys_lds && per_slot + rys_slice_bytes(nr) <= kLdsBudget) {
-          qc.tab_lds = 1;
-          qc.tab_bytes = rys_slice_bytes(nr);
-          pack = pack > qc.tab_bytes + per_slot ? pack - qc.tab_bytes : per_slot;
-        }
-        qc.nbra_max = per_slot > kLdsBudget ? 0
-                      : (int)std::max<size_t>(1, std::min<size_t>((size_t)kChunkBrasMax, pack / per_slot));
-        for (size_t ti = 0; qc.nbra_max >= 1 && ti < pend.size(); ++ti) {
-          const Pending& pd = pend[ti];
-          const SubTask& t = qc.tasks[ti];
-          const int64_t per_q = (int64_t)t.S * split;
-          int b = 0;
-          while (b < pd.nbra) {
-            const int64_t cnt = pd.qoff[b + 1] - pd.qoff[b];
-            if (cnt == 0) { ++b; continue; }
-            if (cnt * per_q > kChunkItems) {
-              // one long bra: several single-bra chunks
-              const int64_t step = std::max<int64_t>(1, kChunkItems / per_q);
-              for (int64_t k0 = 0; k0 < cnt; k0 += step) {
-                BraChunk c;
-                c.q0 = t.q0 + pd.qoff[b] + k0;
-                c.nq = (int32_t)std::min<int64_t>(step, cnt - k0);
-                c.S = t.S;
-                c.bra0 = pd.bra0 + b;
-                c.nbra = 1;
-                c.ket0 = pd.ket0 + (int32_t)k0;
-                c.nket = c.nq;
-                qc.chunks.push_back(c);
-              }
-              ++b;
-              continue;
-            }
-            BraChunk c;
-            c.q0 = t.q0 + pd.qoff[b];
-            c.S = t.S;
-            c.bra0 = pd.bra0 + b;
-            int64_t items = 0;
-            int e = b;
-            while (e < pd.nbra && e - b < qc.nbra_max) {
-              const int64_t ce = pd.qoff[e + 1] - pd.qoff[e];
-              if (items > 0 && (items + ce * per_q > kChunkItems || ce * per_q > kChunkItems)) break;
-              items += ce * per_q;
-              ++e;
-            }
-            c.nbra = e - b;
-            c.nq = (int32_t)(pd.qoff[e] - pd.qoff[b]);
-            c.ket0 = pd.ket0;
-            c.nket = 0;
-            for (int bb = b; bb < e; ++bb) c.nket = std::max<int32_t>(c.nket, (int32_t)(pd.qoff[bb + 1] - pd.qoff[bb]));
-            qc.chunks.push_back(c);
-            b = e;
-          }
-        }
-        // ket-side J in LDS for chunks whose ket window fits kKetLdsBytes per density
-        qc.threads = tu.threads;
-        if (tu.ket_lds) {
-          const int nket_comp = ncomp_c * ncomp_d;
-          const int cap = (int)(kKetLdsBytes / (sizeof(double) * nket_comp));
-          for (BraChunk& c : qc.chunks) {
-            if (c.nket > cap) c.nket = 0;
-            qc.nket_max = std::max(qc.nket_max, c.nket);
-          }
-        } else {
-          for (BraChunk& c : qc.chunks) c.nket = 0;
-        }
-        if (qc.nbra_max >= 1) qc.d_chunks = dev_upload(qc.chunks, s);
-      }
-      HIP_OK(hipMalloc(&qc.d_quartets, sizeof(int2) * qc.nq));
-      for (const Pending& pd : pend) {
-        int64_t* d_qoff = dev_upload(pd.qoff, s);
-        hipLaunchKernelGGL(expand_kernel, dim3(nblocks((size_t)pd.nq)), dim3(256), 0, s, d_qoff, pd.nbra, pd.bra0,
-                           pd.ket0, pd.nq, qc.d_quartets + pd.q0);
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipStreamSynchronize(s));
-        HIP_OK(hipFree(d_qoff));
-      }
-      qc.d_tasks = dev_upload(qc.tasks, s);
-      P.nquartets += qc.nq;
-      P.classes.push_back(std::move(qc));
-    }
   build_c2s_maps(P, s);
   {
     std::vector<int> order(npair);
@@ -1169,14 +1449,16 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
 
 // Device bytes a plan holds, with the J/K workspace for nbd densities and
 // quartet-list / chunk sizes as given (exact after build_plan).
-static int64_t plan_bytes(const PlanData& P, int64_t nq, int64_t ntasks, int64_t nchunks, int64_t nbd) {
+static int64_t plan_bytes(const PlanData& P, int64_t npairs, int64_t nq, int64_t ntasks, int64_t nchunks,
+                          int64_t nbd, int64_t extra) {
   int64_t b = 0;
   b += (int64_t)P.seg.size() * (sizeof(ShellXYZ) + 3 * sizeof(int));         // shells, seg_ao, seg_atom
   b += (int64_t)std::max<size_t>(P.prims.size(), 1) * (sizeof(PrimPair) + sizeof(PrimAux));
   b += (int64_t)(P.seg.size() + 1) * sizeof(int);
   for (const Seg& g : P.seg) b += (int64_t)g.np * (sizeof(int2) + 3 * sizeof(double));  // slots, fsp, sums
   b += (int64_t)P.all_pairs.size() * 2 * sizeof(Pair);                        // all pairs, 1e pairs
-  b += (int64_t)P.pairs.size() * (sizeof(Pair) + sizeof(double));             // screened pairs, pair_q
+  b += npairs * (sizeof(Pair) + sizeof(double));                              // screened pairs, pair_q
+  b += extra;                                                                 // contracted J/K shells and primitives
   b += (int64_t)P.n_cart * 2 * (sizeof(int) + sizeof(double)) + (int64_t)P.n * 3 * (sizeof(int) + sizeof(double));
   b += (int64_t)P.nmol * sizeof(MolInfo) + (int64_t)(P.nmol + 1) * sizeof(MolAO);
   b += std::max<int64_t>(P.nd_tot, 1) * sizeof(double);                       // shell-block density maxima
@@ -1187,13 +1469,19 @@ static int64_t plan_bytes(const PlanData& P, int64_t nq, int64_t ntasks, int64_t
 }
 
 static int64_t plan_bytes(const PlanData& P, int64_t nbd) {
-  int64_t nq = 0, nt = 0, nc = 0;
-  for (const QuartetClass& c : P.classes) {
-    nq += c.nq;
-    nt += (int64_t)c.tasks.size();
-    nc += (int64_t)c.chunks.size();
+  int64_t np = 0, nq = 0, nt = 0, nc = 0, extra = 0;
+  for (const PairSet* S : {&P.jk, &P.seg_set}) {
+    np += (int64_t)S->pairs.size();
+    for (const QuartetClass& c : S->classes) {
+      nq += c.nq;
+      nt += (int64_t)c.tasks.size();
+      nc += (int64_t)c.chunks.size();
+    }
+    if (S->contracted)
+      extra += (int64_t)P.nbas * (sizeof(ShellXYZ) + 2 * sizeof(int)) +
+               S->nprims * (int64_t)(sizeof(PrimPair) + sizeof(PrimCoef));
   }
-  return plan_bytes(P, nq, nt, nc, nbd);
+  return plan_bytes(P, np, nq, nt, nc, nbd, extra);
 }
 
 // Host-only upper bound of the device bytes of a plan plus one build of nb
@@ -1205,7 +1493,7 @@ static int64_t plan_bytes(const PlanData& P, int64_t nbd) {
 static int64_t forecast_bytes(PlanData& P, int nb) {
   build_segments(P);
   P.mol_pair0.assign(P.nmol + 1, 0);
-  int64_t nq = 0;
+  int64_t nq = 0, npairs = 0;
   for (int m = 0; m < P.nmol; ++m) {
     int64_t live = 0;
     for (int S = P.mol_seg[m]; S < P.mol_seg[m + 1]; ++S)
@@ -1215,37 +1503,39 @@ static int64_t forecast_bytes(PlanData& P, int nb) {
         Pair pr;
         make_pair(P, a, c, pr);
         P.all_pairs.push_back(pr);
-        if (pr.np > 0) {
-          P.pairs.push_back(pr);
-          ++live;
-        }
+        if (pr.np > 0) ++live;
       }
     nq += live * (live + 1) / 2;
+    npairs += live;
   }
   // sub-tasks pair a bra run with a ket run of one molecule: at most
-  // (pair classes x primitive-count buckets)^2 per molecule
-  const int64_t runs = (int64_t)kNumPairClasses * 8;
-  return plan_bytes(P, nq, std::min<int64_t>(nq, runs * runs * P.nmol), nq, 2 * (int64_t)nb);
+  // (pair classes x column patterns x primitive-count buckets)^2 per molecule;
+  // a contracted J/K set adds parent shells and at most as many primitive
+  // pairs (with column coefficients) as the segmented set holds
+  const int64_t runs = (int64_t)kNumPairClasses * 2 * 8;
+  const int64_t extra = (int64_t)P.nbas * (sizeof(ShellXYZ) + 2 * sizeof(int)) +
+                        (int64_t)P.prims.size() * (sizeof(PrimPair) + sizeof(PrimCoef));
+  return plan_bytes(P, npairs, nq, std::min<int64_t>(nq, runs * runs * P.nmol), nq, 2 * (int64_t)nb, extra);
 }
 
 static void free_plan(PlanData& P) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  f(P.d_shells); f(P.d_prims); f(P.d_pairs); f(P.d_all_pairs);
+  free_pair_set(P.jk);
+  free_pair_set(P.seg_set);
+  f(P.d_shells); f(P.d_prims); f(P.d_all_pairs);
   f(P.d_c2s_idx); f(P.d_c2s_coef); f(P.d_s2c_idx); f(P.d_s2c_coef);
   f(P.d_dc); f(P.d_jp); f(P.d_kp); f(P.d_flag);
   f(P.d_pairs1e); f(P.d_nuclei);
-  f(P.d_pair_q); f(P.d_dmax); f(P.d_seg_ao); f(P.d_seg_atom);
+  f(P.d_dmax); f(P.d_seg_atom);
   f(P.d_molinfo); f(P.d_molao);
   f(P.d_aux); f(P.d_seg_prim0); f(P.d_prim_slots); f(P.d_prim_fsp); f(P.d_gparam);
   for (double* p : P.retired) f(p);
-  for (auto& c : P.classes) {
-    f(c.d_quartets);
-    f(c.d_tasks);
-    f(c.d_chunks);
-  }
   for (hipEvent_t e : P.ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : P.join_ev) (void)hipEventDestroy(e);
+  if (P.fork_ev) (void)hipEventDestroy(P.fork_ev);
+  for (hipStream_t st : P.side) (void)hipStreamDestroy(st);
   if (P.h_flag) (void)hipHostFree(P.h_flag);
 }
 
@@ -1289,11 +1579,29 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
                      P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, nbd);
   if (mixed)
     hipLaunchKernelGGL(dmax_kernel, dim3(nblocks((size_t)P.nd_tot)), dim3(256), 0, s, P.d_dc, P.ncc_tot, nbd,
-                       P.d_molao, P.nmol, P.nd_tot, P.d_seg_ao, P.d_dmax);
+                       P.d_molao, P.nmol, P.nd_tot, P.jk.d_blk, P.d_dmax);
   hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_jp, ncc * nbd, 0.0);
   hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_kp, ncc * nbd, 0.0);
-  for (size_t ci = 0; ci < P.classes.size(); ++ci) {
-    const QuartetClass& qc = P.classes[ci];
+  const PairSet& S = P.jk;
+  if (P.prof) ++P.prof_builds;
+  // Class launches: in list order on the caller's stream while profiling (so
+  // that the per-class events time one kernel each), otherwise spread over
+  // the plan's streams, forked from and joined back into the caller's stream.
+  const bool fork = !P.prof && !P.side.empty();
+  if (fork) {
+    HIP_OK(hipEventRecord(P.fork_ev, s));
+    for (hipStream_t st : P.side) HIP_OK(hipStreamWaitEvent(st, P.fork_ev, 0));
+  }
+  std::vector<std::pair<int, hipStream_t>> launches;
+  if (fork) {
+    for (size_t l = 0; l < P.lane_classes.size(); ++l)
+      for (int ci : P.lane_classes[l]) launches.push_back({ci, l == 0 ? s : P.side[l - 1]});
+  } else {
+    for (size_t ci = 0; ci < S.classes.size(); ++ci) launches.push_back({(int)ci, s});
+  }
+  for (const auto& [ci, ls] : launches) {
+    const QuartetClass& qc = S.classes[ci];
+    hipStream_t s = ls;  // this class's launch stream
     if (P.prof) {
       if (P.ev_used + 2 > P.ev_pool.size()) {
         for (int e = 0; e < 64; ++e) {
@@ -1302,51 +1610,58 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
           P.ev_pool.push_back(ev);
         }
       }
-      P.prof_recs.push_back({(int)ci, P.ev_used});
+      P.prof_recs.push_back({qc.tuning_index, P.ev_used, P.prof_builds});
       HIP_OK(hipEventRecord(P.ev_pool[P.ev_used], s));
     }
+    // widths count every contraction column of a shell
+    const int wa = ncart_h(qc.la) * (1 + (qc.pb & 1)), wb = ncart_h(qc.lb) * (1 + (qc.pb >> 1));
+    const int wc = ncart_h(qc.lc) * (1 + (qc.pk & 1)), wd = ncart_h(qc.ld) * (1 + (qc.pk >> 1));
+    BraArgs b;
+    b.pairs = S.d_pairs;
+    b.prims = S.d_prims;
+    b.shells = S.d_shells;
+    b.coef = S.d_coef;
+    b.quartets = qc.d_quartets;
+    b.chunks = qc.d_chunks;
+    b.nchunks = (int)qc.chunks.size();
+    b.nbra_max = qc.nbra_max;
+    b.mols = P.d_molinfo;
+    b.n = P.nc_max;
+    b.with_j = with_j;
+    b.with_k = with_k;
+    b.mixed = mixed ? 1 : 0;
+    b.thr = mixed_threshold;
+    b.pair_q = S.d_pair_q;
+    b.dmax = P.d_dmax;
+    b.threads = qc.threads;
+    // LDS of one launch over nbd_l exchange densities and nbj_l Coulomb densities
+    auto layout = [&](int nbd_l, int nbj_l, size_t& lds, int& nket_max, int& tab_lds) {
+      lds = sizeof(double) * ((size_t)nbd_l * qc.nbra_max * (wa + wb) * P.nc_max + (size_t)nbj_l * qc.nbra_max * wa * wb);
+      nket_max = with_j ? qc.nket_max : 0;
+      const size_t jket_bytes = sizeof(double) * (size_t)nbj_l * nket_max * wc * wd;
+      if (nket_max && lds + jket_bytes <= kLdsBudget) lds += jket_bytes; else nket_max = 0;
+      tab_lds = qc.tab_lds;
+      if (tab_lds && lds + qc.tab_bytes <= kLdsBudget) lds += qc.tab_bytes; else tab_lds = 0;
+    };
     const int nbd_eff = with_k ? nbd : nb;
-    const int ncomp_a = (qc.la + 1) * (qc.la + 2) / 2, ncomp_b = (qc.lb + 1) * (qc.lb + 2) / 2;
-    size_t lds = sizeof(double) * ((size_t)nbd_eff * qc.nbra_max * (ncomp_a + ncomp_b) * P.nc_max +
-                                   (size_t)nb * qc.nbra_max * ncomp_a * ncomp_b);
-    const int ncomp_c = (qc.lc + 1) * (qc.lc + 2) / 2, ncomp_d = (qc.ld + 1) * (qc.ld + 2) / 2;
-    int nket_max = with_j ? qc.nket_max : 0;
-    const size_t jket_bytes = sizeof(double) * (size_t)nb * nket_max * ncomp_c * ncomp_d;
-    if (nket_max && lds + jket_bytes <= kLdsBudget) lds += jket_bytes; else nket_max = 0;
-    int tab_lds = qc.tab_lds;
-    if (tab_lds && lds + qc.tab_bytes <= kLdsBudget) lds += qc.tab_bytes; else tab_lds = 0;
+    size_t lds;
+    int nket_max, tab_lds;
+    layout(nbd_eff, nb, lds, nket_max, tab_lds);
     if (qc.d_chunks && lds <= kLdsBudget) {
-      BraArgs b;
-      b.pairs = P.d_pairs;
-      b.prims = P.d_prims;
-      b.shells = P.d_shells;
-      b.quartets = qc.d_quartets;
-      b.chunks = qc.d_chunks;
-      b.nchunks = (int)qc.chunks.size();
-      b.nbra_max = qc.nbra_max;
       b.dens = P.d_dc;
       b.jp = P.d_jp;
       b.kp = P.d_kp;
-      b.mols = P.d_molinfo;
       b.bstride = P.ncc_tot;
-      b.n = P.nc_max;
       b.nbd = nbd_eff;
       b.nbj = nb;
-      b.with_j = with_j;
-      b.with_k = with_k;
-      b.mixed = mixed ? 1 : 0;
-      b.thr = mixed_threshold;
-      b.pair_q = P.d_pair_q;
-      b.dmax = P.d_dmax;
       b.tab_lds = tab_lds;
       b.nket_max = nket_max;
-      b.threads = qc.threads;
-      launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, b, lds, s);
-    } else {
+      launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, lds, s);
+    } else if (!qc.pb && !qc.pk) {
       JKArgs a;
-      a.pairs = P.d_pairs;
-      a.prims = P.d_prims;
-      a.shells = P.d_shells;
+      a.pairs = S.d_pairs;
+      a.prims = S.d_prims;
+      a.shells = S.d_shells;
       a.quartets = qc.d_quartets;
       a.tasks = qc.d_tasks;
       a.ntasks = (int)qc.tasks.size();
@@ -1362,10 +1677,35 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       a.with_j = with_j;
       a.with_k = with_k;
       launch_jk_class(qc.la, qc.lb, qc.lc, qc.ld, a, s);
+    } else {
+      // general-contraction class whose slabs do not fit for all densities at
+      // once: one launch per density (J and K for D_b, K only for D_b^T of a
+      // nonsymmetric K), which the plan guarantees fits in LDS
+      layout(1, 1, lds, nket_max, tab_lds);
+      if (!qc.d_chunks || lds > kLdsBudget) throw Internal("general-contraction class does not fit in LDS");
+      for (int bt = 0; bt < nbd_eff; ++bt) {
+        b.dens = P.d_dc + (size_t)bt * ncc;
+        b.jp = P.d_jp + (size_t)bt * ncc;
+        b.kp = P.d_kp + (size_t)bt * ncc;
+        b.bstride = P.ncc_tot;
+        b.nbd = 1;
+        b.nbj = 1;
+        b.with_j = with_j && bt < nb;
+        b.tab_lds = tab_lds;
+        b.nket_max = b.with_j ? nket_max : 0;
+        if (!b.with_j && !b.with_k) continue;
+        launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, lds, s);
+      }
     }
     if (P.prof) {
       HIP_OK(hipEventRecord(P.ev_pool[P.ev_used + 1], s));
       P.ev_used += 2;
+    }
+  }
+  if (fork) {
+    for (size_t l = 0; l < P.side.size(); ++l) {
+      HIP_OK(hipEventRecord(P.join_ev[l], P.side[l]));
+      HIP_OK(hipStreamWaitEvent(s, P.join_ev[l], 0));
     }
   }
   hipLaunchKernelGGL(finalize_kernel, dim3(nblocks((size_t)(with_j + with_k) * nb * nn)), dim3(256), 0, s,
@@ -1425,9 +1765,10 @@ static void geometry_vjp(PlanData& P, const double* dens, const double* cj, cons
     if (src[k])
       hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc)), dim3(256), 0, s, src[k], P.nn_tot, 1,
                          P.d_dc + k * ncc, P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, 1);
-  for (const QuartetClass& qc : P.classes) {
+  const PairSet& S = ensure_seg_set(P, s);
+  for (const QuartetClass& qc : S.classes) {
     GradArgs a;
-    a.pairs = P.d_pairs;
+    a.pairs = S.d_pairs;
     a.prims = P.d_prims;
     a.shells = P.d_shells;
     a.seg_atom = P.d_seg_atom;
@@ -1467,9 +1808,10 @@ static void jk_param_vjp(PlanData& P, const double* dens, const double* cj, cons
     if (src[k])
       hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc)), dim3(256), 0, s, src[k], P.nn_tot, 1,
                          P.d_dc + k * ncc, P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, 1);
-  for (const QuartetClass& qc : P.classes) {
+  const PairSet& S = ensure_seg_set(P, s);
+  for (const QuartetClass& qc : S.classes) {
     ParamArgs a;
-    a.pairs = P.d_pairs;
+    a.pairs = S.d_pairs;
     a.prims = P.d_prims;
     a.aux = P.d_aux;
     a.shells = P.d_shells;
@@ -1690,6 +2032,23 @@ int gscf_tuning_set(int32_t cls, const gscf_class_tuning* t) {
   });
 }
 
+int gscf_options_set(const gscf_options* o) {
+  return guarded([&] {
+    if (!o) throw InvalidArgument("NULL options");
+    if (o->general_contraction != 0 && o->general_contraction != 1)
+      throw InvalidArgument("general_contraction must be 0 or 1");
+    if (o->streams < 1 || o->streams > kMaxStreams) throw InvalidArgument("streams must be in 1..4");
+    g_options = *o;
+  });
+}
+
+int gscf_options_get(gscf_options* o) {
+  return guarded([&] {
+    if (!o) throw InvalidArgument("NULL options");
+    *o = g_options;
+  });
+}
+
 int gscf_tuning_reset(void) {
   return guarded([&] {
     g_tuning_init = false;
@@ -1721,8 +2080,8 @@ int gscf_plan_get_info(const gscf_plan* plan, gscf_plan_info* info) {
     info->n_cart = P.n_cart;
     info->nbas = P.nbas;
     info->nseg = (int32_t)P.seg.size();
-    info->npairs = (int64_t)P.pairs.size();
-    info->nquartets = P.nquartets;
+    info->npairs = (int64_t)P.jk.pairs.size();
+    info->nquartets = P.jk.nquartets;
     info->cutoff = P.cutoff;
     info->plan_ms = P.plan_ms;
     info->nmol = P.nmol;
@@ -1777,6 +2136,7 @@ int gscf_plan_profile(gscf_plan* plan, int32_t enable) {
     PlanData& P = plan->d;
     P.prof = enable != 0;
     P.prof_recs.clear();
+    P.prof_builds = 0;
     P.ev_used = 0;
   });
 }
@@ -1786,23 +2146,40 @@ int gscf_plan_class_times(gscf_plan* plan, int32_t max_classes, int32_t* classes
   return guarded([&] {
     if (!plan || !count) throw InvalidArgument("NULL plan or output");
     PlanData& P = plan->d;
-    const int nc = (int)P.classes.size();
+    // one entry per angular-momentum class (its column-pattern launches summed);
+    // launches = builds in which the class ran
+    std::vector<int> order;
+    std::map<int, int> slot;
+    for (const QuartetClass& qc : P.jk.classes)
+      if (!slot.count(qc.tuning_index)) {
+        slot[qc.tuning_index] = (int)order.size();
+        order.push_back(qc.tuning_index);
+      }
+    const int nc = (int)order.size();
     *count = nc;
     if (max_classes < nc || !classes || !ms || !launches) throw InvalidArgument("class arrays too small");
+    for (const QuartetClass& qc : P.jk.classes) {
+      const int i = slot[qc.tuning_index];
+      classes[4 * i + 0] = qc.la;
+      classes[4 * i + 1] = qc.lb;
+      classes[4 * i + 2] = qc.lc;
+      classes[4 * i + 3] = qc.ld;
+    }
+    std::vector<int64_t> last(nc, -1);
     for (int i = 0; i < nc; ++i) {
-      classes[4 * i + 0] = P.classes[i].la;
-      classes[4 * i + 1] = P.classes[i].lb;
-      classes[4 * i + 2] = P.classes[i].lc;
-      classes[4 * i + 3] = P.classes[i].ld;
       ms[i] = 0.0;
       launches[i] = 0;
     }
     for (const auto& r : P.prof_recs) {
-      HIP_OK(hipEventSynchronize(P.ev_pool[r.second + 1]));
+      HIP_OK(hipEventSynchronize(P.ev_pool[r.ev + 1]));
       float t = 0.f;
-      HIP_OK(hipEventElapsedTime(&t, P.ev_pool[r.second], P.ev_pool[r.second + 1]));
-      ms[r.first] += t;
-      launches[r.first] += 1;
+      HIP_OK(hipEventElapsedTime(&t, P.ev_pool[r.ev], P.ev_pool[r.ev + 1]));
+      const int i = slot[r.cls];
+      ms[i] += t;
+      if (last[i] != r.build) {
+        launches[i] += 1;
+        last[i] = r.build;
+      }
     }
   });
 }

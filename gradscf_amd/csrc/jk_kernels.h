@@ -465,6 +465,331 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
 // global atomics.  Global FP64 atomics are executed at the memory side on this
 // platform, so their count, not FLOPs, is what this layout minimises.
 // ----------------------------------------------------------------------------
+// Column layout of a quartet class on a contracted pair set: shell X carries
+// CX contraction columns and its "wide" block WX = CX * ncart(lX) components
+// in AO order (columns contiguous), so digestion addresses AO oX + wide index
+// and the exchange slabs, bra J blocks and ket J windows simply use the wide
+// widths.  CX = 1 everywhere is the segmented (one-column) case.
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD>
+struct Wide {
+  using K = Class<LA, LB, LC, LD>;
+  static constexpr int WA = CA * K::NA, WB = CB * K::NB, WC = CC * K::NC, WD = CD * K::ND;
+  static constexpr bool GEN = CA * CB * CC * CD > 1;
+  static constexpr int NBRA = WA * WB, NKET = WC * WD;
+  static constexpr bool kSplit = NBRA * NKET > 36;
+  static constexpr int SPLIT = kSplit ? NKET : 1;
+};
+
+// One work item of the bra-major kernel in real type R: evaluate the quartet
+// (or its wide ket-component slice) over bra primitive pairs s0, s0+S, ... and
+// digest it (exchange rows / bra J block into LDS, ket J block into the LDS
+// window or to global).  With general contraction each primitive quartet is
+// evaluated once, coefficient-free, and added to every column combination
+// with its coefficient product c_A c_B c_C c_D.
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, typename R>
+__device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const Pair& kp, const int2 qk, int slot,
+                                         int s0, int S, int kc, double* __restrict__ krow,
+                                         double* __restrict__ jrow, double* __restrict__ jket, int kwin0,
+                                         const double* __restrict__ tab) {
+  using K = Class<LA, LB, LC, LD>;
+  using W = Wide<LA, LB, LC, LD, CA, CB, CC, CD>;
+  constexpr int WA = W::WA, WB = W::WB, WC = W::WC, WD = W::WD;
+  constexpr int ROWS = WA + WB;
+  const int ns = a.n;  // LDS row length
+  const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
+  const MolInfo mi = a.mols[sA.mol];
+  const int n = mi.n;
+  QuartetGeom geo;
+  double ABd[3], CDd[3];
+  load_geom(sA, sB, sC, sD, geo, ABd, CDd);
+  const R AB[3] = {(R)ABd[0], (R)ABd[1], (R)ABd[2]};
+  const R CD_[3] = {(R)CDd[0], (R)CDd[1], (R)CDd[2]};
+  double f = 1.0;
+  if (bp.sa == bp.sb) f *= 0.5;
+  if (kp.sa == kp.sb) f *= 0.5;
+  if (qk.x == qk.y) f *= 0.5;
+  const int oa = sA.ao, ob = sB.ao;
+  if constexpr (!W::kSplit) {
+    R acc[WA][WB][WC][WD];
+#pragma unroll
+    for (int ia = 0; ia < WA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < WB; ++ib)
+#pragma unroll
+        for (int ic = 0; ic < WC; ++ic)
+#pragma unroll
+          for (int id = 0; id < WD; ++id) acc[ia][ib][ic][id] = R(0);
+    for (int ip = s0; ip < bp.np; ip += S) {
+      const PrimPair P = a.prims[bp.poff + ip];
+      // a side with one column already carries its coefficients in e
+      PrimCoef cP = {{1.0, 0.0}, {1.0, 0.0}};
+      if constexpr (CA * CB > 1) cP = a.coef[bp.poff + ip];
+      for (int kq = 0; kq < kp.np; ++kq) {
+        const PrimPair Q = a.prims[kp.poff + kq];
+        PrimQ<K::NR> q;
+        prim_setup<K::NR>(P, Q, geo, q, tab);
+        if constexpr (!W::GEN) {
+          accumulate_full<LA, LB, LC, LD, R>(q, AB, CD_, acc);
+        } else {
+          PrimCoef cQ = {{1.0, 0.0}, {1.0, 0.0}};
+          if constexpr (CC * CD > 1) cQ = a.coef[kp.poff + kq];
+          R blk[K::NA][K::NB][K::NC][K::ND];
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+              for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+                for (int id = 0; id < K::ND; ++id) blk[ia][ib][ic][id] = R(0);
+          accumulate_full<LA, LB, LC, LD, R>(q, AB, CD_, blk);
+#pragma unroll
+          for (int x = 0; x < CA; ++x)
+#pragma unroll
+            for (int y = 0; y < CB; ++y)
+#pragma unroll
+              for (int z = 0; z < CC; ++z)
+#pragma unroll
+                for (int w = 0; w < CD; ++w) {
+                  const R cf = (R)(cP.ca[x] * cP.cb[y] * cQ.ca[z] * cQ.cb[w]);
+#pragma unroll
+                  for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+                    for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+                      for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+                        for (int id = 0; id < K::ND; ++id)
+                          acc[x * K::NA + ia][y * K::NB + ib][z * K::NC + ic][w * K::ND + id] +=
+                              cf * blk[ia][ib][ic][id];
+                }
+        }
+      }
+    }
+    const int oc = sC.ao, od = sD.ao;
+    for (int bt = 0; bt < a.nbd; ++bt) {
+      const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
+      if (a.with_j && bt < a.nbj) {
+        double* J = a.jp + (size_t)bt * a.bstride + mi.base;
+        double* jr = jrow + (bt * a.nbra_max + slot) * W::NBRA;
+        double dcd[WC][WD], dab[WA][WB];
+#pragma unroll
+        for (int ic = 0; ic < WC; ++ic)
+#pragma unroll
+          for (int id = 0; id < WD; ++id)
+            dcd[ic][id] = Dm[(size_t)(oc + ic) * n + od + id] + Dm[(size_t)(od + id) * n + oc + ic];
+#pragma unroll
+        for (int ia = 0; ia < WA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < WB; ++ib)
+            dab[ia][ib] = Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia];
+#pragma unroll
+        for (int ia = 0; ia < WA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < WB; ++ib) {
+            double sj = 0.0;
+#pragma unroll
+            for (int ic = 0; ic < WC; ++ic)
+#pragma unroll
+              for (int id = 0; id < WD; ++id) sj += (double)acc[ia][ib][ic][id] * dcd[ic][id];
+            atomicAdd(jr + ia * WB + ib, f * sj);
+          }
+#pragma unroll
+        for (int ic = 0; ic < WC; ++ic)
+#pragma unroll
+          for (int id = 0; id < WD; ++id) {
+            double sj = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < WA; ++ia)
+#pragma unroll
+              for (int ib = 0; ib < WB; ++ib) sj += (double)acc[ia][ib][ic][id] * dab[ia][ib];
+            if (jket)
+              atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * W::NKET + ic * WD + id, f * sj);
+            else
+              atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * sj);
+          }
+      }
+      if (a.with_k) {
+        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
+        double dbd[WB][WD], dbc[WB][WC], dad[WA][WD], dac[WA][WC];
+#pragma unroll
+        for (int ib = 0; ib < WB; ++ib) {
+#pragma unroll
+          for (int id = 0; id < WD; ++id) dbd[ib][id] = Dm[(size_t)(ob + ib) * n + od + id];
+#pragma unroll
+          for (int ic = 0; ic < WC; ++ic) dbc[ib][ic] = Dm[(size_t)(ob + ib) * n + oc + ic];
+        }
+#pragma unroll
+        for (int ia = 0; ia < WA; ++ia) {
+#pragma unroll
+          for (int id = 0; id < WD; ++id) dad[ia][id] = Dm[(size_t)(oa + ia) * n + od + id];
+#pragma unroll
+          for (int ic = 0; ic < WC; ++ic) dac[ia][ic] = Dm[(size_t)(oa + ia) * n + oc + ic];
+        }
+#pragma unroll
+        for (int ia = 0; ia < WA; ++ia) {
+#pragma unroll
+          for (int ic = 0; ic < WC; ++ic) {
+            double sk = 0.0;
+#pragma unroll
+            for (int ib = 0; ib < WB; ++ib)
+#pragma unroll
+              for (int id = 0; id < WD; ++id) sk += (double)acc[ia][ib][ic][id] * dbd[ib][id];
+            atomicAdd(kr + (size_t)ia * ns + oc + ic, f * sk);
+          }
+#pragma unroll
+          for (int id = 0; id < WD; ++id) {
+            double sk = 0.0;
+#pragma unroll
+            for (int ib = 0; ib < WB; ++ib)
+#pragma unroll
+              for (int ic = 0; ic < WC; ++ic) sk += (double)acc[ia][ib][ic][id] * dbc[ib][ic];
+            atomicAdd(kr + (size_t)ia * ns + od + id, f * sk);
+          }
+        }
+#pragma unroll
+        for (int ib = 0; ib < WB; ++ib) {
+#pragma unroll
+          for (int ic = 0; ic < WC; ++ic) {
+            double sk = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < WA; ++ia)
+#pragma unroll
+              for (int id = 0; id < WD; ++id) sk += (double)acc[ia][ib][ic][id] * dad[ia][id];
+            atomicAdd(kr + (size_t)(WA + ib) * ns + oc + ic, f * sk);
+          }
+#pragma unroll
+          for (int id = 0; id < WD; ++id) {
+            double sk = 0.0;
+#pragma unroll
+            for (int ia = 0; ia < WA; ++ia)
+#pragma unroll
+              for (int ic = 0; ic < WC; ++ic) sk += (double)acc[ia][ib][ic][id] * dac[ia][ic];
+            atomicAdd(kr + (size_t)(WA + ib) * ns + od + id, f * sk);
+          }
+        }
+      }
+    }
+  } else {
+    // this lane: wide ket component kc = (column z, component ic) x (column w, component id)
+    const int icw = kc / WD, idw = kc - (kc / WD) * WD;
+    const int zc = icw / K::NC, ic = icw - zc * K::NC;
+    const int wcol = idw / K::ND, id = idw - wcol * K::ND;
+    int ec[3], ed[3];
+    cart_xyz<LC>(ic, ec);
+    cart_xyz<LD>(id, ed);
+    R wk[3][K::MM + 1];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) ket_weights<K::MM, R>(ec[x], ed[x], CD_[x], wk[x]);
+    R acc[WA][WB];
+#pragma unroll
+    for (int ia = 0; ia < WA; ++ia)
+#pragma unroll
+      for (int ib = 0; ib < WB; ++ib) acc[ia][ib] = R(0);
+    for (int ip = s0; ip < bp.np; ip += S) {
+      const PrimPair P = a.prims[bp.poff + ip];
+      PrimCoef cP = {{1.0, 0.0}, {1.0, 0.0}};
+      if constexpr (CA * CB > 1) cP = a.coef[bp.poff + ip];
+      for (int kq = 0; kq < kp.np; ++kq) {
+        const PrimPair Q = a.prims[kp.poff + kq];
+        PrimQ<K::NR> q;
+        prim_setup<K::NR>(P, Q, geo, q, tab);
+        if constexpr (!W::GEN) {
+          accumulate_split<LA, LB, LC, LD, R>(q, AB, wk, acc);
+        } else {
+          PrimCoef cQ = {{1.0, 0.0}, {1.0, 0.0}};
+          if constexpr (CC * CD > 1) cQ = a.coef[kp.poff + kq];
+          R blk[K::NA][K::NB];
+#pragma unroll
+          for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+            for (int ib = 0; ib < K::NB; ++ib) blk[ia][ib] = R(0);
+          accumulate_split<LA, LB, LC, LD, R>(q, AB, wk, blk);
+          const double ck = cQ.ca[zc] * cQ.cb[wcol];
+#pragma unroll
+          for (int x = 0; x < CA; ++x)
+#pragma unroll
+            for (int y = 0; y < CB; ++y) {
+              const R cf = (R)(cP.ca[x] * cP.cb[y] * ck);
+#pragma unroll
+              for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+                for (int ib = 0; ib < K::NB; ++ib) acc[x * K::NA + ia][y * K::NB + ib] += cf * blk[ia][ib];
+            }
+        }
+      }
+    }
+    const int c = sC.ao + icw, d = sD.ao + idw;
+    for (int bt = 0; bt < a.nbd; ++bt) {
+      const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
+      if (a.with_j && bt < a.nbj) {
+        double* J = a.jp + (size_t)bt * a.bstride + mi.base;
+        double* jr = jrow + (bt * a.nbra_max + slot) * W::NBRA;
+        const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
+        double s2 = 0.0;
+#pragma unroll
+        for (int ia = 0; ia < WA; ++ia)
+#pragma unroll
+          for (int ib = 0; ib < WB; ++ib) {
+            atomicAdd(jr + ia * WB + ib, f * (double)acc[ia][ib] * dcd);
+            s2 += (double)acc[ia][ib] * (Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia]);
+          }
+        if (jket)
+          atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * W::NKET + kc, f * s2);
+        else
+          atomicAdd(J + (size_t)c * n + d, f * s2);
+      }
+      if (a.with_k) {
+        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
+        double db_d[WB], db_c[WB];
+#pragma unroll
+        for (int ib = 0; ib < WB; ++ib) {
+          db_d[ib] = Dm[(size_t)(ob + ib) * n + d];
+          db_c[ib] = Dm[(size_t)(ob + ib) * n + c];
+        }
+#pragma unroll
+        for (int ia = 0; ia < WA; ++ia) {
+          double sc = 0.0, sd = 0.0;
+#pragma unroll
+          for (int ib = 0; ib < WB; ++ib) {
+            sc += (double)acc[ia][ib] * db_d[ib];
+            sd += (double)acc[ia][ib] * db_c[ib];
+          }
+          atomicAdd(kr + (size_t)ia * ns + c, f * sc);
+          atomicAdd(kr + (size_t)ia * ns + d, f * sd);
+        }
+        double da_d[WA], da_c[WA];
+#pragma unroll
+        for (int ia = 0; ia < WA; ++ia) {
+          da_d[ia] = Dm[(size_t)(oa + ia) * n + d];
+          da_c[ia] = Dm[(size_t)(oa + ia) * n + c];
+        }
+#pragma unroll
+        for (int ib = 0; ib < WB; ++ib) {
+          double sc = 0.0, sd = 0.0;
+#pragma unroll
+          for (int ia = 0; ia < WA; ++ia) {
+            sc += (double)acc[ia][ib] * da_d[ia];
+            sd += (double)acc[ia][ib] * da_c[ia];
+          }
+          atomicAdd(kr + (size_t)(WA + ib) * ns + c, f * sc);
+          atomicAdd(kr + (size_t)(WA + ib) * ns + d, f * sd);
+        }
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Bra-major J/K kernel: one workgroup per chunk of quartets sharing up to
+// nbra_max consecutive bra pairs.  All exchange terms of the 8-fold sum land in
+// rows of the bra shells (Kp[a,c], Kp[a,d], Kp[b,c], Kp[b,d]), so they are
+// accumulated with LDS atomics in per-bra row slabs [rows of A, rows of B] x n
+// and flushed to global memory once per workgroup; the bra-side J block is
+// accumulated in LDS the same way.  Only the ket-side J terms go straight to
+// global atomics.  Global FP64 atomics are executed at the memory side on this
+// platform, so their count, not FLOPs, is what this layout minimises.
+// ----------------------------------------------------------------------------
 // One work item of the bra-major kernel in real type R: evaluate the quartet
 // (or its ket-component slice) over bra primitive pairs s0, s0+S, ... and
 // digest it (exchange rows / bra J block into LDS, ket J block to global).
@@ -712,20 +1037,21 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
 // With TLDS the Rys table slice of the class (rys_slice_size(NR) doubles) is
 // copied to LDS after the slabs, so the per-lane interval gathers of the root
 // evaluation read LDS instead of global memory.
-template <int LA, int LB, int LC, int LD, bool TLDS, bool MIX>
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool TLDS, bool MIX>
 __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   using K = Class<LA, LB, LC, LD>;
-  constexpr int ROWS = K::NA + K::NB;
+  using W = Wide<LA, LB, LC, LD, CA, CB, CC, CD>;
+  constexpr int ROWS = W::WA + W::WB;
   extern __shared__ double lds[];
   const int ns = a.n;  // LDS row length (widest molecule)
   const BraChunk ch = a.chunks[blockIdx.x];
   const int krow_size = a.nbd * a.nbra_max * ROWS * ns;
   double* krow = lds;                      // [nbd][slot][ROWS][n]
   double* jrow = lds + krow_size;          // [nbj][slot][NA][NB]
-  const int jrow_size = a.nbj * a.nbra_max * K::NBRA;
+  const int jrow_size = a.nbj * a.nbra_max * W::NBRA;
   // ket-side J of the chunk's ket window: [nbj][nket_max][NC*ND]
   double* jket = (a.nket_max > 0 && ch.nket > 0) ? jrow + jrow_size : nullptr;
-  const int jket_size = a.nbj * a.nket_max * K::NKET;
+  const int jket_size = a.nbj * a.nket_max * W::NKET;
   const int zero_size = krow_size + jrow_size + (jket ? jket_size : 0);
   for (int i = threadIdx.x; i < zero_size; i += blockDim.x) lds[i] = 0.0;
   const double* tab = rys_tab + rys_table_offset[K::NR - 1];
@@ -735,10 +1061,10 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     tab = ltab;
   }
   __syncthreads();
-  const int64_t total = (int64_t)ch.nq * ch.S * K::SPLIT;
+  const int64_t total = (int64_t)ch.nq * ch.S * W::SPLIT;
   for (int64_t it = threadIdx.x; it < total; it += blockDim.x) {
-    const int kc = (int)(it % K::SPLIT);
-    const int64_t rest = it / K::SPLIT;
+    const int kc = (int)(it % W::SPLIT);
+    const int64_t rest = it / W::SPLIT;
     const int s0 = (int)(rest / ch.nq);
     const int64_t g = ch.q0 + (rest - (int64_t)s0 * ch.nq);
     const int2 qk = a.quartets[g];
@@ -763,11 +1089,11 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     }
     if constexpr (MIX) {
       if (lowp) {
-        bra_item<LA, LB, LC, LD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
+        bra_item<LA, LB, LC, LD, CA, CB, CC, CD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
         continue;
       }
     }
-    bra_item<LA, LB, LC, LD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
+    bra_item<LA, LB, LC, LD, CA, CB, CC, CD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
   }
   __syncthreads();
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
@@ -781,32 +1107,32 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     if (slot >= ch.nbra) continue;
     const Pair bp = a.pairs[ch.bra0 + slot];
     const MolInfo mi = a.mols[a.shells[bp.sa].mol];
-    const int row = r < K::NA ? a.shells[bp.sa].ao + r : a.shells[bp.sb].ao + (r - K::NA);
+    const int row = r < W::WA ? a.shells[bp.sa].ao + r : a.shells[bp.sb].ao + (r - W::WA);
     atomicAdd(a.kp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
   }
   for (int i = threadIdx.x; i < jrow_size; i += blockDim.x) {
     const double v = jrow[i];
     if (v == 0.0) continue;
-    const int e = i % K::NBRA;
-    const int slot = (i / K::NBRA) % a.nbra_max;
-    const int bt = i / (K::NBRA * a.nbra_max);
+    const int e = i % W::NBRA;
+    const int slot = (i / W::NBRA) % a.nbra_max;
+    const int bt = i / (W::NBRA * a.nbra_max);
     if (slot >= ch.nbra) continue;
     const Pair bp = a.pairs[ch.bra0 + slot];
     const MolInfo mi = a.mols[a.shells[bp.sa].mol];
-    const int row = a.shells[bp.sa].ao + e / K::NB, col = a.shells[bp.sb].ao + e % K::NB;
+    const int row = a.shells[bp.sa].ao + e / W::WB, col = a.shells[bp.sb].ao + e % W::WB;
     atomicAdd(a.jp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
   }
   if (jket) {
-    const int nk = a.nbj * ch.nket * K::NKET;
+    const int nk = a.nbj * ch.nket * W::NKET;
     for (int i = threadIdx.x; i < nk; i += blockDim.x) {
-      const int e = i % K::NKET;
-      const int w = (i / K::NKET) % ch.nket;
-      const int bt = i / (K::NKET * ch.nket);
-      const double v = jket[((size_t)bt * a.nket_max + w) * K::NKET + e];
+      const int e = i % W::NKET;
+      const int w = (i / W::NKET) % ch.nket;
+      const int bt = i / (W::NKET * ch.nket);
+      const double v = jket[((size_t)bt * a.nket_max + w) * W::NKET + e];
       if (v == 0.0) continue;
       const Pair kp = a.pairs[ch.ket0 + w];
       const MolInfo mi = a.mols[a.shells[kp.sa].mol];
-      const int row = a.shells[kp.sa].ao + e / K::ND, col = a.shells[kp.sb].ao + e % K::ND;
+      const int row = a.shells[kp.sa].ao + e / W::WD, col = a.shells[kp.sb].ao + e % W::WD;
       atomicAdd(a.jp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
     }
   }
@@ -884,19 +1210,52 @@ void launch_blocks_t(const BlockArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((blocks_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD>
 void launch_jk_bra_t(const BraArgs& a, size_t lds_bytes, hipStream_t s) {
   if (a.nchunks <= 0) return;
   const dim3 block(a.threads), grid((unsigned)a.nchunks);
   // variants: Rys table in LDS or global; FP64 only, or with the FP32 quartet path
   if (a.tab_lds && a.mixed)
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, true, true>), grid, block, lds_bytes, s, a);
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, true, true>), grid, block, lds_bytes, s, a);
   else if (a.tab_lds)
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, true, false>), grid, block, lds_bytes, s, a);
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, true, false>), grid, block, lds_bytes, s, a);
   else if (a.mixed)
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, false, true>), grid, block, lds_bytes, s, a);
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, true>), grid, block, lds_bytes, s, a);
   else
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, false, false>), grid, block, lds_bytes, s, a);
+    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false>), grid, block, lds_bytes, s, a);
+}
+
+// Column-pattern dispatch of the bra-major kernel (patterns: columns of the
+// pair's shells (A, B) = (1 + (p & 1), 1 + (p >> 1))).
+template <int LA, int LB, int LC, int LD, int PB, int PK>
+bool launch_jk_bra_if(const BraArgs& a, int pb, int pk, size_t lds, hipStream_t s) {
+  if constexpr (pattern_valid(LA, LB, PB) && pattern_valid(LC, LD, PK)) {
+    if (pb == PB && pk == PK) {
+      launch_jk_bra_t<LA, LB, LC, LD, 1 + (PB & 1), 1 + (PB >> 1), 1 + (PK & 1), 1 + (PK >> 1)>(a, lds, s);
+      return true;
+    }
+  }
+  return false;
+}
+
+template <int LA, int LB, int LC, int LD>
+bool launch_jk_bra_patterns(const BraArgs& a, int pb, int pk, size_t lds, hipStream_t s) {
+  return launch_jk_bra_if<LA, LB, LC, LD, 0, 0>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 0, 1>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 0, 2>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 0, 3>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 1, 0>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 1, 1>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 1, 3>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 2, 0>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 2, 1>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 2, 2>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 2, 3>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 3, 0>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 3, 1>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 3, 3>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 1, 2>(a, pb, pk, lds, s) ||
+         launch_jk_bra_if<LA, LB, LC, LD, 3, 2>(a, pb, pk, lds, s);
 }
 
 }  // namespace gscf

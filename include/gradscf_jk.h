@@ -84,6 +84,17 @@ int gscf_tuning_set(int32_t cls, const gscf_class_tuning* t);
 int gscf_tuning_get(int32_t cls, gscf_class_tuning* t);
 int gscf_tuning_reset(void);
 
+/* Process-wide engine options (no counterpart in the reference), copied by
+ * plans when they are created, like the tuning table. */
+typedef struct {
+  int32_t general_contraction; /* 1: two-column s shells (cc-pVXZ 1s/2s) evaluated once per
+                                  primitive quartet for both columns; 0: as two segments */
+  int32_t streams;             /* 1..4 HIP streams the class launches of one J/K build are
+                                  spread over (fork/join on the caller's stream) */
+} gscf_options;
+int gscf_options_set(const gscf_options* o);
+int gscf_options_get(gscf_options* o);
+
 /* Last error message of the calling thread. */
 const char* gscf_last_error(void);
 

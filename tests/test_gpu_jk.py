@@ -356,3 +356,44 @@ def test_schedules_do_not_change_results(sched):
         assert float((got_m - ref_m).abs().max()) < 1e-11
     finally:
         at.reset()
+
+
+@pytest.mark.parametrize("with_j,with_k", [(True, True), (True, False), (False, True)])
+def test_many_nonsymmetric_densities_match_single_builds(with_j, with_k):
+    """(H2O)10 cc-pVDZ (250 Cartesian AO, general s contractions): two
+    nonsymmetric densities in one build exceed the LDS of the ket-general
+    classes, which then run one density per launch; the result must equal two
+    single-density builds (operator selection included)."""
+    import torch
+    from gradscf_amd import molecule as M
+    _, _, atm, bas, env = systems.tables(M.water_cluster(10, seed=10, box=9.0), "cc-pvdz", cart=True)
+    plan = _plan(atm, bas, env, True, 1e-12)
+    d = _cuda(systems.random_density(plan.nao, nb=2, seed=12, symmetric=False))
+    both = plan.jk(d, with_j=with_j, with_k=with_k)
+    for b in range(2):
+        one = plan.jk(d[b:b + 1], with_j=with_j, with_k=with_k)
+        assert float((both[:, b] - one[:, 0]).abs().max()) < 1e-12
+
+
+@pytest.mark.parametrize("gc,streams", [(0, 4), (1, 1), (0, 1)])
+def test_engine_options_do_not_change_results(gc, streams):
+    """(H2O)4 cc-pVDZ spherical: segmented instead of native general
+    contraction, and class launches on one stream instead of four, change J/K
+    only by the order of FP64 additions (nonsymmetric density, J and K)."""
+    import torch
+    from gradscf_amd import autotune as at
+    from gradscf_amd import molecule as M
+    _, _, atm, bas, env = systems.tables(M.water_cluster(4, seed=4, box=7.0), "cc-pvdz", cart=False)
+    ref_plan = _plan(atm, bas, env, False, 1e-12)
+    d = _cuda(systems.random_density(ref_plan.nao, nb=1, seed=5, symmetric=False))
+    ref = ref_plan.jk(d)
+    before = at.get_options()
+    try:
+        at.set_options(general_contraction=gc, streams=streams)
+        plan = _plan(atm, bas, env, False, 1e-12)
+        got = plan.jk(d)
+        got2 = plan.jk(d)
+    finally:
+        at.set_options(**before)
+    assert float((got - ref).abs().max()) < 1e-11
+    assert float((got2 - got).abs().max()) < 1e-11

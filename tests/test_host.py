@@ -260,6 +260,25 @@ def test_tuning_table_api_without_gpu():
     assert at.get_schedule((2, 1, 1, 0)) == at.Schedule()
 
 
+def test_engine_options_api_without_gpu():
+    if not (ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so").is_file():
+        pytest.skip("engine not built")
+    from gradscf_amd import autotune as at
+    before = at.get_options()
+    assert before == {"general_contraction": 1, "streams": 4}
+    try:
+        at.set_options(general_contraction=0, streams=2)
+        assert at.get_options() == {"general_contraction": 0, "streams": 2}
+        for bad in ({"streams": 0}, {"streams": 5}, {"general_contraction": 2}):
+            with pytest.raises(ValueError):
+                at.set_options(**bad)
+        assert at.get_options() == {"general_contraction": 0, "streams": 2}
+        with pytest.raises(ValueError):
+            at.set_options(threads=4)
+    finally:
+        at.set_options(**before)
+
+
 def test_basis_gradient_chain_rule_matches_fd():
     """The normalisation chain (env-slot gradient -> raw exponents and raw
     coefficients) against central differences of the numpy normalisation."""
