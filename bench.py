@@ -520,6 +520,10 @@ def main() -> int:
                     help="run the autotuner even where a committed schedule table exists")
     ap.add_argument("--save-tuning", metavar="DIR", help="write autotuned schedule tables into DIR")
     ap.add_argument("--mixed-threshold", type=float, default=1e-7)
+    ap.add_argument("--streams", type=int, default=None,
+                    help="HIP streams the class launches of a build are spread over (engine default 4; "
+                         "1 serialises them, as for a kernel trace whose per-kernel durations are compared "
+                         "with the per-class timings)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the copy kernel that calibrates the HBM byte counters")
     ap.add_argument("--summarize-profile", nargs="+", metavar="DIR",
@@ -560,6 +564,8 @@ def main() -> int:
         at.load_table(table_path)   # every plan below uses the committed schedules
     else:
         at.reset()
+    if args.streams is not None:
+        at.set_options(streams=args.streams)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plan = DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff)
@@ -691,6 +697,7 @@ def main() -> int:
                            "tflops_eval": tot["flop_eval"] / (total_kernel_ms * 1e-3) / 1e12},
         "classes": per_class,
         "autotune": tuning,
+        "engine_options": at.get_options(),
     }
     water = None
     if not args.no_extra:

@@ -984,7 +984,11 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
               t.q0 = qc.nq;
               t.nq = nq;
               const int64_t work = (int64_t)rb.npmax * rk.npmax;
-              t.S = (int)std::max<int64_t>(1, std::min<int64_t>(rb.npmin, (work + kTargetPrim - 1) / kTargetPrim));
+              // primitive split: a power of two <= 64 (the bra-major kernel
+              // sums the S slices of a quartet across adjacent lanes)
+              const int64_t sreq = std::max<int64_t>(1, std::min<int64_t>(rb.npmin, (work + kTargetPrim - 1) / kTargetPrim));
+              t.S = 1;
+              while (t.S < 64 && 2 * t.S <= sreq) t.S *= 2;
               t.pad = 0;
               t.item0 = qc.nitems;
               qc.nitems += (int64_t)t.S * nq;

@@ -455,15 +455,16 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
   }
 }
 
-// ----------------------------------------------------------------------------
-// Bra-major J/K kernel: one workgroup per chunk of quartets sharing up to
-// nbra_max consecutive bra pairs.  All exchange terms of the 8-fold sum land in
-// rows of the bra shells (Kp[a,c], Kp[a,d], Kp[b,c], Kp[b,d]), so they are
-// accumulated with LDS atomics in per-bra row slabs [rows of A, rows of B] x n
-// and flushed to global memory once per workgroup; the bra-side J block is
-// accumulated in LDS the same way.  Only the ket-side J terms go straight to
-// global atomics.  Global FP64 atomics are executed at the memory side on this
-// platform, so their count, not FLOPs, is what this layout minimises.
+// Sum of v[0..N) over the S adjacent lanes of a lane group (S a power of two
+// <= 64, groups aligned to S): butterfly exchange, every lane ends with the sum.
+template <int N, typename R>
+__device__ __forceinline__ void group_sum(R* v, int S) {
+  for (int m = 1; m < S; m <<= 1) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] += __shfl_xor(v[i], m);
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Column layout of a quartet class on a contracted pair set: shell X carries
 // CX contraction columns and its "wide" block WX = CX * ncart(lX) components
@@ -524,8 +525,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
       // a side with one column already carries its coefficients in e
       PrimCoef cP = {{1.0, 0.0}, {1.0, 0.0}};
       if constexpr (CA * CB > 1) cP = a.coef[bp.poff + ip];
+      const PrimPair* __restrict__ kpr = a.prims + kp.poff;
       for (int kq = 0; kq < kp.np; ++kq) {
-        const PrimPair Q = a.prims[kp.poff + kq];
+        const PrimPair Q = kpr[kq];
         PrimQ<K::NR> q;
         prim_setup<K::NR>(P, Q, geo, q, tab);
         if constexpr (!W::GEN) {
@@ -566,6 +568,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
         }
       }
     }
+    group_sum<WA * WB * WC * WD>(&acc[0][0][0][0], S);
+    if (s0 != 0) return;
     const int oc = sC.ao, od = sD.ao;
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
@@ -690,8 +694,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
       const PrimPair P = a.prims[bp.poff + ip];
       PrimCoef cP = {{1.0, 0.0}, {1.0, 0.0}};
       if constexpr (CA * CB > 1) cP = a.coef[bp.poff + ip];
+      const PrimPair* __restrict__ kpr = a.prims + kp.poff;
       for (int kq = 0; kq < kp.np; ++kq) {
-        const PrimPair Q = a.prims[kp.poff + kq];
+        const PrimPair Q = kpr[kq];
         PrimQ<K::NR> q;
         prim_setup<K::NR>(P, Q, geo, q, tab);
         if constexpr (!W::GEN) {
@@ -719,6 +724,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
         }
       }
     }
+    group_sum<WA * WB>(&acc[0][0], S);
+    if (s0 != 0) return;
     const int c = sC.ao + icw, d = sD.ao + idw;
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
@@ -789,246 +796,6 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
 // accumulated in LDS the same way.  Only the ket-side J terms go straight to
 // global atomics.  Global FP64 atomics are executed at the memory side on this
 // platform, so their count, not FLOPs, is what this layout minimises.
-// ----------------------------------------------------------------------------
-// One work item of the bra-major kernel in real type R: evaluate the quartet
-// (or its ket-component slice) over bra primitive pairs s0, s0+S, ... and
-// digest it (exchange rows / bra J block into LDS, ket J block to global).
-template <int LA, int LB, int LC, int LD, typename R>
-__device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const Pair& kp, const int2 qk, int slot,
-                                         int s0, int S, int kc, double* __restrict__ krow,
-                                         double* __restrict__ jrow, double* __restrict__ jket, int kwin0,
-                                         const double* __restrict__ tab) {
-  using K = Class<LA, LB, LC, LD>;
-  constexpr int ROWS = K::NA + K::NB;
-  const int ns = a.n;  // LDS row length
-  const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
-  const MolInfo mi = a.mols[sA.mol];
-  const int n = mi.n;
-  QuartetGeom geo;
-  double ABd[3], CDd[3];
-  load_geom(sA, sB, sC, sD, geo, ABd, CDd);
-  const R AB[3] = {(R)ABd[0], (R)ABd[1], (R)ABd[2]};
-  const R CD[3] = {(R)CDd[0], (R)CDd[1], (R)CDd[2]};
-  double f = 1.0;
-  if (bp.sa == bp.sb) f *= 0.5;
-  if (kp.sa == kp.sb) f *= 0.5;
-  if (qk.x == qk.y) f *= 0.5;
-  const int oa = sA.ao, ob = sB.ao;
-  if constexpr (!K::kSplit) {
-    R acc[K::NA][K::NB][K::NC][K::ND];
-#pragma unroll
-    for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-      for (int ib = 0; ib < K::NB; ++ib)
-#pragma unroll
-        for (int ic = 0; ic < K::NC; ++ic)
-#pragma unroll
-          for (int id = 0; id < K::ND; ++id) acc[ia][ib][ic][id] = R(0);
-    for (int ip = s0; ip < bp.np; ip += S) {
-      const PrimPair P = a.prims[bp.poff + ip];
-      for (int kq = 0; kq < kp.np; ++kq) {
-        const PrimPair Q = a.prims[kp.poff + kq];
-        PrimQ<K::NR> q;
-        prim_setup<K::NR>(P, Q, geo, q, tab);
-        accumulate_full<LA, LB, LC, LD, R>(q, AB, CD, acc);
-      }
-    }
-    const int oc = sC.ao, od = sD.ao;
-    for (int bt = 0; bt < a.nbd; ++bt) {
-      const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
-      if (a.with_j && bt < a.nbj) {
-        double* J = a.jp + (size_t)bt * a.bstride + mi.base;
-        double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
-        double dcd[K::NC][K::ND], dab[K::NA][K::NB];
-#pragma unroll
-        for (int ic = 0; ic < K::NC; ++ic)
-#pragma unroll
-          for (int id = 0; id < K::ND; ++id)
-            dcd[ic][id] = Dm[(size_t)(oc + ic) * n + od + id] + Dm[(size_t)(od + id) * n + oc + ic];
-#pragma unroll
-        for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-          for (int ib = 0; ib < K::NB; ++ib)
-            dab[ia][ib] = Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia];
-#pragma unroll
-        for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-          for (int ib = 0; ib < K::NB; ++ib) {
-            double sj = 0.0;
-#pragma unroll
-            for (int ic = 0; ic < K::NC; ++ic)
-#pragma unroll
-              for (int id = 0; id < K::ND; ++id) sj += (double)acc[ia][ib][ic][id] * dcd[ic][id];
-            atomicAdd(jr + ia * K::NB + ib, f * sj);
-          }
-#pragma unroll
-        for (int ic = 0; ic < K::NC; ++ic)
-#pragma unroll
-          for (int id = 0; id < K::ND; ++id) {
-            double sj = 0.0;
-#pragma unroll
-            for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-              for (int ib = 0; ib < K::NB; ++ib) sj += (double)acc[ia][ib][ic][id] * dab[ia][ib];
-            if (jket)
-              atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * K::NKET + ic * K::ND + id, f * sj);
-            else
-              atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * sj);
-          }
-      }
-      if (a.with_k) {
-        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
-        double dbd[K::NB][K::ND], dbc[K::NB][K::NC], dad[K::NA][K::ND], dac[K::NA][K::NC];
-#pragma unroll
-        for (int ib = 0; ib < K::NB; ++ib) {
-#pragma unroll
-          for (int id = 0; id < K::ND; ++id) dbd[ib][id] = Dm[(size_t)(ob + ib) * n + od + id];
-#pragma unroll
-          for (int ic = 0; ic < K::NC; ++ic) dbc[ib][ic] = Dm[(size_t)(ob + ib) * n + oc + ic];
-        }
-#pragma unroll
-        for (int ia = 0; ia < K::NA; ++ia) {
-#pragma unroll
-          for (int id = 0; id < K::ND; ++id) dad[ia][id] = Dm[(size_t)(oa + ia) * n + od + id];
-#pragma unroll
-          for (int ic = 0; ic < K::NC; ++ic) dac[ia][ic] = Dm[(size_t)(oa + ia) * n + oc + ic];
-        }
-#pragma unroll
-        for (int ia = 0; ia < K::NA; ++ia) {
-#pragma unroll
-          for (int ic = 0; ic < K::NC; ++ic) {
-            double sk = 0.0;
-#pragma unroll
-            for (int ib = 0; ib < K::NB; ++ib)
-#pragma unroll
-              for (int id = 0; id < K::ND; ++id) sk += (double)acc[ia][ib][ic][id] * dbd[ib][id];
-            atomicAdd(kr + (size_t)ia * ns + oc + ic, f * sk);
-          }
-#pragma unroll
-          for (int id = 0; id < K::ND; ++id) {
-            double sk = 0.0;
-#pragma unroll
-            for (int ib = 0; ib < K::NB; ++ib)
-#pragma unroll
-              for (int ic = 0; ic < K::NC; ++ic) sk += (double)acc[ia][ib][ic][id] * dbc[ib][ic];
-            atomicAdd(kr + (size_t)ia * ns + od + id, f * sk);
-          }
-        }
-#pragma unroll
-        for (int ib = 0; ib < K::NB; ++ib) {
-#pragma unroll
-          for (int ic = 0; ic < K::NC; ++ic) {
-            double sk = 0.0;
-#pragma unroll
-            for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-              for (int id = 0; id < K::ND; ++id) sk += (double)acc[ia][ib][ic][id] * dad[ia][id];
-            atomicAdd(kr + (size_t)(K::NA + ib) * ns + oc + ic, f * sk);
-          }
-#pragma unroll
-          for (int id = 0; id < K::ND; ++id) {
-            double sk = 0.0;
-#pragma unroll
-            for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-              for (int ic = 0; ic < K::NC; ++ic) sk += (double)acc[ia][ib][ic][id] * dac[ia][ic];
-            atomicAdd(kr + (size_t)(K::NA + ib) * ns + od + id, f * sk);
-          }
-        }
-      }
-    }
-  } else {
-    const int ic = kc / K::ND, id = kc - (kc / K::ND) * K::ND;
-    int ec[3], ed[3];
-    cart_xyz<LC>(ic, ec);
-    cart_xyz<LD>(id, ed);
-    R wk[3][K::MM + 1];
-#pragma unroll
-    for (int x = 0; x < 3; ++x) ket_weights<K::MM, R>(ec[x], ed[x], CD[x], wk[x]);
-    R acc[K::NA][K::NB];
-#pragma unroll
-    for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-      for (int ib = 0; ib < K::NB; ++ib) acc[ia][ib] = R(0);
-    for (int ip = s0; ip < bp.np; ip += S) {
-      const PrimPair P = a.prims[bp.poff + ip];
-      for (int kq = 0; kq < kp.np; ++kq) {
-        const PrimPair Q = a.prims[kp.poff + kq];
-        PrimQ<K::NR> q;
-        prim_setup<K::NR>(P, Q, geo, q, tab);
-        accumulate_split<LA, LB, LC, LD, R>(q, AB, wk, acc);
-      }
-    }
-    const int c = sC.ao + ic, d = sD.ao + id;
-    for (int bt = 0; bt < a.nbd; ++bt) {
-      const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
-      if (a.with_j && bt < a.nbj) {
-        double* J = a.jp + (size_t)bt * a.bstride + mi.base;
-        double* jr = jrow + (bt * a.nbra_max + slot) * K::NBRA;
-        const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
-        double s2 = 0.0;
-#pragma unroll
-        for (int ia = 0; ia < K::NA; ++ia)
-#pragma unroll
-          for (int ib = 0; ib < K::NB; ++ib) {
-            atomicAdd(jr + ia * K::NB + ib, f * (double)acc[ia][ib] * dcd);
-            s2 += (double)acc[ia][ib] * (Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia]);
-          }
-        if (jket)
-          atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * K::NKET + kc, f * s2);
-        else
-          atomicAdd(J + (size_t)c * n + d, f * s2);
-      }
-      if (a.with_k) {
-        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
-        double db_d[K::NB], db_c[K::NB];
-#pragma unroll
-        for (int ib = 0; ib < K::NB; ++ib) {
-          db_d[ib] = Dm[(size_t)(ob + ib) * n + d];
-          db_c[ib] = Dm[(size_t)(ob + ib) * n + c];
-        }
-#pragma unroll
-        for (int ia = 0; ia < K::NA; ++ia) {
-          double sc = 0.0, sd = 0.0;
-#pragma unroll
-          for (int ib = 0; ib < K::NB; ++ib) {
-            sc += (double)acc[ia][ib] * db_d[ib];
-            sd += (double)acc[ia][ib] * db_c[ib];
-          }
-          atomicAdd(kr + (size_t)ia * ns + c, f * sc);
-          atomicAdd(kr + (size_t)ia * ns + d, f * sd);
-        }
-        double da_d[K::NA], da_c[K::NA];
-#pragma unroll
-        for (int ia = 0; ia < K::NA; ++ia) {
-          da_d[ia] = Dm[(size_t)(oa + ia) * n + d];
-          da_c[ia] = Dm[(size_t)(oa + ia) * n + c];
-        }
-#pragma unroll
-        for (int ib = 0; ib < K::NB; ++ib) {
-          double sc = 0.0, sd = 0.0;
-#pragma unroll
-          for (int ia = 0; ia < K::NA; ++ia) {
-            sc += (double)acc[ia][ib] * da_d[ia];
-            sd += (double)acc[ia][ib] * da_c[ia];
-          }
-          atomicAdd(kr + (size_t)(K::NA + ib) * ns + c, f * sc);
-          atomicAdd(kr + (size_t)(K::NA + ib) * ns + d, f * sd);
-        }
-      }
-    }
-  }
-}
-
-// ----------------------------------------------------------------------------
-// Bra-major J/K kernel: one workgroup per chunk of quartets sharing up to
-// nbra_max consecutive bra pairs.  All exchange terms of the 8-fold sum land in
-// rows of the bra shells (Kp[a,c], Kp[a,d], Kp[b,c], Kp[b,d]), so they are
-// accumulated with LDS atomics in per-bra row slabs [rows of A, rows of B] x n
-// and flushed to global memory once per workgroup; the bra-side J block is
-// accumulated in LDS the same way.  Only the ket-side J terms go straight to
-// global atomics.  Global FP64 atomics are executed at the memory side on this
-// platform, so their count, not FLOPs, is what this layout minimises.
 // With a.mixed set, a quartet whose Q_ij Q_kl max|D| (max over the six density
 // blocks it touches) is below a.thr may be evaluated in FP32 (roots, weights
 // and prefactors stay FP64, digestion accumulates in FP64); the choice is made
@@ -1061,12 +828,17 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     tab = ltab;
   }
   __syncthreads();
+  // item -> (quartet, ket component kc, bra-primitive slice s0), s0 fastest:
+  // the S slices of one quartet component occupy S adjacent lanes of one
+  // wavefront (S is a power of two <= 64 and blockDim a multiple of 64), which
+  // add their partial integrals by cross-lane exchange before the s0 = 0 lane
+  // digests them alone
   const int64_t total = (int64_t)ch.nq * ch.S * W::SPLIT;
   for (int64_t it = threadIdx.x; it < total; it += blockDim.x) {
-    const int kc = (int)(it % W::SPLIT);
-    const int64_t rest = it / W::SPLIT;
-    const int s0 = (int)(rest / ch.nq);
-    const int64_t g = ch.q0 + (rest - (int64_t)s0 * ch.nq);
+    const int s0 = (int)(it & (ch.S - 1));
+    const int64_t rest = it / ch.S;
+    const int kc = (int)(rest % W::SPLIT);
+    const int64_t g = ch.q0 + rest / W::SPLIT;
     const int2 qk = a.quartets[g];
     const int slot = qk.x - ch.bra0;
     const Pair bp = a.pairs[qk.x];
