@@ -38,7 +38,7 @@ SEARCH = {
     "threads": (256, 512),
     "chunk_items": (2048, 4096, 8192),
     "bras_max": (8, 16, 32),
-    "target_prim": (24, 48, 96),
+    "target_prim": (12, 24, 48, 96),
 }
 
 
