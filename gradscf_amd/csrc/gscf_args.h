@@ -128,6 +128,9 @@ struct BraArgs {
   int tab_lds;           // stage the class's Rys table slice in LDS (after the slabs)
   int nket_max;          // LDS ket-J window capacity in ket pairs (0: ket-side J by global atomics)
   int threads;           // workgroup size (256 or 512)
+  // ket-side J with one 64-byte line per element (element i of jp at
+  // jket_pad[8 i]), so atomics to neighbouring J elements do not share a line
+  double* jket_pad;
 };
 
 struct ParamArgs {

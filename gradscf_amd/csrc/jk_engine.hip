@@ -234,6 +234,8 @@ struct PlanData {
   // workspace
   double* d_dc = nullptr;
   double* d_jp = nullptr;
+  double* d_jpad = nullptr;     // ket-side J, one 64-byte line per element
+
   double* d_kp = nullptr;
   size_t ws_nbd = 0;
   std::vector<double*> retired;  // outgrown workspaces, freed with the plan
@@ -437,7 +439,8 @@ __global__ void dens_to_cart_kernel(const double* __restrict__ D, int64_t nn_tot
 }
 
 // out[op][b][p][q]: J = T (Jp + Jp^T) T^T, K = T (Kp_b + Kp_b'^T) T^T, per molecule
-__global__ void finalize_kernel(const double* __restrict__ jp, const double* __restrict__ kp, int64_t ncc_tot,
+__global__ void finalize_kernel(const double* __restrict__ jp, const double* __restrict__ jpad,
+                                const double* __restrict__ kp, int64_t ncc_tot,
                                 int64_t nn_tot, int nb, int sym, int with_j, int with_k,
                                 const MolAO* __restrict__ M, int nmol, const int* __restrict__ idx,
                                 const double* __restrict__ coef, double* __restrict__ out) {
@@ -454,6 +457,8 @@ __global__ void finalize_kernel(const double* __restrict__ jp, const double* __r
   const bool isj = with_j && op == 0;
   const double* A = (isj ? jp + (size_t)b * ncc_tot : kp + (size_t)b * ncc_tot) + m.cbase;
   const double* B = isj ? A : kp + (size_t)(sym ? b : nb + b) * ncc_tot + m.cbase;
+  // ket-side J partials, one element per 64-byte line
+  const double* X = isj ? jpad + 8 * ((size_t)b * ncc_tot + m.cbase) : nullptr;
   const int nc = m.nc, gp = m.ao0 + p, gq = m.ao0 + q;
   double s = 0.0;
   for (int e1 = 0; e1 < 3; ++e1) {
@@ -464,7 +469,9 @@ __global__ void finalize_kernel(const double* __restrict__ jp, const double* __r
       const double c2 = coef[gq * 3 + e2];
       if (c2 == 0.0) continue;
       const int Q = idx[gq * 3 + e2];
-      s += c1 * c2 * (A[(size_t)P * nc + Q] + B[(size_t)Q * nc + P]);
+      double v = A[(size_t)P * nc + Q] + B[(size_t)Q * nc + P];
+      if (X) v += X[8 * ((size_t)P * nc + Q)] + X[8 * ((size_t)Q * nc + P)];
+      s += c1 * c2 * v;
     }
   }
   out[i] = s;
@@ -1469,6 +1476,7 @@ static int64_t plan_bytes(const PlanData& P, int64_t npairs, int64_t nq, int64_t
   b += (int64_t)5 * sizeof(double) * std::max(P.natm, 1);                     // nuclei
   b += nq * sizeof(int2) + ntasks * sizeof(SubTask) + nchunks * sizeof(BraChunk);
   b += 3 * P.ncc_tot * (int64_t)sizeof(double) * nbd;                         // Cartesian D, Jp, Kp
+  b += 8 * P.ncc_tot * (int64_t)sizeof(double) * nbd;                         // ket-side J, one line per element
   return b;
 }
 
@@ -1530,7 +1538,7 @@ static void free_plan(PlanData& P) {
   free_pair_set(P.seg_set);
   f(P.d_shells); f(P.d_prims); f(P.d_all_pairs);
   f(P.d_c2s_idx); f(P.d_c2s_coef); f(P.d_s2c_idx); f(P.d_s2c_coef);
-  f(P.d_dc); f(P.d_jp); f(P.d_kp); f(P.d_flag);
+  f(P.d_dc); f(P.d_jp); f(P.d_jpad); f(P.d_kp); f(P.d_flag);
   f(P.d_pairs1e); f(P.d_nuclei);
   f(P.d_dmax); f(P.d_seg_atom);
   f(P.d_molinfo); f(P.d_molao);
@@ -1548,13 +1556,14 @@ static void free_plan(PlanData& P) {
 // they are retired and released with the plan.
 static void ensure_workspace(PlanData& P, size_t nbd) {
   if (P.ws_nbd >= nbd) return;
-  for (double** p : {&P.d_dc, &P.d_jp, &P.d_kp}) {
+  for (double** p : {&P.d_dc, &P.d_jp, &P.d_jpad, &P.d_kp}) {
     if (*p) P.retired.push_back(*p);
     *p = nullptr;
   }
   const size_t nn = (size_t)P.ncc_tot;
   HIP_OK(hipMalloc(&P.d_dc, sizeof(double) * nn * nbd));
   HIP_OK(hipMalloc(&P.d_jp, sizeof(double) * nn * nbd));
+  HIP_OK(hipMalloc(&P.d_jpad, sizeof(double) * 8 * nn * nbd));
   HIP_OK(hipMalloc(&P.d_kp, sizeof(double) * nn * nbd));
   P.ws_nbd = nbd;
 }
@@ -1585,6 +1594,8 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     hipLaunchKernelGGL(dmax_kernel, dim3(nblocks((size_t)P.nd_tot)), dim3(256), 0, s, P.d_dc, P.ncc_tot, nbd,
                        P.d_molao, P.nmol, P.nd_tot, P.jk.d_blk, P.d_dmax);
   hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_jp, ncc * nbd, 0.0);
+  if (with_j)
+    hipLaunchKernelGGL(fill_kernel, dim3(nblocks(8 * ncc * nb)), dim3(256), 0, s, P.d_jpad, 8 * ncc * nb, 0.0);
   hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_kp, ncc * nbd, 0.0);
   const PairSet& S = P.jk;
   if (P.prof) ++P.prof_builds;
@@ -1654,6 +1665,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     if (qc.d_chunks && lds <= kLdsBudget) {
       b.dens = P.d_dc;
       b.jp = P.d_jp;
+      b.jket_pad = P.d_jpad;
       b.kp = P.d_kp;
       b.bstride = P.ncc_tot;
       b.nbd = nbd_eff;
@@ -1690,6 +1702,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       for (int bt = 0; bt < nbd_eff; ++bt) {
         b.dens = P.d_dc + (size_t)bt * ncc;
         b.jp = P.d_jp + (size_t)bt * ncc;
+        b.jket_pad = P.d_jpad + 8 * (size_t)bt * ncc;
         b.kp = P.d_kp + (size_t)bt * ncc;
         b.bstride = P.ncc_tot;
         b.nbd = 1;
@@ -1713,8 +1726,8 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     }
   }
   hipLaunchKernelGGL(finalize_kernel, dim3(nblocks((size_t)(with_j + with_k) * nb * nn)), dim3(256), 0, s,
-                     P.d_jp, P.d_kp, P.ncc_tot, P.nn_tot, nb, sym ? 1 : 0, with_j, with_k, P.d_molao, P.nmol,
-                     P.d_s2c_idx, P.d_s2c_coef, out);
+                     P.d_jp, P.d_jpad, P.d_kp, P.ncc_tot, P.nn_tot, nb, sym ? 1 : 0, with_j, with_k, P.d_molao,
+                     P.nmol, P.d_s2c_idx, P.d_s2c_coef, out);
   HIP_OK(hipGetLastError());
 }
 

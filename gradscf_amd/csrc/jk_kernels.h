@@ -574,7 +574,6 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
       if (a.with_j && bt < a.nbj) {
-        double* J = a.jp + (size_t)bt * a.bstride + mi.base;
         double* jr = jrow + (bt * a.nbra_max + slot) * W::NBRA;
         double dcd[WC][WD], dab[WA][WB];
 #pragma unroll
@@ -610,7 +609,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             if (jket)
               atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * W::NKET + ic * WD + id, f * sj);
             else
-              atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * sj);
+              atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base + (size_t)(oc + ic) * n + od + id),
+                        f * sj);
           }
       }
       if (a.with_k) {
@@ -730,7 +730,6 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
       if (a.with_j && bt < a.nbj) {
-        double* J = a.jp + (size_t)bt * a.bstride + mi.base;
         double* jr = jrow + (bt * a.nbra_max + slot) * W::NBRA;
         const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
         double s2 = 0.0;
@@ -744,7 +743,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
         if (jket)
           atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * W::NKET + kc, f * s2);
         else
-          atomicAdd(J + (size_t)c * n + d, f * s2);
+          atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base + (size_t)c * n + d), f * s2);
       }
       if (a.with_k) {
         double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
@@ -832,13 +831,22 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   // the S slices of one quartet component occupy S adjacent lanes of one
   // wavefront (S is a power of two <= 64 and blockDim a multiple of 64), which
   // add their partial integrals by cross-lane exchange before the s0 = 0 lane
-  // digests them alone
+  // digests them alone.
+  //
+  // The chunk's quartets are visited from a workgroup-dependent starting
+  // point: chunks of neighbouring bras share their kets, and workgroups that
+  // walk the kets in the same order make their ket-side J atomics collide on
+  // the same addresses at the same time (the memory-side atomic unit then
+  // serialises them).
   const int64_t total = (int64_t)ch.nq * ch.S * W::SPLIT;
+  const int64_t rot = ((int64_t)blockIdx.x * 40503) % ch.nq;
   for (int64_t it = threadIdx.x; it < total; it += blockDim.x) {
     const int s0 = (int)(it & (ch.S - 1));
     const int64_t rest = it / ch.S;
     const int kc = (int)(rest % W::SPLIT);
-    const int64_t g = ch.q0 + rest / W::SPLIT;
+    int64_t qr = rest / W::SPLIT + rot;
+    if (qr >= ch.nq) qr -= ch.nq;
+    const int64_t g = ch.q0 + qr;
     const int2 qk = a.quartets[g];
     const int slot = qk.x - ch.bra0;
     const Pair bp = a.pairs[qk.x];
@@ -905,7 +913,7 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
       const Pair kp = a.pairs[ch.ket0 + w];
       const MolInfo mi = a.mols[a.shells[kp.sa].mol];
       const int row = a.shells[kp.sa].ao + e / W::WD, col = a.shells[kp.sb].ao + e % W::WD;
-      atomicAdd(a.jp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
+      atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col), v);
     }
   }
 }
