@@ -233,7 +233,9 @@ def test_workspace_forecast_without_gpu():
     four = native_jk_workspace_bytes(prov, 1, 1e-12, nb=4)
     n = topo.nao()
     assert one > 3 * 2 * n * n * 8
-    assert four - one == 3 * 2 * 3 * n * n * 8  # six more n x n FP64 workspace matrices per extra density
+    # per extra density two more workspace slots (D and D^T for nonsymmetric K),
+    # each with Cartesian D, Jp, Kp and the ket-side J padded to one line per element
+    assert four - one == 3 * 2 * (3 + 8) * n * n * 8
     assert native_jk_workspace_bytes(object(), 1, 0.0) is None
     with pytest.raises(ValueError):
         native_jk_workspace_bytes(prov, 0, 0.0)
