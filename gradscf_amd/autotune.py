@@ -32,13 +32,14 @@ class Schedule:
     ket_lds: int = 0
 
 
+# Coordinate descent, one entry at a time; a tuple key is searched jointly
+# (chunk size and primitive split interact: both set the items per chunk).
 SEARCH = {
     "rys_lds": (0, 1),
     "ket_lds": (0, 1),
     "threads": (256, 512),
-    "chunk_items": (2048, 4096, 8192),
+    ("chunk_items", "target_prim"): tuple((c, t) for c in (2048, 4096, 8192) for t in (12, 24, 48, 96)),
     "bras_max": (8, 16, 32),
-    "target_prim": (12, 24, 48, 96),
 }
 
 
@@ -144,18 +145,20 @@ def autotune(make_plan, run, *, reps: int = 3, search=None, log=None) -> dict:
     default_t = dict(best_t)
     best_s = {cls: base for cls in best_t}
     for param, values in search.items():
+        names = param if isinstance(param, tuple) else (param,)
         current = {cls: best_s[cls] for cls in best_t}
         for v in values:
-            if all(getattr(current[c], param) == v for c in current):
+            upd = dict(zip(names, v if isinstance(param, tuple) else (v,)))
+            if all(all(getattr(current[c], k) == x for k, x in upd.items()) for c in current):
                 continue
-            # every class tries value v of this parameter on top of its current best
+            # every class tries this value on top of its current best
             for cls in current:
-                set_schedule(Schedule(**{**asdict(current[cls]), param: v}), cls)
+                set_schedule(Schedule(**{**asdict(current[cls]), **upd}), cls)
             t = _class_times(make_plan, run, reps)
             for cls, ms in t.items():
                 if cls in best_t and ms < best_t[cls]:
                     best_t[cls] = ms
-                    best_s[cls] = Schedule(**{**asdict(current[cls]), param: v})
+                    best_s[cls] = Schedule(**{**asdict(current[cls]), **upd})
             if log:
                 log(f"autotune {param}={v}: {sum(t.values()):.3f} ms over classes")
         for cls in best_s:
