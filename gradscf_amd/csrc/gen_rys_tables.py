@@ -9,6 +9,12 @@ accuracy as Clenshaw's recurrence since the coefficients decay); above XMAX the
 large-x limit (positive Gauss-Hermite nodes r_i of order 2n: s_i = r_i^2/x,
 w_i = W_i/sqrt(x)) is exact to double precision.
 
+For one and two roots the kernels instead use Boys functions F_0..F_3 from
+a K-term Taylor expansion about the nearest point of a grid T_i = i H_B
+(dF_m/dT = -F_{m+1}, so F_m(T_i - d) = sum_k F_{m+k}(T_i) d^k / k!, |d| <=
+H_B/2) and the closed-form one- and two-point quadratures of those moments;
+this module also tabulates F_0..F_{NB-1} on that grid (BOYS_* below).
+
 Reference values are computed in 60-digit arithmetic (mpmath): Boys moments
 F_k(x) = 1F1(k+1/2; k+3/2; -x)/(2k+1), the Chebyshev algorithm for the
 recurrence coefficients of the measure, and the eigen-decomposition of the
@@ -30,6 +36,12 @@ XMAX = 80.0
 H = 2.0
 DEG = 13
 DPS = 60
+# Boys grid: exact binary spacing, Taylor terms, functions per grid point
+# (F_0..F_3 need F_0..F_{3+K-1}; padded to a 16-byte multiple)
+BOYS_H = 0.0625
+BOYS_TMAX = 48.0
+BOYS_K = 8
+BOYS_NB = 12
 
 
 def _rys_mp(n: int, x: float):
@@ -99,6 +111,14 @@ def _interval_coeffs(args):
     return n, k, coeffs
 
 
+def _boys_row(i: int):
+    """F_0..F_{NB-1} at T_i = i * BOYS_H (the double the device forms)."""
+    import mpmath as mp
+    mp.mp.dps = DPS
+    t = mp.mpf(float(i * BOYS_H))
+    return i, [float(mp.hyp1f1(m + mp.mpf(1) / 2, m + mp.mpf(3) / 2, -t) / (2 * m + 1)) for m in range(BOYS_NB)]
+
+
 def _hermite(n: int):
     import numpy as np
     from numpy.polynomial.hermite import hermgauss
@@ -109,11 +129,13 @@ def _hermite(n: int):
     return [float(v) for v in r[o] ** 2], [float(v) for v in W[o]]
 
 
-def generate() -> str:
+def generate():
     nint = int(XMAX / H)
     jobs = [(n, k) for n in range(1, NMAX + 1) for k in range(nint)]
+    npt = int(BOYS_TMAX / BOYS_H) + 1
     with ProcessPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         results = list(ex.map(_interval_coeffs, jobs, chunksize=4))
+        boys = dict(ex.map(_boys_row, range(npt), chunksize=16))
     table = {(n, k): c for n, k, c in results}
     lines = ["// Generated by gen_rys_tables.py -- do not edit.",
              f"#define RYS_NMAX {NMAX}", f"#define RYS_XMAX {XMAX!r}", f"#define RYS_H {H!r}",
@@ -138,7 +160,11 @@ def generate() -> str:
         hoff.append(len(hr))
     lines.append("constexpr double rys_asym_root_c[%d] = {%s};" % (len(hr), ", ".join(v.hex() for v in hr)))
     lines.append("constexpr double rys_asym_weight_c[%d] = {%s};" % (len(hw), ", ".join(v.hex() for v in hw)))
-    return "\n".join(lines) + "\n", "\n".join(body) + "\n"
+    lines += [f"#define BOYS_H {BOYS_H!r}", f"#define BOYS_TMAX {BOYS_TMAX!r}", f"#define BOYS_K {BOYS_K}",
+              f"#define BOYS_NB {BOYS_NB}", f"#define BOYS_NPT {npt}"]
+    bflat = [v for i in range(npt) for v in boys[i]]
+    bbody = [", ".join(float(v).hex() for v in bflat[i:i + 4]) + "," for i in range(0, len(bflat), 4)]
+    return "\n".join(lines) + "\n", "\n".join(body) + "\n", "\n".join(bbody) + "\n"
 
 
 def source_hash() -> str:
@@ -149,11 +175,12 @@ def ensure(out: Path) -> Path:
     """(Re)generate `out` (+ the table body beside it) unless it carries this generator's hash."""
     tag = f"// source-hash {source_hash()}\n"
     body_path = out.with_name("rys_table_body.inc")
-    if out.is_file() and body_path.is_file() and out.read_text().startswith(tag):
+    boys_path = out.with_name("boys_table_body.inc")
+    if out.is_file() and body_path.is_file() and boys_path.is_file() and out.read_text().startswith(tag):
         return out
     out.parent.mkdir(parents=True, exist_ok=True)
-    head, body = generate()
-    for path, text in ((body_path, body), (out, tag + head)):
+    head, body, bbody = generate()
+    for path, text in ((body_path, body), (boys_path, bbody), (out, tag + head)):
         tmp = path.with_suffix(".tmp")
         tmp.write_text(text)
         tmp.replace(path)

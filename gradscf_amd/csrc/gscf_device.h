@@ -22,6 +22,11 @@ __device__ static const double rys_tab[] = {
 #include "_generated/rys_table_body.inc"
 };
 
+// F_0..F_{BOYS_NB-1} at T_i = i BOYS_H, i < BOYS_NPT (BOYS_NB per point)
+__device__ static const double boys_tab[] = {
+#include "_generated/boys_table_body.inc"
+};
+
 // ---------------------------------------------------------------------------
 // Rys roots (s = t^2) and weights for weight exp(-x t^2) on [0,1]
 // ---------------------------------------------------------------------------
@@ -51,38 +56,106 @@ __device__ __forceinline__ double rsqrt_pos(double x) {
   return fma(y, t, y);
 }
 
+// Boys functions F_0..F_{M-1}(x), 0 <= x < BOYS_TMAX, by a BOYS_K-term Taylor
+// expansion about the nearest grid point T_i = i BOYS_H (dF_m/dT = -F_{m+1}):
+//   F_m(T_i - d) = sum_{k < K} F_{m+k}(T_i) d^k / k!,   |d| <= BOYS_H / 2,
+// nested as f_m + (d/1)(f_{m+1} + (d/2)(f_{m+2} + ...)).  Truncation below
+// 2e-17 relative; every function shares the d/k factors and one grid row.
+template <int M>
+__device__ __forceinline__ void boys_taylor(double x, double (&F)[M]) {
+  static_assert(M + BOYS_K - 1 <= BOYS_NB, "Boys grid row too short");
+  const int i = (int)(x * (1.0 / BOYS_H) + 0.5);
+  const double d = i * BOYS_H - x;
+  const double* __restrict__ f = boys_tab + i * BOYS_NB;
+  double dk[BOYS_K];
+#pragma unroll
+  for (int k = 1; k < BOYS_K; ++k) dk[k] = d * (1.0 / k);
+#pragma unroll
+  for (int m = 0; m < M; ++m) {
+    double r = f[m + BOYS_K - 1];
+#pragma unroll
+    for (int k = BOYS_K - 1; k >= 1; --k) r = fma(r, dk[k], f[m + k - 1]);
+    F[m] = r;
+  }
+}
+
+// Rys roots s_i = t_i^2 and weights of one or two points from the Boys
+// moments mu_k = F_k(x) = sum_i w_i s_i^k (exact for k < 2N):
+//   N = 1:  w = F_0, s = F_1 / F_0;
+//   N = 2:  s_1 < s_2 are the zeros of s^2 + a1 s + a0 with
+//           [mu0 mu1; mu1 mu2] (a0, a1) = -(mu2, mu3), the larger root from
+//           the quadratic formula without cancellation (-a1 > 0), the smaller
+//           as a0 / s_2; w_1 = (s_2 mu0 - mu1) / (s_2 - s_1), w_2 = mu0 - w_1.
+// Relative error below 4e-15 for x < BOYS_TMAX (tested against 60-digit values).
+template <int N>
+__device__ __forceinline__ void rys_roots_boys(double x, double* __restrict__ s, double* __restrict__ w) {
+  static_assert(N == 1 || N == 2, "closed-form quadrature for one or two roots");
+  if constexpr (N == 1) {
+    double F[2];
+    boys_taylor<2>(x, F);
+    w[0] = F[0];
+    s[0] = F[1] * rcp_pos(F[0]);
+  } else {
+    double F[4];
+    boys_taylor<4>(x, F);
+    const double iD = rcp_pos(F[0] * F[2] - F[1] * F[1]);
+    const double a1 = (F[1] * F[2] - F[0] * F[3]) * iD;
+    const double a0 = (F[1] * F[3] - F[2] * F[2]) * iD;
+    const double disc = a1 * a1 - 4.0 * a0;
+    const double s2 = 0.5 * (disc * rsqrt_pos(disc) - a1);
+    const double s1 = a0 * rcp_pos(s2);
+    const double w1 = (s2 * F[0] - F[1]) * rcp_pos(s2 - s1);
+    s[0] = s1;
+    s[1] = s2;
+    w[0] = w1;
+    w[1] = F[0] - w1;
+  }
+}
+
 // `tab` is the N-root slice of the table (rys_tab + rys_table_offset[N - 1],
 // or a copy of it staged in LDS): RYS_NINT intervals of (RYS_DEG + 1) x 2N
-// monomial coefficients in y (degree-major), roots then weights.
-template <int N>
+// monomial coefficients in y (degree-major), roots then weights.  With
+// MOMENTS (one or two roots) the quadrature comes from the Boys moments and
+// the table is not read: fewer memory reads, more arithmetic; which one wins
+// depends on where the table lives (LDS or global) and is chosen per class.
+template <int N, bool MOMENTS = (N <= 2)>
 __device__ __forceinline__ void rys_roots_tab(const double* __restrict__ tab, double x, double* __restrict__ s,
                                               double* __restrict__ w) {
-  if (x < RYS_XMAX) {
-    const int k = (int)(x * (1.0 / RYS_H));
-    const double y = x - (RYS_H * k + 0.5 * RYS_H);
-    const double* __restrict__ c = tab + k * ((RYS_DEG + 1) * 2 * N);
-    // Horner in y on [-1, 1): one FMA per degree and function
-    double acc[2 * N];
-#pragma unroll
-    for (int f = 0; f < 2 * N; ++f) acc[f] = c[RYS_DEG * 2 * N + f];
-#pragma unroll
-    for (int j = RYS_DEG - 1; j >= 0; --j)
-#pragma unroll
-      for (int f = 0; f < 2 * N; ++f) acc[f] = fma(acc[f], y, c[j * 2 * N + f]);
-#pragma unroll
-    for (int f = 0; f < N; ++f) {
-      s[f] = acc[f];
-      w[f] = acc[N + f];
+  static_assert(!MOMENTS || N <= 2, "moment quadrature for one or two roots");
+  if constexpr (MOMENTS) {
+    if (x < BOYS_TMAX) {
+      rys_roots_boys<N>(x, s, w);
+      return;
     }
   } else {
-    const double sq = rsqrt_pos(x);
-    const double ix = sq * sq;
-    constexpr int base = N * (N - 1) / 2;
+    if (x < RYS_XMAX) {
+      const int k = (int)(x * (1.0 / RYS_H));
+      const double y = x - (RYS_H * k + 0.5 * RYS_H);
+      const double* __restrict__ c = tab + k * ((RYS_DEG + 1) * 2 * N);
+      // Horner in y on [-1, 1): one FMA per degree and function
+      double acc[2 * N];
 #pragma unroll
-    for (int f = 0; f < N; ++f) {
-      s[f] = rys_asym_root_c[base + f] * ix;
-      w[f] = rys_asym_weight_c[base + f] * sq;
+      for (int f = 0; f < 2 * N; ++f) acc[f] = c[RYS_DEG * 2 * N + f];
+#pragma unroll
+      for (int j = RYS_DEG - 1; j >= 0; --j)
+#pragma unroll
+        for (int f = 0; f < 2 * N; ++f) acc[f] = fma(acc[f], y, c[j * 2 * N + f]);
+#pragma unroll
+      for (int f = 0; f < N; ++f) {
+        s[f] = acc[f];
+        w[f] = acc[N + f];
+      }
+      return;
     }
+  }
+  // large x: Gauss-Hermite limit, exact to double precision above the switch
+  const double sq = rsqrt_pos(x);
+  const double ix = sq * sq;
+  constexpr int base = N * (N - 1) / 2;
+#pragma unroll
+  for (int f = 0; f < N; ++f) {
+    s[f] = rys_asym_root_c[base + f] * ix;
+    w[f] = rys_asym_weight_c[base + f] * sq;
   }
 }
 

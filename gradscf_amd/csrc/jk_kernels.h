@@ -56,7 +56,7 @@ struct PrimQ {
   double PA[3], QC[3], PQ[3];
 };
 
-template <int NR>
+template <int NR, bool MOMENTS = (NR <= 2)>
 __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q, const QuartetGeom& g,
                                            PrimQ<NR>& o, const double* __restrict__ tab) {
   const double pq = P.p + Q.p;
@@ -78,7 +78,7 @@ __device__ __forceinline__ void prim_setup(const PrimPair& P, const PrimPair& Q,
   const double rho = P.p * Q.p * o.ipq;
   const double T = rho * (o.PQ[0] * o.PQ[0] + o.PQ[1] * o.PQ[1] + o.PQ[2] * o.PQ[2]);
   o.pref = P.e * Q.e * rs;
-  rys_roots_tab<NR>(tab, T, o.s, o.w);
+  rys_roots_tab<NR, MOMENTS>(tab, T, o.s, o.w);
 }
 
 template <int NR>
@@ -487,7 +487,7 @@ struct Wide {
 // window or to global).  With general contraction each primitive quartet is
 // evaluated once, coefficient-free, and added to every column combination
 // with its coefficient product c_A c_B c_C c_D.
-template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, typename R>
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool MOMENTS, typename R>
 __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const Pair& kp, const int2 qk, int slot,
                                          int s0, int S, int kc, double* __restrict__ krow,
                                          double* __restrict__ jrow, double* __restrict__ jket, int kwin0,
@@ -529,7 +529,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
       for (int kq = 0; kq < kp.np; ++kq) {
         const PrimPair Q = kpr[kq];
         PrimQ<K::NR> q;
-        prim_setup<K::NR>(P, Q, geo, q, tab);
+        prim_setup<K::NR, MOMENTS>(P, Q, geo, q, tab);
         if constexpr (!W::GEN) {
           accumulate_full<LA, LB, LC, LD, R>(q, AB, CD_, acc);
         } else {
@@ -698,7 +698,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
       for (int kq = 0; kq < kp.np; ++kq) {
         const PrimPair Q = kpr[kq];
         PrimQ<K::NR> q;
-        prim_setup<K::NR>(P, Q, geo, q, tab);
+        prim_setup<K::NR, MOMENTS>(P, Q, geo, q, tab);
         if constexpr (!W::GEN) {
           accumulate_split<LA, LB, LC, LD, R>(q, AB, wk, acc);
         } else {
@@ -821,6 +821,9 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   const int zero_size = krow_size + jrow_size + (jket ? jket_size : 0);
   for (int i = threadIdx.x; i < zero_size; i += blockDim.x) lds[i] = 0.0;
   const double* tab = rys_tab + rys_table_offset[K::NR - 1];
+  // one or two roots: the table staged in LDS, or the Boys-moment quadrature
+  // (global Boys grid) when the table stays in global memory
+  constexpr bool kMoments = K::NR <= 2 && !TLDS;
   if constexpr (TLDS) {
     double* ltab = lds + ((krow_size + jrow_size + jket_size + 1) & ~1);  // 16-byte aligned
     for (int i = threadIdx.x; i < rys_slice_size(K::NR); i += blockDim.x) ltab[i] = tab[i];
@@ -869,11 +872,13 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     }
     if constexpr (MIX) {
       if (lowp) {
-        bra_item<LA, LB, LC, LD, CA, CB, CC, CD, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
+        bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket,
+                                                                 ch.ket0, tab);
         continue;
       }
     }
-    bra_item<LA, LB, LC, LD, CA, CB, CC, CD, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket, ch.ket0, tab);
+    bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket,
+                                                                ch.ket0, tab);
   }
   __syncthreads();
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)

@@ -45,7 +45,8 @@ typedef struct {
   int32_t chunk_items;  /* lane-items per workgroup before a bra chunk is closed (64..2^20) */
   int32_t bras_max;     /* bra pairs (LDS exchange slots) per chunk (1..256) */
   int32_t lds_pack;     /* LDS bytes the slots of one workgroup may use (1024..65536) */
-  int32_t rys_lds;      /* 1: stage the class's Rys table slice in LDS */
+  int32_t rys_lds;      /* 1: stage the class's Rys table slice in LDS; 0: global table
+                           (one or two roots: Boys-moment quadrature instead) */
   int32_t target_prim;  /* primitive quartets per work item (primitive split), 1..4096 */
   int32_t threads;      /* workgroup size, 256 or 512 */
   int32_t ket_lds;      /* 1: accumulate the ket-side J of each chunk's ket window in LDS */

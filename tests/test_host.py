@@ -132,6 +132,67 @@ def test_rys_table_against_mpmath():
             assert np.max(np.abs(f[n:] - w) / np.array(w)) < 5e-15
 
 
+def test_boys_moment_quadrature_against_mpmath():
+    """One- and two-point Rys quadrature from Taylor-interpolated Boys moments
+    (the device's rys_roots_boys, restated operation by operation) against
+    60-digit roots and weights, across the grid and at the switch to the
+    large-x limit."""
+    pytest.importorskip("mpmath")
+    import importlib.util
+    path = ROOT / "gradscf_amd" / "csrc" / "gen_rys_tables.py"
+    spec = importlib.util.spec_from_file_location("gen", path)
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    body = ROOT / "gradscf_amd/csrc/_generated/boys_table_body.inc"
+    head = ROOT / "gradscf_amd/csrc/_generated/rys_table.inc"
+    if not body.is_file():
+        pytest.skip("engine not built")
+    tab = np.array([float.fromhex(v.strip()) for v in body.read_text().replace("\n", "").split(",") if v.strip()])
+    txt = head.read_text()
+    asym_r = [float.fromhex(v) for v in re.search(r"rys_asym_root_c\[28\] = \{([^}]*)\}", txt).group(1).split(",")]
+    asym_w = [float.fromhex(v) for v in re.search(r"rys_asym_weight_c\[28\] = \{([^}]*)\}", txt).group(1).split(",")]
+    H, K, NB, TMAX = gen.BOYS_H, gen.BOYS_K, gen.BOYS_NB, gen.BOYS_TMAX
+    assert tab.size == (int(TMAX / H) + 1) * NB
+
+    def boys(x, m_count):
+        i = int(x * (1.0 / H) + 0.5)
+        d = i * H - x
+        f = tab[i * NB:(i + 1) * NB]
+        dk = [0.0] + [d * (1.0 / k) for k in range(1, K)]
+        out = []
+        for m in range(m_count):
+            r = f[m + K - 1]
+            for k in range(K - 1, 0, -1):
+                r = r * dk[k] + f[m + k - 1]
+            out.append(r)
+        return out
+
+    def quad(n, x):
+        if x >= TMAX:
+            b = n * (n - 1) // 2
+            return [asym_r[b + f] / x for f in range(n)], [asym_w[b + f] / np.sqrt(x) for f in range(n)]
+        if n == 1:
+            f0, f1 = boys(x, 2)
+            return [f1 / f0], [f0]
+        m0, m1, m2, m3 = boys(x, 4)
+        i_d = 1.0 / (m0 * m2 - m1 * m1)
+        a1 = (m1 * m2 - m0 * m3) * i_d
+        a0 = (m1 * m3 - m2 * m2) * i_d
+        s2 = 0.5 * (np.sqrt(a1 * a1 - 4.0 * a0) - a1)
+        s1 = a0 / s2
+        w1 = (s2 * m0 - m1) / (s2 - s1)
+        return [s1, s2], [w1, m0 - w1]
+
+    rng = np.random.default_rng(7)
+    xs = list(rng.uniform(0, TMAX, 40)) + [0.0, 1e-13, H / 2 - 1e-12, H / 2 + 1e-12, 25.0, TMAX - 1e-9, TMAX, 60.0]
+    for n in (1, 2):
+        for x in xs:
+            s, w = quad(n, float(x))
+            r, ww = gen._rys_mp(n, float(x))
+            assert max(abs(a - b) / b for a, b in zip(s, r)) < 5e-15, (n, x)
+            assert max(abs(a - b) / b for a, b in zip(w, ww)) < 5e-15, (n, x)
+
+
 def test_library_exports_every_declared_symbol():
     header = (ROOT / "include" / "gradscf_jk.h").read_text()
     declared = set(re.findall(r"\b(gscf_[a-z0-9_]+)\s*\(", header))
