@@ -131,11 +131,13 @@ def _class_times(make_plan, run, reps):
     return times
 
 
-def autotune(make_plan, run, *, reps: int = 3, search=None, log=None) -> dict:
+def autotune(make_plan, run, *, reps: int = 3, search=None, passes: int = 2, log=None) -> dict:
     """Choose a schedule per class.
 
     make_plan(): builds a DirectJKPlan under the currently installed tuning.
     run(plan): one J/K build of the workload on that plan.
+    passes: sweeps of the coordinate descent (the second revisits early
+    choices, e.g. the root evaluation, under the later ones).
     Returns {class name: {"schedule": dict, "ms": best avg ms, "default_ms": ms}}
     and leaves the chosen schedules installed."""
     search = SEARCH if search is None else search
@@ -144,7 +146,7 @@ def autotune(make_plan, run, *, reps: int = 3, search=None, log=None) -> dict:
     best_t = _class_times(make_plan, run, reps)
     default_t = dict(best_t)
     best_s = {cls: base for cls in best_t}
-    for param, values in search.items():
+    for param, values in [kv for _ in range(passes) for kv in search.items()]:
         names = param if isinstance(param, tuple) else (param,)
         current = {cls: best_s[cls] for cls in best_t}
         for v in values:
