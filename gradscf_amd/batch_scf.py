@@ -27,6 +27,7 @@ import numpy as np
 import torch
 
 from .jk import DirectJKPlan
+from .linalg import eigh
 
 DIIS_SPACE = 8
 
@@ -58,14 +59,14 @@ class _Group:
 
 
 def _orthogonalizer(s, eps=1e-10):
-    w, v = torch.linalg.eigh(s)
+    w, v = eigh(s)
     return (v * torch.clamp(w, min=eps).rsqrt()[:, None, :]) @ v.transpose(1, 2)
 
 
 def _diagonalize(f, x):
     fo = x.transpose(1, 2) @ f @ x
     fo = 0.5 * (fo + fo.transpose(1, 2))
-    e, c = torch.linalg.eigh(fo)
+    e, c = eigh(fo)
     return e, x @ c
 
 

@@ -24,6 +24,7 @@ import torch
 
 from .basis import prepare_basis
 from .jk import DirectJKPlan
+from .linalg import eigh
 from .molecule import MoleculeSpec, parse_molecule_spec
 from .packing import pack
 
@@ -46,14 +47,14 @@ class RHFResult:
 
 
 def orthogonalizer(s: torch.Tensor, eps: float = 1e-10) -> torch.Tensor:
-    w, v = torch.linalg.eigh(s)
+    w, v = eigh(s)
     return (v * torch.clamp(w, min=eps).rsqrt()) @ v.T
 
 
 def _diagonalize(f, x):
     fo = x.T @ f @ x
     fo = 0.5 * (fo + fo.T)
-    e, c = torch.linalg.eigh(fo)
+    e, c = eigh(fo)
     return e, x @ c
 
 
@@ -130,9 +131,12 @@ def rhf(plan: DirectJKPlan, nelectron: int, enuc: float, *, conv_tol=1e-10, conv
         n_jk += 1
         j, k = j + dj, k + dk
         fock_new = h + j - 0.5 * k
-        e_new = float(torch.sum(d_new * h) + 0.5 * torch.sum(d_new * j) - 0.25 * torch.sum(d_new * k)) + enuc
-        rms = float(torch.sqrt(torch.mean((d_new - d) ** 2)))
-        grad = float(torch.linalg.norm(2.0 * c[:, nocc:].T @ fock_new @ c[:, :nocc]))
+        # energy, density change and orbital gradient in one host transfer
+        e_el, rms, grad = torch.stack([
+            torch.sum(d_new * h) + 0.5 * torch.sum(d_new * j) - 0.25 * torch.sum(d_new * k),
+            torch.sqrt(torch.mean((d_new - d) ** 2)),
+            torch.linalg.norm(2.0 * c[:, nocc:].T @ fock_new @ c[:, :nocc])]).tolist()
+        e_new = e_el + enuc
         done = abs(e_new - energy) < conv_tol and rms < conv_tol_density and grad < conv_tol_grad and cycle > 0
         d, fock, energy = d_new, fock_new, e_new
         cycles = cycle + 1

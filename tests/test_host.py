@@ -193,6 +193,20 @@ def test_boys_moment_quadrature_against_mpmath():
             assert max(abs(a - b) / b for a, b in zip(w, ww)) < 5e-15, (n, x)
 
 
+def test_eigh_placement_and_host_path():
+    """Small eigen-problems go to host LAPACK, large ones stay on the GPU; the
+    host path returns the same decomposition as torch's."""
+    import torch
+    from gradscf_amd import linalg
+    assert linalg.on_host((13, 13)) and linalg.on_host((384, 384)) and not linalg.on_host((608, 608))
+    assert linalg.on_host((32, 57, 57)) and not linalg.on_host((32, 76, 76))
+    assert linalg.on_host((1, 384, 384))
+    a = torch.randn(6, 6, dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+    a = a + a.T
+    e, v = linalg.eigh(a)
+    assert torch.allclose(v @ torch.diag(e) @ v.T, a, atol=1e-12)
+
+
 def test_library_exports_every_declared_symbol():
     header = (ROOT / "include" / "gradscf_jk.h").read_text()
     declared = set(re.findall(r"\b(gscf_[a-z0-9_]+)\s*\(", header))
