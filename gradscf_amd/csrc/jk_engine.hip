@@ -140,6 +140,7 @@ static void ensure_tuning() {
 }
 
 static gscf_options g_options = {1, 4};
+constexpr int kMaxMoleculeCartAO = 16384;  // per molecule (32-bit in-molecule offsets)
 constexpr int kMaxStreams = 4;
 
 static size_t rys_slice_bytes(int nr) {
@@ -636,6 +637,10 @@ static void validate_tables(PlanData& P, const int32_t* atm, int natm, const int
     cb += (int64_t)M.nc * M.nc;
     P.nc_max = std::max(P.nc_max, M.nc);
   }
+  // the quartet kernels address one molecule's matrices with 32-bit offsets
+  if (P.nc_max > kMaxMoleculeCartAO)
+    throw InvalidArgument("A molecule has more than 16384 Cartesian AOs; the J/K engine addresses one "
+                          "molecule's matrices with 32-bit offsets");
   P.mols[nmol].sbase = sb;
   P.mols[nmol].cbase = cb;
   P.nn_tot = sb;

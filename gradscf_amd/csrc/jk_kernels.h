@@ -455,6 +455,19 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
   }
 }
 
+// A value every lane of the wavefront holds alike, moved to scalar registers.
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Element idx (unsigned, < 2^29) of a chunk-uniform base through a 32-bit
+// byte offset, which lets the load use the scalar-base + vector-offset form.
+__device__ __forceinline__ double ld_u(const double* __restrict__ base, unsigned idx) {
+  return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + (idx << 3));
+}
+
 // Sum of v[0..N) over the S adjacent lanes of a lane group (S a power of two
 // <= 64, groups aligned to S): butterfly exchange, every lane ends with the sum.
 template <int N, typename R>
@@ -488,8 +501,8 @@ struct Wide {
 // evaluated once, coefficient-free, and added to every column combination
 // with its coefficient product c_A c_B c_C c_D.
 template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool MOMENTS, typename R>
-__device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const Pair& kp, const int2 qk, int slot,
-                                         int s0, int S, int kc, double* __restrict__ krow,
+__device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, const Pair& bp, const Pair& kp,
+                                         const int2 qk, int slot, int s0, int S, int kc, double* __restrict__ krow,
                                          double* __restrict__ jrow, double* __restrict__ jket, int kwin0,
                                          const double* __restrict__ tab) {
   using K = Class<LA, LB, LC, LD>;
@@ -498,8 +511,10 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
   constexpr int ROWS = WA + WB;
   const int ns = a.n;  // LDS row length
   const ShellXYZ sA = a.shells[bp.sa], sB = a.shells[bp.sb], sC = a.shells[kp.sa], sD = a.shells[kp.sb];
-  const MolInfo mi = a.mols[sA.mol];
-  const int n = mi.n;
+  // digestion addresses: molecule bases are chunk-uniform (scalar), element
+  // offsets 32-bit (n <= 16384 Cartesian AOs per molecule, checked at plan
+  // creation)
+  const unsigned n = (unsigned)mi.n;
   QuartetGeom geo;
   double ABd[3], CDd[3];
   load_geom(sA, sB, sC, sD, geo, ABd, CDd);
@@ -570,7 +585,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
     }
     group_sum<WA * WB * WC * WD>(&acc[0][0][0][0], S);
     if (s0 != 0) return;
-    const int oc = sC.ao, od = sD.ao;
+    const unsigned oc = sC.ao, od = sD.ao;
+    const unsigned ra = (unsigned)oa * n, rb = (unsigned)ob * n, rc = oc * n, rd = od * n;
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
       if (a.with_j && bt < a.nbj) {
@@ -580,12 +596,12 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
         for (int ic = 0; ic < WC; ++ic)
 #pragma unroll
           for (int id = 0; id < WD; ++id)
-            dcd[ic][id] = Dm[(size_t)(oc + ic) * n + od + id] + Dm[(size_t)(od + id) * n + oc + ic];
+            dcd[ic][id] = ld_u(Dm, rc + ic * n + od + id) + ld_u(Dm, rd + id * n + oc + ic);
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
           for (int ib = 0; ib < WB; ++ib)
-            dab[ia][ib] = Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia];
+            dab[ia][ib] = ld_u(Dm, ra + ia * n + ob + ib) + ld_u(Dm, rb + ib * n + oa + ia);
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
@@ -606,29 +622,31 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
               for (int ib = 0; ib < WB; ++ib) sj += (double)acc[ia][ib][ic][id] * dab[ia][ib];
+            // (workgroup scope for the LDS window keeps the two atomics from
+            // being merged into one generic-address atomic)
             if (jket)
-              atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * W::NKET + ic * WD + id, f * sj);
+              __hip_atomic_fetch_add(jket + (bt * a.nket_max + (qk.y - kwin0)) * W::NKET + ic * WD + id, f * sj,
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             else
-              atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base + (size_t)(oc + ic) * n + od + id),
-                        f * sj);
+              atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base) + 8u * (rc + ic * n + od + id), f * sj);
           }
       }
       if (a.with_k) {
-        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
+        double* kr = krow + (bt * a.nbra_max + slot) * ROWS * ns;
         double dbd[WB][WD], dbc[WB][WC], dad[WA][WD], dac[WA][WC];
 #pragma unroll
         for (int ib = 0; ib < WB; ++ib) {
 #pragma unroll
-          for (int id = 0; id < WD; ++id) dbd[ib][id] = Dm[(size_t)(ob + ib) * n + od + id];
+          for (int id = 0; id < WD; ++id) dbd[ib][id] = ld_u(Dm, rb + ib * n + od + id);
 #pragma unroll
-          for (int ic = 0; ic < WC; ++ic) dbc[ib][ic] = Dm[(size_t)(ob + ib) * n + oc + ic];
+          for (int ic = 0; ic < WC; ++ic) dbc[ib][ic] = ld_u(Dm, rb + ib * n + oc + ic);
         }
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia) {
 #pragma unroll
-          for (int id = 0; id < WD; ++id) dad[ia][id] = Dm[(size_t)(oa + ia) * n + od + id];
+          for (int id = 0; id < WD; ++id) dad[ia][id] = ld_u(Dm, ra + ia * n + od + id);
 #pragma unroll
-          for (int ic = 0; ic < WC; ++ic) dac[ia][ic] = Dm[(size_t)(oa + ia) * n + oc + ic];
+          for (int ic = 0; ic < WC; ++ic) dac[ia][ic] = ld_u(Dm, ra + ia * n + oc + ic);
         }
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia) {
@@ -639,7 +657,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ib = 0; ib < WB; ++ib)
 #pragma unroll
               for (int id = 0; id < WD; ++id) sk += (double)acc[ia][ib][ic][id] * dbd[ib][id];
-            atomicAdd(kr + (size_t)ia * ns + oc + ic, f * sk);
+            atomicAdd(kr + ia * ns + oc + ic, f * sk);
           }
 #pragma unroll
           for (int id = 0; id < WD; ++id) {
@@ -648,7 +666,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ib = 0; ib < WB; ++ib)
 #pragma unroll
               for (int ic = 0; ic < WC; ++ic) sk += (double)acc[ia][ib][ic][id] * dbc[ib][ic];
-            atomicAdd(kr + (size_t)ia * ns + od + id, f * sk);
+            atomicAdd(kr + ia * ns + od + id, f * sk);
           }
         }
 #pragma unroll
@@ -660,7 +678,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
               for (int id = 0; id < WD; ++id) sk += (double)acc[ia][ib][ic][id] * dad[ia][id];
-            atomicAdd(kr + (size_t)(WA + ib) * ns + oc + ic, f * sk);
+            atomicAdd(kr + (WA + ib) * ns + oc + ic, f * sk);
           }
 #pragma unroll
           for (int id = 0; id < WD; ++id) {
@@ -669,7 +687,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
               for (int ic = 0; ic < WC; ++ic) sk += (double)acc[ia][ib][ic][id] * dac[ia][ic];
-            atomicAdd(kr + (size_t)(WA + ib) * ns + od + id, f * sk);
+            atomicAdd(kr + (WA + ib) * ns + od + id, f * sk);
           }
         }
       }
@@ -726,32 +744,34 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
     }
     group_sum<WA * WB>(&acc[0][0], S);
     if (s0 != 0) return;
-    const int c = sC.ao + icw, d = sD.ao + idw;
+    const unsigned c = sC.ao + icw, d = sD.ao + idw;
+    const unsigned ra = (unsigned)oa * n, rb = (unsigned)ob * n;
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
       if (a.with_j && bt < a.nbj) {
         double* jr = jrow + (bt * a.nbra_max + slot) * W::NBRA;
-        const double dcd = Dm[(size_t)c * n + d] + Dm[(size_t)d * n + c];
+        const double dcd = ld_u(Dm, c * n + d) + ld_u(Dm, d * n + c);
         double s2 = 0.0;
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
           for (int ib = 0; ib < WB; ++ib) {
             atomicAdd(jr + ia * WB + ib, f * (double)acc[ia][ib] * dcd);
-            s2 += (double)acc[ia][ib] * (Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia]);
+            s2 += (double)acc[ia][ib] * (ld_u(Dm, ra + ia * n + ob + ib) + ld_u(Dm, rb + ib * n + oa + ia));
           }
         if (jket)
-          atomicAdd(jket + ((size_t)bt * a.nket_max + (qk.y - kwin0)) * W::NKET + kc, f * s2);
+          __hip_atomic_fetch_add(jket + (bt * a.nket_max + (qk.y - kwin0)) * W::NKET + kc, f * s2, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
         else
-          atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base + (size_t)c * n + d), f * s2);
+          atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base) + 8u * (c * n + d), f * s2);
       }
       if (a.with_k) {
-        double* kr = krow + (size_t)(bt * a.nbra_max + slot) * ROWS * ns;
+        double* kr = krow + (bt * a.nbra_max + slot) * ROWS * ns;
         double db_d[WB], db_c[WB];
 #pragma unroll
         for (int ib = 0; ib < WB; ++ib) {
-          db_d[ib] = Dm[(size_t)(ob + ib) * n + d];
-          db_c[ib] = Dm[(size_t)(ob + ib) * n + c];
+          db_d[ib] = ld_u(Dm, rb + ib * n + d);
+          db_c[ib] = ld_u(Dm, rb + ib * n + c);
         }
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia) {
@@ -761,14 +781,14 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             sc += (double)acc[ia][ib] * db_d[ib];
             sd += (double)acc[ia][ib] * db_c[ib];
           }
-          atomicAdd(kr + (size_t)ia * ns + c, f * sc);
-          atomicAdd(kr + (size_t)ia * ns + d, f * sd);
+          atomicAdd(kr + ia * ns + c, f * sc);
+          atomicAdd(kr + ia * ns + d, f * sd);
         }
         double da_d[WA], da_c[WA];
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia) {
-          da_d[ia] = Dm[(size_t)(oa + ia) * n + d];
-          da_c[ia] = Dm[(size_t)(oa + ia) * n + c];
+          da_d[ia] = ld_u(Dm, ra + ia * n + d);
+          da_c[ia] = ld_u(Dm, ra + ia * n + c);
         }
 #pragma unroll
         for (int ib = 0; ib < WB; ++ib) {
@@ -778,8 +798,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const Pair& bp, const
             sc += (double)acc[ia][ib] * da_d[ia];
             sd += (double)acc[ia][ib] * da_c[ia];
           }
-          atomicAdd(kr + (size_t)(WA + ib) * ns + c, f * sc);
-          atomicAdd(kr + (size_t)(WA + ib) * ns + d, f * sd);
+          atomicAdd(kr + (WA + ib) * ns + c, f * sc);
+          atomicAdd(kr + (WA + ib) * ns + d, f * sd);
         }
       }
     }
@@ -811,6 +831,16 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   extern __shared__ double lds[];
   const int ns = a.n;  // LDS row length (widest molecule)
   const BraChunk ch = a.chunks[blockIdx.x];
+  // every quartet of a chunk belongs to one molecule (runs never cross
+  // molecules); its layout is workgroup-uniform, kept in scalar registers
+  MolInfo mi;
+  {
+    const MolInfo m = a.mols[a.shells[a.pairs[ch.bra0].sa].mol];
+    mi.base = uniform_i64(m.base);
+    mi.dbase = uniform_i64(m.dbase);
+    mi.n = __builtin_amdgcn_readfirstlane(m.n);
+    mi.nseg = __builtin_amdgcn_readfirstlane(m.nseg);
+  }
   const int krow_size = a.nbd * a.nbra_max * ROWS * ns;
   double* krow = lds;                      // [nbd][slot][ROWS][n]
   double* jrow = lds + krow_size;          // [nbj][slot][NA][NB]
@@ -858,7 +888,6 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     if constexpr (MIX) {
       const int la = a.shells[bp.sa].lseg, lb = a.shells[bp.sb].lseg;
       const int lc = a.shells[kp.sa].lseg, ld = a.shells[kp.sb].lseg;
-      const MolInfo mi = a.mols[a.shells[bp.sa].mol];
       const double* dm = a.dmax + mi.dbase;
       const int nsg = mi.nseg;
       const double d6 = fmax(fmax(fmax(dm[la * nsg + lb], dm[lc * nsg + ld]),
@@ -872,13 +901,13 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     }
     if constexpr (MIX) {
       if (lowp) {
-        bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, float>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket,
-                                                                 ch.ket0, tab);
+        bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, float>(a, mi, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow,
+                                                                 jket, ch.ket0, tab);
         continue;
       }
     }
-    bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double>(a, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow, jket,
-                                                                ch.ket0, tab);
+    bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double>(a, mi, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow,
+                                                                jket, ch.ket0, tab);
   }
   __syncthreads();
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)

@@ -104,7 +104,9 @@ const char* gscf_last_error(void);
  * Schwarz bounds Q_ij = sqrt(max |(pq|pq)|) on the GPU when cutoff > 0
  * (csrc/coulomb_shells.h:60-77) and builds Schwarz-sorted pair lists per
  * angular-momentum class.  Replaces the per-call table copy, libcint optimizer
- * and bounds of direct_jk.cc:18-33; the plan may be reused for any density. */
+ * and bounds of direct_jk.cc:18-33; the plan may be reused for any density.
+ * A molecule may have at most 16384 Cartesian AOs (larger ones are rejected
+ * with GSCF_INVALID_ARGUMENT: the kernels use 32-bit in-molecule offsets). */
 int gscf_plan_create(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas,
                      const double* env, int32_t nenv, int32_t cart, double cutoff,
                      void* stream, gscf_plan** plan);
