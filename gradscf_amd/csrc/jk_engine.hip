@@ -139,7 +139,9 @@ static void ensure_tuning() {
   g_tuning_init = true;
 }
 
-static gscf_options g_options = {1, 4};
+// defaults: segmented general contractions (faster on the measured cc-pVDZ
+// workloads; a tuning table may switch it on), four launch streams
+static gscf_options g_options = {0, 4};
 constexpr int kMaxMoleculeCartAO = 16384;  // per molecule (32-bit in-molecule offsets)
 constexpr int kMaxStreams = 4;
 

@@ -89,9 +89,9 @@ int gscf_tuning_reset(void);
  * plans when they are created, like the tuning table. */
 typedef struct {
   int32_t general_contraction; /* 1: two-column s shells (cc-pVXZ 1s/2s) evaluated once per
-                                  primitive quartet for both columns; 0: as two segments */
+                                  primitive quartet for both columns; 0 (default): as two segments */
   int32_t streams;             /* 1..4 HIP streams the class launches of one J/K build are
-                                  spread over (fork/join on the caller's stream) */
+                                  spread over (fork/join on the caller's stream; default 4) */
 } gscf_options;
 int gscf_options_set(const gscf_options* o);
 int gscf_options_get(gscf_options* o);

@@ -48,14 +48,23 @@ def test_eri_blocks_every_class(oracle_mod):
     assert worst < 1e-12, worst
 
 
-@pytest.mark.parametrize("basis,cart", [("6-31g", True), ("cc-pvdz", True), ("cc-pvdz", False)])
-def test_direct_jk_matches_oracle(oracle_mod, basis, cart):
+@pytest.mark.parametrize("basis,cart,gc", [("6-31g", True, 0), ("cc-pvdz", True, 0), ("cc-pvdz", False, 0),
+                                           ("cc-pvdz", True, 1), ("cc-pvdz", False, 1)])
+def test_direct_jk_matches_oracle(oracle_mod, basis, cart, gc):
+    """gc = 1: the O 1s/2s general contraction evaluated natively (one
+    primitive quartet for both columns) instead of as two segments."""
+    from gradscf_amd import autotune as at
     from gradscf_amd.jk import direct_jk
     _, _, atm, bas, env = systems.tables(systems.water(), basis, cart=cart)
     n = oracle_mod.nao(bas, cart)
     d = systems.random_density(n, nb=2, seed=1, symmetric=False)
     ref = oracle_mod.direct_jk(atm, bas, env, d, cart=cart, cutoff=0.0, workers=1)
-    j, k = direct_jk(atm, bas, env, _cuda(d), cart=cart, cutoff=0.0)
+    before = at.get_options()
+    try:
+        at.set_options(general_contraction=gc)
+        j, k = direct_jk(atm, bas, env, _cuda(d), cart=cart, cutoff=0.0)
+    finally:
+        at.set_options(**before)
     assert np.max(np.abs(j.cpu().numpy() - ref[0])) < 1e-10
     assert np.max(np.abs(k.cpu().numpy() - ref[1])) < 1e-10
 
@@ -360,14 +369,20 @@ def test_schedules_do_not_change_results(sched):
 
 @pytest.mark.parametrize("with_j,with_k", [(True, True), (True, False), (False, True)])
 def test_many_nonsymmetric_densities_match_single_builds(with_j, with_k):
-    """(H2O)10 cc-pVDZ (250 Cartesian AO, general s contractions): two
+    """(H2O)10 cc-pVDZ (250 Cartesian AO) with native general contraction: two
     nonsymmetric densities in one build exceed the LDS of the ket-general
     classes, which then run one density per launch; the result must equal two
     single-density builds (operator selection included)."""
     import torch
+    from gradscf_amd import autotune as at
     from gradscf_amd import molecule as M
     _, _, atm, bas, env = systems.tables(M.water_cluster(10, seed=10, box=9.0), "cc-pvdz", cart=True)
-    plan = _plan(atm, bas, env, True, 1e-12)
+    before = at.get_options()
+    try:
+        at.set_options(general_contraction=1)
+        plan = _plan(atm, bas, env, True, 1e-12)
+    finally:
+        at.set_options(**before)
     d = _cuda(systems.random_density(plan.nao, nb=2, seed=12, symmetric=False))
     both = plan.jk(d, with_j=with_j, with_k=with_k)
     for b in range(2):
@@ -375,11 +390,12 @@ def test_many_nonsymmetric_densities_match_single_builds(with_j, with_k):
         assert float((both[:, b] - one[:, 0]).abs().max()) < 1e-12
 
 
-@pytest.mark.parametrize("gc,streams", [(0, 4), (1, 1), (0, 1)])
+@pytest.mark.parametrize("gc,streams", [(1, 4), (1, 1), (0, 1)])
 def test_engine_options_do_not_change_results(gc, streams):
-    """(H2O)4 cc-pVDZ spherical: segmented instead of native general
-    contraction, and class launches on one stream instead of four, change J/K
-    only by the order of FP64 additions (nonsymmetric density, J and K)."""
+    """(H2O)4 cc-pVDZ spherical: native general contraction instead of
+    segments (the default, checked against the oracle elsewhere), and class
+    launches on one stream instead of four, change J/K only by the order of
+    FP64 additions (nonsymmetric density, J and K)."""
     import torch
     from gradscf_amd import autotune as at
     from gradscf_amd import molecule as M

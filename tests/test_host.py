@@ -342,14 +342,14 @@ def test_engine_options_api_without_gpu():
         pytest.skip("engine not built")
     from gradscf_amd import autotune as at
     before = at.get_options()
-    assert before == {"general_contraction": 1, "streams": 4}
+    assert before == {"general_contraction": 0, "streams": 4}
     try:
-        at.set_options(general_contraction=0, streams=2)
-        assert at.get_options() == {"general_contraction": 0, "streams": 2}
+        at.set_options(general_contraction=1, streams=2)
+        assert at.get_options() == {"general_contraction": 1, "streams": 2}
         for bad in ({"streams": 0}, {"streams": 5}, {"general_contraction": 2}):
             with pytest.raises(ValueError):
                 at.set_options(**bad)
-        assert at.get_options() == {"general_contraction": 0, "streams": 2}
+        assert at.get_options() == {"general_contraction": 1, "streams": 2}
         with pytest.raises(ValueError):
             at.set_options(threads=4)
     finally:
