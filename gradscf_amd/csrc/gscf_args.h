@@ -105,6 +105,11 @@ struct BraChunk {
   int32_t nket;   // window length when the ket-side J is accumulated in LDS (0: global atomics)
 };
 
+// Copies of each bra slot's J block in LDS: lane l adds into copy l % 8, so
+// the lanes of a wavefront (which mostly share one bra) do not serialise on
+// the same LDS addresses; the copies are summed at the flush.
+constexpr int kJBraCopies = 8;
+
 struct BraArgs {
   const Pair* pairs;
   const PrimPair* prims;

@@ -1027,7 +1027,7 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             const int wc = ncart_h(qc.lc) * (1 + (pk & 1)), wd = ncart_h(qc.ld) * (1 + (pk >> 1));
             const int rows = wa + wb;
             const int split = (wa * wb * wc * wd > 36) ? wc * wd : 1;
-            const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb);
+            const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb * kJBraCopies);
             const int nr = (qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1;
             size_t pack = (size_t)tu.lds_pack;
             if (tu.rys_lds && per_slot + rys_slice_bytes(nr) <= kLdsBudget) {
@@ -1658,7 +1658,8 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     b.threads = qc.threads;
     // LDS of one launch over nbd_l exchange densities and nbj_l Coulomb densities
     auto layout = [&](int nbd_l, int nbj_l, size_t& lds, int& nket_max, int& tab_lds) {
-      lds = sizeof(double) * ((size_t)nbd_l * qc.nbra_max * (wa + wb) * P.nc_max + (size_t)nbj_l * qc.nbra_max * wa * wb);
+      lds = sizeof(double) * ((size_t)nbd_l * qc.nbra_max * (wa + wb) * P.nc_max +
+                              (size_t)nbj_l * qc.nbra_max * wa * wb * kJBraCopies);
       nket_max = with_j ? qc.nket_max : 0;
       const size_t jket_bytes = sizeof(double) * (size_t)nbj_l * nket_max * wc * wd;
       if (nket_max && lds + jket_bytes <= kLdsBudget) lds += jket_bytes; else nket_max = 0;

@@ -515,6 +515,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
   // offsets 32-bit (n <= 16384 Cartesian AOs per molecule, checked at plan
   // creation)
   const unsigned n = (unsigned)mi.n;
+  // bra J copy of this lane: the digesting lanes are the group leaders
+  // (multiples of S), so consecutive groups take consecutive copies
+  const int jcopy = (threadIdx.x >> (__ffs(S) - 1)) & (kJBraCopies - 1);
   QuartetGeom geo;
   double ABd[3], CDd[3];
   load_geom(sA, sB, sC, sD, geo, ABd, CDd);
@@ -590,7 +593,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
       if (a.with_j && bt < a.nbj) {
-        double* jr = jrow + (bt * a.nbra_max + slot) * W::NBRA;
+        double* jr = jrow + ((bt * a.nbra_max + slot) * kJBraCopies + jcopy) * W::NBRA;
         double dcd[WC][WD], dab[WA][WB];
 #pragma unroll
         for (int ic = 0; ic < WC; ++ic)
@@ -749,7 +752,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
       if (a.with_j && bt < a.nbj) {
-        double* jr = jrow + (bt * a.nbra_max + slot) * W::NBRA;
+        double* jr = jrow + ((bt * a.nbra_max + slot) * kJBraCopies + jcopy) * W::NBRA;
         const double dcd = ld_u(Dm, c * n + d) + ld_u(Dm, d * n + c);
         double s2 = 0.0;
 #pragma unroll
@@ -843,8 +846,8 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   }
   const int krow_size = a.nbd * a.nbra_max * ROWS * ns;
   double* krow = lds;                      // [nbd][slot][ROWS][n]
-  double* jrow = lds + krow_size;          // [nbj][slot][NA][NB]
-  const int jrow_size = a.nbj * a.nbra_max * W::NBRA;
+  double* jrow = lds + krow_size;          // [nbj][slot][copy][NA][NB]
+  const int jrow_size = a.nbj * a.nbra_max * kJBraCopies * W::NBRA;
   // ket-side J of the chunk's ket window: [nbj][nket_max][NC*ND]
   double* jket = (a.nket_max > 0 && ch.nket > 0) ? jrow + jrow_size : nullptr;
   const int jket_size = a.nbj * a.nket_max * W::NKET;
@@ -924,13 +927,15 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     const int row = r < W::WA ? a.shells[bp.sa].ao + r : a.shells[bp.sb].ao + (r - W::WA);
     atomicAdd(a.kp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
   }
-  for (int i = threadIdx.x; i < jrow_size; i += blockDim.x) {
-    const double v = jrow[i];
-    if (v == 0.0) continue;
+  for (int i = threadIdx.x; i < jrow_size / kJBraCopies; i += blockDim.x) {
     const int e = i % W::NBRA;
     const int slot = (i / W::NBRA) % a.nbra_max;
     const int bt = i / (W::NBRA * a.nbra_max);
     if (slot >= ch.nbra) continue;
+    double v = 0.0;
+#pragma unroll
+    for (int c = 0; c < kJBraCopies; ++c) v += jrow[((bt * a.nbra_max + slot) * kJBraCopies + c) * W::NBRA + e];
+    if (v == 0.0) continue;
     const Pair bp = a.pairs[ch.bra0 + slot];
     const MolInfo mi = a.mols[a.shells[bp.sa].mol];
     const int row = a.shells[bp.sa].ao + e / W::WB, col = a.shells[bp.sb].ao + e % W::WB;
