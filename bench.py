@@ -99,6 +99,46 @@ def copy_calibration() -> int:
     return 0
 
 
+def measured_peaks() -> dict:
+    """Achievable FP64 FMA rate (the library's probe: 8 independent FMA chains
+    per lane) and device copy bandwidth (1 GiB read + 1 GiB written per copy)
+    on this GPU, quoted beside the published peaks the roofline uses."""
+    import ctypes
+    import torch
+    from gradscf_amd import _native
+    blocks, iters = 256 * 16, 1 << 14
+    out = torch.empty(blocks * 256, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def probe():
+        _native.check(_native.lib().gscf_fma_probe(blocks, iters, ctypes.c_void_p(out.data_ptr()),
+                                                   ctypes.c_void_p(stream.cuda_stream)))
+
+    def timed(f, reps):
+        f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    t = timed(probe, 3)
+    fma_tflops = blocks * 256 * 16 * iters / t / 1e12
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    t = timed(lambda: b.copy_(a), 5)
+    copy_gbs = 2 * n * 8 / t / 1e9
+    del a, b, out
+    return {"fp64_fma_tflops": fma_tflops, "copy_gbs": copy_gbs,
+            "published": {"fp64_tflops": FP64_PEAK_TFLOPS, "hbm_gbs": 8000.0},
+            "method": "FMA: 4096 x 256 lanes x 8 chains x 16384 steps (library probe); copy: torch copy_ of "
+                      "1 GiB FP64, read + write bytes counted; HIP events, after a warm-up"}
+
+
 def summarize_profile(paths) -> int:
     """Summarise rocprofv3 CSV output (--output-format csv): per-kernel
     statistics from *kernel_stats.csv and, from *counter_collection.csv, the
@@ -509,7 +549,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-builds", type=int, default=3)
+    ap.add_argument("--cpu-builds", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision builds and SCF")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-molecule batched workload")
@@ -707,7 +747,10 @@ def main() -> int:
         "classes": per_class,
         "autotune": tuning,
         "engine_options": at.get_options(),
+        "measured_peaks": measured_peaks(),
     }
+    mp = result["measured_peaks"]
+    log(f"measured peaks: FP64 FMA {mp['fp64_fma_tflops']:.1f} TFLOP/s, device copy {mp['copy_gbs']:.0f} GB/s")
     water = None
     if not args.no_extra:
         water = water_section()

@@ -380,6 +380,24 @@ namespace gscf {
 // ----------------------------------------------------------------------------
 // small device kernels
 // ----------------------------------------------------------------------------
+// FP64 FMA issue-rate probe: eight independent FMA chains per lane (enough
+// to cover the FMA latency), the result stored so nothing is eliminated.
+__global__ __launch_bounds__(256) void fma_probe_kernel(int64_t iters, double* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double a[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = 1e-3 * (double)(t + k);
+  const double m = 0.9999999, c = 1e-9;
+  for (int64_t i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = fma(a[k], m, c);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += a[k];
+  out[t] = s;
+}
+
 __global__ void fill_kernel(double* p, size_t count, double v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) p[i] = v;
@@ -2071,6 +2089,14 @@ int gscf_options_get(gscf_options* o) {
   return guarded([&] {
     if (!o) throw InvalidArgument("NULL options");
     *o = g_options;
+  });
+}
+
+int gscf_fma_probe(int32_t blocks, int64_t iters, double* out, void* stream) {
+  return guarded([&] {
+    if (blocks < 1 || iters < 1 || !out) throw InvalidArgument("fma probe needs blocks >= 1, iters >= 1 and an output");
+    hipLaunchKernelGGL(fma_probe_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, iters, out);
+    HIP_OK(hipGetLastError());
   });
 }
 

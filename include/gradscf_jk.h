@@ -96,6 +96,12 @@ typedef struct {
 int gscf_options_set(const gscf_options* o);
 int gscf_options_get(gscf_options* o);
 
+/* FP64 FMA issue-rate probe (no counterpart in the reference; measures the
+ * device's vector FP64 peak for the roofline): launches blocks x 256 lanes,
+ * each running 8 independent FMA chains of iters steps (16 * iters FLOP per
+ * lane), and writes one double per lane to the device buffer out. */
+int gscf_fma_probe(int32_t blocks, int64_t iters, double* out, void* stream);
+
 /* Last error message of the calling thread. */
 const char* gscf_last_error(void);
 
