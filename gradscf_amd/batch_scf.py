@@ -87,7 +87,8 @@ def _diis_solve(fock_hist, gram, count):
     rhs = torch.zeros((k, m + 1), dtype=dt, device=dev)
     rhs[:, m] = -1.0
     coeff = torch.linalg.solve(b, rhs)[:, :m] * valid
-    return torch.einsum("km,kmij->kij", coeff, fock_hist)
+    k_, m_, n_ = fock_hist.shape[0], fock_hist.shape[1], fock_hist.shape[2]
+    return torch.bmm(coeff[:, None, :], fock_hist.reshape(k_, m_, n_ * n_)).reshape(k_, n_, n_)
 
 
 class _GraphJK:
@@ -180,6 +181,9 @@ def rhf_batch(plan: DirectJKPlan, nelectrons, enucs, *, conv_tol=1e-10, conv_tol
         ti = time.perf_counter()
         dd_packed = torch.zeros_like(d_packed)
         d_new_all = []
+        # every group's DIIS step is queued before the first diagonalisation,
+        # whose host transfer or error check synchronises with the device
+        f_effs = []
         for g, st in zip(groups, state):
             F, D, S, C0 = st["F"], st["D"], st["S"], st["C0"]
             f_eff = F
@@ -187,11 +191,14 @@ def rhf_batch(plan: DirectJKPlan, nelectrons, enucs, *, conv_tol=1e-10, conv_tol
                 err = C0.transpose(1, 2) @ (F @ D @ S - S @ D @ F) @ C0
                 st["fh"][:, head] = F
                 st["eh"][:, head] = err.reshape(g.k, -1)
-                row = torch.einsum("kmx,kx->km", st["eh"], st["eh"][:, head])
+                row = torch.bmm(st["eh"], st["eh"][:, head, :, None])[:, :, 0]
                 st["gram"][:, head, :] = row
                 st["gram"][:, :, head] = row
             if cycle >= 1 and min(count + 1, DIIS_SPACE) >= 2:
                 f_eff = _diis_solve(st["fh"], st["gram"], min(count + 1, DIIS_SPACE))
+            f_effs.append(f_eff)
+        for (g, st), f_eff in zip(zip(groups, state), f_effs):
+            D = st["D"]
             mo_e, C = _diagonalize(f_eff, st["X"])
             Co = C[:, :, :g.nocc]
             D_new = 2.0 * Co @ Co.transpose(1, 2)
@@ -208,8 +215,7 @@ def rhf_batch(plan: DirectJKPlan, nelectrons, enucs, *, conv_tol=1e-10, conv_tol
             st["J"] = st["J"] + g.gather(dj_packed)
             st["K"] = st["K"] + g.gather(dk_packed)
             F_new = st["H"] + st["J"] - 0.5 * st["K"]
-            e_new = (torch.einsum("kij,kij->k", D_new, st["H"]) + 0.5 * torch.einsum("kij,kij->k", D_new, st["J"])
-                     - 0.25 * torch.einsum("kij,kij->k", D_new, st["K"]) + st["enuc"])
+            e_new = (D_new * (st["H"] + 0.5 * st["J"] - 0.25 * st["K"])).sum(dim=(1, 2)) + st["enuc"]
             rms = torch.sqrt(torch.mean((D_new - st["D"]).reshape(g.k, -1) ** 2, dim=1))
             grad = torch.linalg.norm((2.0 * C[:, :, g.nocc:].transpose(1, 2) @ F_new @ C[:, :, :g.nocc])
                                      .reshape(g.k, -1), dim=1)
