@@ -245,6 +245,9 @@ def test_c_abi_rejects_bad_tables_without_gpu():
     rc = L.gscf_direct_jk(atm.ctypes.data, len(atm), bas.ctypes.data, len(bas), env_f.ctypes.data, len(env_f),
                           None, 1, 13, None, 1, ctypes.c_double(0.0), 0, 1, 1, None)
     assert rc == _native.GSCF_INVALID_ARGUMENT and b"workers" in L.gscf_last_error()
+    for blocks, iters in ((0, 10), (4, 0)):
+        assert L.gscf_fma_probe(blocks, iters, ctypes.c_void_p(8), None) == _native.GSCF_INVALID_ARGUMENT
+    assert L.gscf_fma_probe(4, 10, None, None) == _native.GSCF_INVALID_ARGUMENT
 
 
 def test_concat_tables_preserves_each_molecule():
