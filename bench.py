@@ -267,7 +267,10 @@ def batch_section(thr: float, reps: int = 5, autotune: bool = True, retune: bool
             del p0
             result = at.autotune(lambda: DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12),
                                  lambda p: p.jk(d0[None], symmetric=True, validate=False), reps=2,
-                                 search={"rys_lds": (0, 1), "chunk_items": (2048, 8192), "bras_max": (8, 32)},
+                                 search={"rys_lds": (0, 1), "ket_lds": (0, 1), "threads": (256, 512),
+                                         ("chunk_items", "target_prim"): tuple(
+                                             (c, t) for c in (2048, 4096, 8192) for t in (24, 48, 96)),
+                                         "bras_max": (8, 16, 32)},
                                  log=log)
             tuning = {"source": "autotuned in this run", "classes": result}
             if save_dir:
