@@ -119,9 +119,27 @@ class _GraphJK:
 
 
 def rhf_batch(plan: DirectJKPlan, nelectrons, enucs, *, conv_tol=1e-10, conv_tol_density=1e-8,
-              conv_tol_grad=1e-7, max_cycle=80, mixed_threshold=None, use_graph=True) -> BatchRHFResult:
+              conv_tol_grad=1e-7, max_cycle=80, mixed_threshold=None, use_graph=True,
+              incore=False) -> BatchRHFResult:
     """RHF for every molecule of a batched plan (packed layout); nelectrons and
-    enucs give each molecule's electron count and nuclear repulsion."""
+    enucs give each molecule's electron count and nuclear repulsion.
+
+    incore: evaluate the plan's integrals once and digest stored blocks in
+    every iteration (DirectJKPlan.incore; released again at the end unless the
+    plan already held them)."""
+    own_incore = bool(incore) and not getattr(plan, "incore_enabled", False)
+    if own_incore:
+        plan.incore(True)
+    try:
+        return _rhf_batch(plan, nelectrons, enucs, conv_tol, conv_tol_density, conv_tol_grad, max_cycle,
+                          mixed_threshold, use_graph)
+    finally:
+        if own_incore:
+            plan.incore(False)
+
+
+def _rhf_batch(plan, nelectrons, enucs, conv_tol, conv_tol_density, conv_tol_grad, max_cycle, mixed_threshold,
+               use_graph) -> BatchRHFResult:
     nmol = plan.nmol
     nelectrons = np.asarray(nelectrons, dtype=np.int64)
     enucs = torch.as_tensor(np.asarray(enucs, dtype=np.float64))

@@ -136,6 +136,11 @@ struct BraArgs {
   // ket-side J with one 64-byte line per element (element i of jp at
   // jket_pad[8 i]), so atomics to neighbouring J elements do not share a line
   double* jket_pad;
+  // in-core integrals: 0 evaluates quartets, 1 evaluates and stores each
+  // contracted Cartesian block (the slice of one ket component for split
+  // classes) at cache[(quartet * SPLIT + kc) * block], 2 reads them back
+  int mode;
+  double* cache;
 };
 
 struct ParamArgs {

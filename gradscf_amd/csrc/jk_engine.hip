@@ -167,6 +167,7 @@ struct QuartetClass {
   std::vector<BraChunk> chunks;
   BraChunk* d_chunks = nullptr;
   int nbra_max = 0;
+  size_t cache_off = 0, cache_doubles = 0;  // in-core integrals of this class (doubles)
 };
 
 // LDS of the bra-major kernel per workgroup: slots are packed up to the
@@ -238,6 +239,8 @@ struct PlanData {
   double* d_dc = nullptr;
   double* d_jp = nullptr;
   double* d_jpad = nullptr;     // ket-side J, one 64-byte line per element
+  double* d_incore = nullptr;   // in-core integrals: stored Cartesian blocks of the J/K quartets
+  size_t incore_doubles = 0;
 
   double* d_kp = nullptr;
   size_t ws_nbd = 0;
@@ -524,8 +527,17 @@ __global__ void sandwich_kernel(const double* __restrict__ Mc, int64_t nn_tot, c
 }
 
 // Quartet list of one sub-task: g -> (bra0 + b, ket0 + g - qoff[b]) with qoff[b] <= g < qoff[b+1].
+// Quartets of one (bra run, ket run) sub-task: bra b keeps the ket-run prefix
+// [0, qoff[b+1] - qoff[b]) (Schwarz-descending run).  The prefix is cut into
+// segments of seg kets (the ket windows of single-bra chunks) and each segment
+// is written in the run's primitive-count order: groups[k] = (begin, count) of
+// group k in order[], whose entries are run positions, ascending inside a
+// group, groups largest primitive count first.  Adjacent lanes then take kets
+// of near-equal primitive counts, so their loop trip counts agree, and every
+// segment keeps its set of kets.
 __global__ void expand_kernel(const int64_t* __restrict__ qoff, int nbra, int bra0, int ket0, int64_t nq,
-                              int2* __restrict__ out) {
+                              int seg, const int* __restrict__ order, const int2* __restrict__ groups,
+                              int ngroups, int2* __restrict__ out) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nq) return;
   int lo = 0, hi = nbra;
@@ -533,7 +545,32 @@ __global__ void expand_kernel(const int64_t* __restrict__ qoff, int nbra, int br
     const int mid = (lo + hi) >> 1;
     if (qoff[mid] <= g) lo = mid; else hi = mid;
   }
-  out[g] = make_int2(bra0 + lo, ket0 + (int)(g - qoff[lo]));
+  const int cnt = (int)(qoff[lo + 1] - qoff[lo]);
+  const int r = (int)(g - qoff[lo]);
+  const int s0 = r - r % seg, s1 = (cnt - s0 > seg) ? s0 + seg : cnt;
+  int rr = r - s0, ket = r;
+  // the groups partition the run, so their members in [s0, s1) number exactly
+  // s1 - s0 and rr < s1 - s0 lands in one of them
+  for (int k = 0; k < ngroups; ++k) {
+    const int2 gr = groups[k];
+    int a = 0, b = gr.y;
+    while (a < b) {
+      const int mid = (a + b) >> 1;
+      if (order[gr.x + mid] < s0) a = mid + 1; else b = mid;
+    }
+    int c = a;
+    b = gr.y;
+    while (c < b) {
+      const int mid = (c + b) >> 1;
+      if (order[gr.x + mid] < s1) c = mid + 1; else b = mid;
+    }
+    if (rr < c - a) {
+      ket = order[gr.x + a + rr];
+      break;
+    }
+    rr -= c - a;
+  }
+  out[g] = make_int2(bra0 + lo, ket0 + ket);
 }
 
 // dmax[m][S][T] = max over batches and over the Cartesian block (S, T) of |D_m|
@@ -953,10 +990,38 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
     r.end = k;
     runs.push_back(r);
   }
+  // ket order of every run for the quartet expansion: run positions grouped by
+  // primitive count (1/16-octave groups, largest first), ascending inside a
+  // group, so that every group keeps the run's Schwarz order and a bra's
+  // surviving prefix is a prefix of each group
+  std::vector<int> run_order(S.pairs.size());
+  std::vector<int2> run_groups;
+  std::vector<int> run_g0(runs.size() + 1, 0);
+  for (size_t ri = 0; ri < runs.size(); ++ri) {
+    const Run& r = runs[ri];
+    const int len = r.end - r.beg;
+    std::vector<int> key(len);
+    for (int j = 0; j < len; ++j) key[j] = (int)std::floor(16.0 * std::log2((double)S.pairs[r.beg + j].np));
+    int* ord = run_order.data() + r.beg;
+    for (int j = 0; j < len; ++j) ord[j] = j;
+    std::stable_sort(ord, ord + len, [&](int x, int y) { return key[x] > key[y]; });
+    run_g0[ri] = (int)run_groups.size();
+    for (int j = 0; j < len;) {
+      int e = j;
+      while (e < len && key[ord[e]] == key[ord[j]]) ++e;
+      run_groups.push_back(make_int2(j, e - j));
+      j = e;
+    }
+  }
+  run_g0[runs.size()] = (int)run_groups.size();
+  if (run_order.empty()) run_order.push_back(0);
+  if (run_groups.empty()) run_groups.push_back(make_int2(0, 0));
   // runs of each (class, molecule) in list order, for the intra-molecule pairing
   std::map<std::pair<int, int>, std::vector<int>> runs_of;
   for (int i = 0; i < (int)runs.size(); ++i) runs_of[{runs[i].cls, runs[i].mol}].push_back(i);
   S.d_pairs = dev_upload(S.pairs, s);
+  int* d_run_order = dev_upload(run_order, s);
+  int2* d_run_groups = dev_upload(run_groups, s);
   {
     std::vector<double> pq = S.pair_q;
     if (pq.empty()) pq.push_back(0.0);
@@ -980,7 +1045,7 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
           const int kChunkBrasMax = tu.bras_max;
           qc.nq = 0;
           qc.nitems = 0;
-          struct Pending { std::vector<int64_t> qoff; int bra0, ket0, nbra; int64_t q0, nq; };
+          struct Pending { std::vector<int64_t> qoff; int bra0, ket0, nbra; int64_t q0, nq; int rk, seg; };
           std::vector<Pending> pend;
           for (int rbi = 0; rbi < (int)runs.size(); ++rbi) {
             const Run& rb = runs[rbi];
@@ -1034,7 +1099,7 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
               }
               qc.nq += nq;
               qc.tasks.push_back(t);
-              pend.push_back({std::move(qoff), rb.beg, rk.beg, nbra, t.q0, nq});
+              pend.push_back({std::move(qoff), rb.beg, rk.beg, nbra, t.q0, nq, rki, std::numeric_limits<int>::max()});
             }
           }
           if (qc.nq == 0) continue;
@@ -1057,9 +1122,13 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             qc.nbra_max = per_slot > kLdsBudget ? 0
                           : (int)std::max<size_t>(1, std::min<size_t>((size_t)kChunkBrasMax, pack / per_slot));
             for (size_t ti = 0; qc.nbra_max >= 1 && ti < pend.size(); ++ti) {
-              const Pending& pd = pend[ti];
+              Pending& pd = pend[ti];
               const SubTask& t = qc.tasks[ti];
               const int64_t per_q = (int64_t)t.S * split;
+              // a bra longer than one chunk is cut into windows of step kets;
+              // shorter bras fit one window (cnt <= step)
+              pd.seg = (int)std::min<int64_t>(std::numeric_limits<int>::max(),
+                                              std::max<int64_t>(1, kChunkItems / per_q));
               int b = 0;
               while (b < pd.nbra) {
                 const int64_t cnt = pd.qoff[b + 1] - pd.qoff[b];
@@ -1120,7 +1189,8 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
           for (const Pending& pd : pend) {
             int64_t* d_qoff = dev_upload(pd.qoff, s);
             hipLaunchKernelGGL(expand_kernel, dim3(nblocks((size_t)pd.nq)), dim3(256), 0, s, d_qoff, pd.nbra,
-                               pd.bra0, pd.ket0, pd.nq, qc.d_quartets + pd.q0);
+                               pd.bra0, pd.ket0, pd.nq, pd.seg, d_run_order + pd.ket0, d_run_groups + run_g0[pd.rk],
+                               run_g0[pd.rk + 1] - run_g0[pd.rk], qc.d_quartets + pd.q0);
             HIP_OK(hipGetLastError());
             HIP_OK(hipStreamSynchronize(s));
             HIP_OK(hipFree(d_qoff));
@@ -1129,6 +1199,8 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
           S.nquartets += qc.nq;
           S.classes.push_back(std::move(qc));
         }
+  HIP_OK(hipFree(d_run_order));
+  HIP_OK(hipFree(d_run_groups));
   S.built = true;
 }
 
@@ -1563,7 +1635,7 @@ static void free_plan(PlanData& P) {
   free_pair_set(P.seg_set);
   f(P.d_shells); f(P.d_prims); f(P.d_all_pairs);
   f(P.d_c2s_idx); f(P.d_c2s_coef); f(P.d_s2c_idx); f(P.d_s2c_coef);
-  f(P.d_dc); f(P.d_jp); f(P.d_jpad); f(P.d_kp); f(P.d_flag);
+  f(P.d_dc); f(P.d_jp); f(P.d_jpad); f(P.d_kp); f(P.d_flag); f(P.d_incore);
   f(P.d_pairs1e); f(P.d_nuclei);
   f(P.d_dmax); f(P.d_seg_atom);
   f(P.d_molinfo); f(P.d_molao);
@@ -1657,6 +1729,12 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     const int wa = ncart_h(qc.la) * (1 + (qc.pb & 1)), wb = ncart_h(qc.lb) * (1 + (qc.pb >> 1));
     const int wc = ncart_h(qc.lc) * (1 + (qc.pk & 1)), wd = ncart_h(qc.ld) * (1 + (qc.pk >> 1));
     BraArgs b;
+    b.mode = 0;
+    b.cache = nullptr;
+    if (P.d_incore && qc.cache_doubles) {
+      b.mode = 2;
+      b.cache = P.d_incore + qc.cache_off;
+    }
     b.pairs = S.d_pairs;
     b.prims = S.d_prims;
     b.shells = S.d_shells;
@@ -1681,7 +1759,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       nket_max = with_j ? qc.nket_max : 0;
       const size_t jket_bytes = sizeof(double) * (size_t)nbj_l * nket_max * wc * wd;
       if (nket_max && lds + jket_bytes <= kLdsBudget) lds += jket_bytes; else nket_max = 0;
-      tab_lds = qc.tab_lds;
+      tab_lds = b.mode == 2 ? 0 : qc.tab_lds;  // stored blocks need no Rys table
       if (tab_lds && lds + qc.tab_bytes <= kLdsBudget) lds += qc.tab_bytes; else tab_lds = 0;
     };
     const int nbd_eff = with_k ? nbd : nb;
@@ -1755,6 +1833,74 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
                      P.d_jp, P.d_jpad, P.d_kp, P.ncc_tot, P.nn_tot, nb, sym ? 1 : 0, with_j, with_k, P.d_molao,
                      P.nmol, P.d_s2c_idx, P.d_s2c_coef, out);
   HIP_OK(hipGetLastError());
+}
+
+// In-core integrals: the contracted Cartesian blocks of every quartet of the
+// segmented bra-major classes, stored once (quartet-major per class, a split
+// class's block as its ket-component slices) so that later builds digest
+// them instead of evaluating them.  Classes without bra-major chunks or with
+// general-contraction patterns keep evaluating.
+static size_t incore_layout(PlanData& P) {
+  size_t total = 0;
+  for (QuartetClass& qc : P.jk.classes) {
+    qc.cache_off = total;
+    qc.cache_doubles = 0;
+    if (qc.pb || qc.pk || !qc.d_chunks || qc.nbra_max < 1) continue;
+    qc.cache_doubles = (size_t)qc.nq * ncart_h(qc.la) * ncart_h(qc.lb) * ncart_h(qc.lc) * ncart_h(qc.ld);
+    total += qc.cache_doubles;
+  }
+  return total;
+}
+
+static void incore_set(PlanData& P, int enable, hipStream_t s) {
+  if (enable != 0 && enable != 1) throw InvalidArgument("in-core flag must be 0 or 1");
+  if (!enable) {
+    if (P.d_incore) {
+      HIP_OK(hipStreamSynchronize(s));
+      HIP_OK(hipFree(P.d_incore));
+    }
+    P.d_incore = nullptr;
+    P.incore_doubles = 0;
+    for (QuartetClass& qc : P.jk.classes) qc.cache_doubles = 0;
+    return;
+  }
+  if (P.d_incore) return;
+  const size_t total = incore_layout(P);
+  if (total == 0) throw InvalidArgument("no quartet class of this plan can keep its integrals in core");
+  double* buf = nullptr;
+  if (hipMalloc(&buf, sizeof(double) * total) != hipSuccess) {
+    (void)hipGetLastError();
+    for (QuartetClass& qc : P.jk.classes) qc.cache_doubles = 0;
+    throw InvalidArgument("in-core integrals need " + std::to_string(sizeof(double) * total) +
+                          " bytes of device memory, which could not be allocated");
+  }
+  const PairSet& S = P.jk;
+  for (const QuartetClass& qc : S.classes) {
+    if (!qc.cache_doubles) continue;
+    BraArgs b;
+    std::memset(&b, 0, sizeof(b));
+    b.mode = 1;
+    b.cache = buf + qc.cache_off;
+    b.pairs = S.d_pairs;
+    b.prims = S.d_prims;
+    b.shells = S.d_shells;
+    b.coef = S.d_coef;
+    b.quartets = qc.d_quartets;
+    b.chunks = qc.d_chunks;
+    b.nchunks = (int)qc.chunks.size();
+    b.nbra_max = qc.nbra_max;
+    b.mols = P.d_molinfo;
+    b.n = P.nc_max;
+    b.pair_q = S.d_pair_q;
+    b.threads = qc.threads;
+    b.bstride = P.ncc_tot;
+    // no densities: the pass only evaluates and stores
+    launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, 0, s);
+  }
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(s));
+  P.d_incore = buf;
+  P.incore_doubles = total;
 }
 
 static void int1e(PlanData& P, double* s_out, double* t_out, double* v_out, hipStream_t st) {
@@ -2178,6 +2324,24 @@ int gscf_plan_get_schwarz(const gscf_plan* plan, double* q) {
     const PlanData& P = plan->d;
     if (P.parent_q.empty()) throw InvalidArgument("Schwarz bounds are computed only for a positive cutoff");
     std::copy(P.parent_q.begin(), P.parent_q.end(), q);
+  });
+}
+
+int gscf_plan_incore_bytes(const gscf_plan* plan, int64_t* bytes) {
+  return guarded([&] {
+    if (!plan || !bytes) throw InvalidArgument("NULL plan or output");
+    size_t total = 0;
+    for (const QuartetClass& qc : plan->d.jk.classes)
+      if (!qc.pb && !qc.pk && qc.d_chunks && qc.nbra_max >= 1)
+        total += (size_t)qc.nq * ncart_h(qc.la) * ncart_h(qc.lb) * ncart_h(qc.lc) * ncart_h(qc.ld);
+    *bytes = (int64_t)(sizeof(double) * total);
+  });
+}
+
+int gscf_plan_incore(gscf_plan* plan, int32_t enable, void* stream) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("NULL plan");
+    incore_set(plan->d, enable, (hipStream_t)stream);
   });
 }
 

@@ -500,11 +500,13 @@ struct Wide {
 // window or to global).  With general contraction each primitive quartet is
 // evaluated once, coefficient-free, and added to every column combination
 // with its coefficient product c_A c_B c_C c_D.
-template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool MOMENTS, typename R>
+// MODE 1 also stores the block (after the lane-group sum) at cq, MODE 2 reads
+// it from cq instead of evaluating it (in-core integrals).
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool MOMENTS, typename R, int MODE = 0>
 __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, const Pair& bp, const Pair& kp,
                                          const int2 qk, int slot, int s0, int S, int kc, double* __restrict__ krow,
                                          double* __restrict__ jrow, double* __restrict__ jket, int kwin0,
-                                         const double* __restrict__ tab) {
+                                         const double* __restrict__ tab, double* __restrict__ cq) {
   using K = Class<LA, LB, LC, LD>;
   using W = Wide<LA, LB, LC, LD, CA, CB, CC, CD>;
   constexpr int WA = W::WA, WB = W::WB, WC = W::WC, WD = W::WD;
@@ -530,6 +532,10 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
   const int oa = sA.ao, ob = sB.ao;
   if constexpr (!W::kSplit) {
     R acc[WA][WB][WC][WD];
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int e = 0; e < WA * WB * WC * WD; ++e) (&acc[0][0][0][0])[e] = cq[e];
+    } else {
 #pragma unroll
     for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
@@ -588,6 +594,11 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
     }
     group_sum<WA * WB * WC * WD>(&acc[0][0][0][0], S);
     if (s0 != 0) return;
+    if constexpr (MODE == 1) {
+#pragma unroll
+      for (int e = 0; e < WA * WB * WC * WD; ++e) cq[e] = (double)(&acc[0][0][0][0])[e];
+    }
+    }
     const unsigned oc = sC.ao, od = sD.ao;
     const unsigned ra = (unsigned)oa * n, rb = (unsigned)ob * n, rc = oc * n, rd = od * n;
     for (int bt = 0; bt < a.nbd; ++bt) {
@@ -707,6 +718,10 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
 #pragma unroll
     for (int x = 0; x < 3; ++x) ket_weights<K::MM, R>(ec[x], ed[x], CD_[x], wk[x]);
     R acc[WA][WB];
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int e = 0; e < WA * WB; ++e) (&acc[0][0])[e] = cq[e];
+    } else {
 #pragma unroll
     for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
@@ -747,6 +762,11 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
     }
     group_sum<WA * WB>(&acc[0][0], S);
     if (s0 != 0) return;
+    if constexpr (MODE == 1) {
+#pragma unroll
+      for (int e = 0; e < WA * WB; ++e) cq[e] = (double)(&acc[0][0])[e];
+    }
+    }
     const unsigned c = sC.ao + icw, d = sD.ao + idw;
     const unsigned ra = (unsigned)oa * n, rb = (unsigned)ob * n;
     for (int bt = 0; bt < a.nbd; ++bt) {
@@ -826,7 +846,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
 // With TLDS the Rys table slice of the class (rys_slice_size(NR) doubles) is
 // copied to LDS after the slabs, so the per-lane interval gathers of the root
 // evaluation read LDS instead of global memory.
-template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool TLDS, bool MIX>
+// MODE (in-core integrals): 0 evaluates, 1 evaluates and stores the blocks,
+// 2 digests stored blocks (one lane per quartet component: no primitive split).
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool TLDS, bool MIX, int MODE = 0>
 __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   using K = Class<LA, LB, LC, LD>;
   using W = Wide<LA, LB, LC, LD, CA, CB, CC, CD>;
@@ -857,7 +879,7 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   // one or two roots: the table staged in LDS, or the Boys-moment quadrature
   // (global Boys grid) when the table stays in global memory
   constexpr bool kMoments = K::NR <= 2 && !TLDS;
-  if constexpr (TLDS) {
+  if constexpr (TLDS && MODE != 2) {
     double* ltab = lds + ((krow_size + jrow_size + jket_size + 1) & ~1);  // 16-byte aligned
     for (int i = threadIdx.x; i < rys_slice_size(K::NR); i += blockDim.x) ltab[i] = tab[i];
     tab = ltab;
@@ -874,11 +896,13 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   // walk the kets in the same order make their ket-side J atomics collide on
   // the same addresses at the same time (the memory-side atomic unit then
   // serialises them).
-  const int64_t total = (int64_t)ch.nq * ch.S * W::SPLIT;
+  const int S = MODE == 2 ? 1 : ch.S;
+  const int64_t total = (int64_t)ch.nq * S * W::SPLIT;
   const int64_t rot = ((int64_t)blockIdx.x * 40503) % ch.nq;
+  constexpr int kBlock = W::kSplit ? W::NBRA : W::NBRA * W::NKET;
   for (int64_t it = threadIdx.x; it < total; it += blockDim.x) {
-    const int s0 = (int)(it & (ch.S - 1));
-    const int64_t rest = it / ch.S;
+    const int s0 = (int)(it & (S - 1));
+    const int64_t rest = it / S;
     const int kc = (int)(rest % W::SPLIT);
     int64_t qr = rest / W::SPLIT + rot;
     if (qr >= ch.nq) qr -= ch.nq;
@@ -887,6 +911,8 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     const int slot = qk.x - ch.bra0;
     const Pair bp = a.pairs[qk.x];
     const Pair kp = a.pairs[qk.y];
+    double* cq = nullptr;
+    if constexpr (MODE != 0) cq = a.cache + ((size_t)g * W::SPLIT + kc) * kBlock;
     bool lowp = false;
     if constexpr (MIX) {
       const int la = a.shells[bp.sa].lseg, lb = a.shells[bp.sb].lseg;
@@ -904,13 +930,13 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     }
     if constexpr (MIX) {
       if (lowp) {
-        bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, float>(a, mi, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow,
-                                                                 jket, ch.ket0, tab);
+        bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, float>(a, mi, bp, kp, qk, slot, s0, S, kc, krow, jrow,
+                                                                 jket, ch.ket0, tab, cq);
         continue;
       }
     }
-    bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double>(a, mi, bp, kp, qk, slot, s0, ch.S, kc, krow, jrow,
-                                                                jket, ch.ket0, tab);
+    bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double, MODE>(a, mi, bp, kp, qk, slot, s0, S, kc, krow, jrow,
+                                                                      jket, ch.ket0, tab, cq);
   }
   __syncthreads();
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
@@ -1033,6 +1059,17 @@ template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD>
 void launch_jk_bra_t(const BraArgs& a, size_t lds_bytes, hipStream_t s) {
   if (a.nchunks <= 0) return;
   const dim3 block(a.threads), grid((unsigned)a.nchunks);
+  // in-core integrals (FP64 blocks; segmented patterns only)
+  if constexpr (CA * CB * CC * CD == 1) {
+    if (a.mode == 1) {
+      hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false, 1>), grid, block, lds_bytes, s, a);
+      return;
+    }
+    if (a.mode == 2) {
+      hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false, 2>), grid, block, lds_bytes, s, a);
+      return;
+    }
+  }
   // variants: Rys table in LDS or global; FP64 only, or with the FP32 quartet path
   if (a.tab_lds && a.mixed)
     hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, true, true>), grid, block, lds_bytes, s, a);

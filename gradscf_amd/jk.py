@@ -103,6 +103,7 @@ class DirectJKPlan:
                 self.env.ctypes.data, len(self.env), len(ma) - 1, ma.ctypes.data, mb.ctypes.data,
                 int(self.cart), ctypes.c_double(self.cutoff), _stream_handle(), ctypes.byref(handle)))
         self._handle = handle
+        self.incore_enabled = False
         info = _native.PlanInfo()
         _native.check(L.gscf_plan_get_info(handle, ctypes.byref(info)))
         self.info = {k: getattr(info, k) for k, _ in _native.PlanInfo._fields_}
@@ -198,6 +199,20 @@ class DirectJKPlan:
             out.append(flat[o:o + c * c].reshape(c, c))
             o += c * c
         return out
+
+    def incore_bytes(self) -> int:
+        """Device bytes that keeping the plan's integrals in core would take."""
+        b = ctypes.c_int64()
+        _native.check(_native.lib().gscf_plan_incore_bytes(self._handle, ctypes.byref(b)))
+        return int(b.value)
+
+    def incore(self, enable: bool = True) -> None:
+        """Evaluate every screened-in quartet block once and keep it in device
+        memory, so that later jk() calls digest stored integrals (the packed-ERI
+        route of the reference, csrc/packed_eri.cc + csrc/packed_jk.cc); False
+        releases them."""
+        _native.check(_native.lib().gscf_plan_incore(self._handle, int(bool(enable)), _stream_handle()))
+        self.incore_enabled = bool(enable)
 
     def profile(self, enable: bool) -> None:
         """Start (and reset) or stop per-class HIP-event timing of jk() launches."""

@@ -80,8 +80,23 @@ def _diis_solve(fock_hist, gram, count):
 
 
 def rhf(plan: DirectJKPlan, nelectron: int, enuc: float, *, conv_tol=1e-10, conv_tol_density=1e-8,
-        conv_tol_grad=1e-7, max_cycle=80, jk_flags=None) -> RHFResult:
-    """Run RHF with the direct J/K engine (`plan` already holds the basis and screening)."""
+        conv_tol_grad=1e-7, max_cycle=80, jk_flags=None, incore=False) -> RHFResult:
+    """Run RHF with the direct J/K engine (`plan` already holds the basis and screening).
+
+    incore: evaluate the plan's integrals once and digest stored blocks in
+    every cycle (DirectJKPlan.incore; released again at the end unless the
+    plan already held them)."""
+    own_incore = bool(incore) and not getattr(plan, "incore_enabled", False)
+    if own_incore:
+        plan.incore(True)
+    try:
+        return _rhf(plan, nelectron, enuc, conv_tol, conv_tol_density, conv_tol_grad, max_cycle, jk_flags)
+    finally:
+        if own_incore:
+            plan.incore(False)
+
+
+def _rhf(plan, nelectron, enuc, conv_tol, conv_tol_density, conv_tol_grad, max_cycle, jk_flags) -> RHFResult:
     jk_flags = dict(jk_flags or {})
     t0 = time.perf_counter()
     s, t, v = plan.int1e()

@@ -144,6 +144,22 @@ int gscf_plan_get_molecules(const gscf_plan* plan, int32_t max_mol, int32_t* n, 
  * with local shell indices. */
 int gscf_plan_get_schwarz(const gscf_plan* plan, double* q);
 
+/* In-core integrals: the plan-resident form of the reference's packed-ERI
+ * route (producer csrc/packed_eri.cc:5-14 + consumer csrc/packed_jk.cc:6-79,
+ * selected for small molecules at scf/rks.py:323-350).  gscf_plan_incore(plan,
+ * 1, stream) evaluates the contracted Cartesian block of every quartet the
+ * plan screens in once and keeps it in device memory (stored in the order the
+ * J/K kernels visit quartets); until gscf_plan_incore(plan, 0, stream)
+ * releases it, every gscf_jk_build of the plan digests stored blocks instead
+ * of evaluating them (FP64 values; GSCF_FLAG_MIXED_PRECISION has no effect on
+ * stored classes).  Classes the plan runs with general contraction keep
+ * evaluating.  gscf_plan_incore_bytes gives the device bytes enabling would
+ * allocate; enabling fails with InvalidArgument when they cannot be
+ * allocated.  Enabling synchronises `stream` (not graph-capturable); builds
+ * afterwards are capturable as before. */
+int gscf_plan_incore_bytes(const gscf_plan* plan, int64_t* bytes);
+int gscf_plan_incore(gscf_plan* plan, int32_t enable, void* stream);
+
 /* Measurement hooks (no counterpart in the reference): while enabled, each
  * gscf_jk_build records a HIP event pair around every class kernel on the
  * launch stream; gscf_plan_class_times returns the summed elapsed ms and
