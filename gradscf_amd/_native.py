@@ -40,7 +40,8 @@ class ClassTuning(ctypes.Structure):
 
 
 class Options(ctypes.Structure):
-    _fields_ = [("general_contraction", ctypes.c_int32), ("streams", ctypes.c_int32)]
+    _fields_ = [("general_contraction", ctypes.c_int32), ("streams", ctypes.c_int32),
+                ("partial_sums", ctypes.c_int32)]
 
 
 class NativeError(RuntimeError):

@@ -70,7 +70,8 @@ def reset() -> None:
 
 
 def get_options() -> dict:
-    """Process-wide engine options (general contraction, launch streams)."""
+    """Process-wide engine options (general contraction, launch streams,
+    partial-sum flushes)."""
     o = _native.Options()
     _native.check(_native.lib().gscf_options_get(ctypes.byref(o)))
     return {f: getattr(o, f) for f, _ in _native.Options._fields_}

@@ -103,6 +103,9 @@ struct BraChunk {
   int32_t nbra;
   int32_t ket0;   // first ket pair of the chunk's ket window
   int32_t nket;   // window length when the ket-side J is accumulated in LDS (0: global atomics)
+  // partial-sum slab of the chunk (per density): its exchange rows
+  // [nbra][ROWS][n_m], then its ket window [nket][NKET]
+  int64_t poff;
 };
 
 // Copies of each bra slot's J block in LDS: lane l adds into copy l % 8, so
@@ -141,6 +144,12 @@ struct BraArgs {
   // classes) at cache[(quartet * SPLIT + kc) * block], 2 reads them back
   int mode;
   double* cache;
+  // partial sums instead of flush atomics: bit 0 stores the exchange rows, bit
+  // 1 the ket window, into part[bt * part_stride + chunk.poff + ...] with plain
+  // stores; reduction kernels add them into K and J afterwards
+  int part_mode;
+  int64_t part_stride;
+  double* part;
 };
 
 struct ParamArgs {

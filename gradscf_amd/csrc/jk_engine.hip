@@ -141,7 +141,7 @@ static void ensure_tuning() {
 
 // defaults: segmented general contractions (faster on the measured cc-pVDZ
 // workloads; a tuning table may switch it on), four launch streams
-static gscf_options g_options = {0, 4};
+static gscf_options g_options = {0, 4, 1};
 constexpr int kMaxMoleculeCartAO = 16384;  // per molecule (32-bit in-molecule offsets)
 constexpr int kMaxStreams = 4;
 
@@ -168,6 +168,8 @@ struct QuartetClass {
   BraChunk* d_chunks = nullptr;
   int nbra_max = 0;
   size_t cache_off = 0, cache_doubles = 0;  // in-core integrals of this class (doubles)
+  int64_t part_beg = 0, part_end = 0;       // partial-slab range of this class's chunks (per density)
+  bool part = false;                        // chunks flush into partial slabs
 };
 
 // LDS of the bra-major kernel per workgroup: slots are packed up to the
@@ -175,6 +177,9 @@ struct QuartetClass {
 // slot (with the table, when staged) may use up to 64 KB.
 constexpr size_t kLdsBudget = 64 * 1024;
 constexpr size_t kKetLdsBytes = 16 * 1024;  // LDS ket-J window per density
+// Partial-sum slabs per density at most; a plan whose slabs would be larger
+// flushes with atomics instead.
+constexpr int64_t kPartCapBytes = int64_t(2) << 30;
 
 // The shell pairs quartet kernels run over.  Segmented sets hold one
 // contraction column per shell; contracted sets (J/K only) hold parent shells
@@ -195,6 +200,19 @@ struct PairSet {
   std::vector<QuartetClass> classes;
   int64_t nquartets = 0;
   int64_t nprims = 0;             // primitive pairs owned by a contracted set
+  // partial-sum slabs of the bra-major chunks (segmented sets): doubles per
+  // density, and plan-time lists of the slab rows that add into each
+  // Cartesian K row (krp/kent, row-major CSR over all molecules' rows) and
+  // of the window entries that add into each ket pair's J block (jrp/jent)
+  int64_t part_total = 0;
+  int64_t* d_krp = nullptr;
+  int64_t* d_kent = nullptr;
+  int64_t* d_jrp = nullptr;
+  int64_t* d_jent = nullptr;
+  int64_t* d_row_dst = nullptr;   // per global Cartesian row: first K element and length
+  int* d_row_n = nullptr;
+  int64_t nrows = 0;
+  int nc_max = 0;
 };
 
 struct PlanData {
@@ -239,6 +257,8 @@ struct PlanData {
   double* d_dc = nullptr;
   double* d_jp = nullptr;
   double* d_jpad = nullptr;     // ket-side J, one 64-byte line per element
+  double* d_part = nullptr;     // partial-sum slabs of the bra-major chunks, [nbd][part_total]
+  size_t part_nbd = 0;
   double* d_incore = nullptr;   // in-core integrals: stored Cartesian blocks of the J/K quartets
   size_t incore_doubles = 0;
 
@@ -463,6 +483,65 @@ __global__ void dens_to_cart_kernel(const double* __restrict__ D, int64_t nn_tot
 }
 
 // out[op][b][p][q]: J = T (Jp + Jp^T) T^T, K = T (Kp_b + Kp_b'^T) T^T, per molecule
+// K rows += their partial exchange rows: block (x: global Cartesian row R,
+// y: 64-column range, z: density); the row's slab list is block-uniform (scalar
+// loads) and eight slab rows are read per step so that many loads are in
+// flight.  row_dst[R] / row_n[R]: first element and length of row R in K.
+__global__ __launch_bounds__(64) void kpart_reduce_kernel(const double* __restrict__ part, int64_t stride,
+                                                          const int64_t* __restrict__ rp,
+                                                          const int64_t* __restrict__ ent,
+                                                          const int64_t* __restrict__ row_dst,
+                                                          const int* __restrict__ row_n, int64_t ncc_tot,
+                                                          double* __restrict__ kp) {
+  const int64_t R = blockIdx.x;
+  const int bt = blockIdx.z;
+  const int col = blockIdx.y * 64 + threadIdx.x;
+  const int n = row_n[R];
+  if (col >= n) return;
+  const int64_t e0 = rp[R], e1 = rp[R + 1];
+  const double* __restrict__ pb = part + (size_t)bt * stride + col;
+  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int64_t e = e0;
+  for (; e + 8 <= e1; e += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] += pb[ent[e + u]];
+  }
+  for (int u = 0; e < e1; ++e, ++u) v[u] += pb[ent[e]];
+  kp[(size_t)bt * ncc_tot + row_dst[R] + col] += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+}
+
+// Ket-side J blocks from the partial ket windows: one wavefront per (ket
+// pair, density), lane e < block size; every pair owns its J block, so the
+// adds do not race.
+__global__ __launch_bounds__(64) void jpart_reduce_kernel(const double* __restrict__ part, int64_t stride,
+                                                          const int64_t* __restrict__ rp,
+                                                          const int64_t* __restrict__ ent,
+                                                          const Pair* __restrict__ pairs,
+                                                          const ShellXYZ* __restrict__ shells,
+                                                          const MolInfo* __restrict__ mols, int64_t ncc_tot,
+                                                          double* __restrict__ jpad) {
+  const int64_t k = blockIdx.x;
+  const int bt = blockIdx.y;
+  const int e = threadIdx.x;
+  const int64_t e0 = rp[k], e1 = rp[k + 1];
+  if (e0 == e1) return;
+  const Pair pr = pairs[k];
+  const ShellXYZ C = shells[pr.sa], D = shells[pr.sb];
+  const int wd = (D.l + 1) * (D.l + 2) / 2;
+  if (e >= (C.l + 1) * (C.l + 2) / 2 * wd) return;
+  const double* __restrict__ pb = part + (size_t)bt * stride + e;
+  double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int64_t q = e0;
+  for (; q + 8 <= e1; q += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] += pb[ent[q + u]];
+  }
+  for (int u = 0; q < e1; ++q, ++u) v[u] += pb[ent[q]];
+  const MolInfo mi = mols[C.mol];
+  jpad[8 * ((size_t)bt * ncc_tot + mi.base + (size_t)(C.ao + e / wd) * mi.n + D.ao + e % wd)] +=
+      ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+}
+
 __global__ void finalize_kernel(const double* __restrict__ jp, const double* __restrict__ jpad,
                                 const double* __restrict__ kp, int64_t ncc_tot,
                                 int64_t nn_tot, int nb, int sym, int with_j, int with_k,
@@ -1029,6 +1108,8 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
   }
   static const int pl[kNumPairClasses][2] = {{0, 0}, {1, 0}, {1, 1}, {2, 0}, {2, 1}, {2, 2}};
   S.nquartets = 0;
+  S.part_total = 0;
+  std::vector<std::pair<int64_t, int64_t>> kent, jent;  // (key, slab offset)
   for (int cb = 0; cb < kNumPairClasses; ++cb)
     for (int ck = 0; ck <= cb; ++ck)
       for (int pb = 0; pb < 4; ++pb)
@@ -1183,6 +1264,29 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             } else {
               for (BraChunk& c : qc.chunks) c.nket = 0;
             }
+            // partial slabs (segmented sets): each chunk's exchange rows
+            // [nbra][rows][n_m], then its ket window [nket][wc*wd]
+            if (!S.contracted && qc.nbra_max >= 1 && !qc.pb && !qc.pk) {
+              qc.part = true;
+              qc.part_beg = S.part_total;
+              for (BraChunk& c : qc.chunks) {
+                const int m = P.seg[S.pairs[c.bra0].sa].mol;
+                const int nm = P.mols[m].nc;
+                c.poff = S.part_total;
+                for (int sl = 0; sl < c.nbra; ++sl) {
+                  const Pair& pr = S.pairs[c.bra0 + sl];
+                  for (int r = 0; r < rows; ++r) {
+                    const int ao = r < wa ? P.seg[pr.sa].ao_cart + r : P.seg[pr.sb].ao_cart + (r - wa);
+                    kent.push_back({(int64_t)P.mols[m].aoc0 + ao, S.part_total + ((int64_t)sl * rows + r) * nm});
+                  }
+                }
+                S.part_total += (int64_t)c.nbra * rows * nm;
+                for (int w = 0; w < c.nket; ++w)
+                  jent.push_back({(int64_t)c.ket0 + w, S.part_total + (int64_t)w * wc * wd});
+                S.part_total += (int64_t)c.nket * wc * wd;
+              }
+              qc.part_end = S.part_total;
+            }
             if (qc.nbra_max >= 1) qc.d_chunks = dev_upload(qc.chunks, s);
           }
           HIP_OK(hipMalloc(&qc.d_quartets, sizeof(int2) * qc.nq));
@@ -1201,6 +1305,37 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
         }
   HIP_OK(hipFree(d_run_order));
   HIP_OK(hipFree(d_run_groups));
+  if (S.part_total * (int64_t)sizeof(double) > kPartCapBytes) {
+    S.part_total = 0;
+    for (QuartetClass& qc : S.classes) qc.part = false;
+  }
+  if (S.part_total > 0) {
+    // CSR lists: K by global Cartesian row, J by ket pair
+    auto csr = [&](std::vector<std::pair<int64_t, int64_t>>& ent, int64_t nkeys, int64_t*& d_rp, int64_t*& d_ent) {
+      std::vector<int64_t> rp(nkeys + 1, 0), off(std::max<size_t>(1, ent.size()));
+      for (const auto& e : ent) ++rp[e.first + 1];
+      for (int64_t k = 0; k < nkeys; ++k) rp[k + 1] += rp[k];
+      std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
+      for (const auto& e : ent) off[fill[e.first]++] = e.second;
+      d_rp = dev_upload(rp, s);
+      d_ent = dev_upload(off, s);
+    };
+    int64_t nrows = 0;
+    for (int m = 0; m < P.nmol; ++m) nrows = std::max<int64_t>(nrows, (int64_t)P.mols[m].aoc0 + P.mols[m].nc);
+    std::vector<int64_t> row_dst(nrows, 0);
+    std::vector<int> row_n(nrows, 0);
+    for (int m = 0; m < P.nmol; ++m)
+      for (int a = 0; a < P.mols[m].nc; ++a) {
+        row_dst[P.mols[m].aoc0 + a] = P.mols[m].cbase + (int64_t)a * P.mols[m].nc;
+        row_n[P.mols[m].aoc0 + a] = P.mols[m].nc;
+        S.nc_max = std::max(S.nc_max, P.mols[m].nc);
+      }
+    S.nrows = nrows;
+    S.d_row_dst = dev_upload(row_dst, s);
+    S.d_row_n = dev_upload(row_n, s);
+    csr(kent, nrows, S.d_krp, S.d_kent);
+    csr(jent, (int64_t)S.pairs.size(), S.d_jrp, S.d_jent);
+  }
   S.built = true;
 }
 
@@ -1213,6 +1348,7 @@ static void free_pair_set(PairSet& S) {
     f(S.d_prims);
   }
   f(S.d_coef); f(S.d_blk); f(S.d_pairs); f(S.d_pair_q);
+  f(S.d_krp); f(S.d_kent); f(S.d_jrp); f(S.d_jent); f(S.d_row_dst); f(S.d_row_n);
   for (auto& c : S.classes) {
     f(c.d_quartets);
     f(c.d_tasks);
@@ -1558,8 +1694,13 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
 // Device bytes a plan holds, with the J/K workspace for nbd densities and
 // quartet-list / chunk sizes as given (exact after build_plan).
 static int64_t plan_bytes(const PlanData& P, int64_t npairs, int64_t nq, int64_t ntasks, int64_t nchunks,
-                          int64_t nbd, int64_t extra) {
+                          int64_t nbd, int64_t extra, int64_t part_doubles) {
   int64_t b = 0;
+  // partial-sum slabs per density, their row lists (one entry per slab row
+  // and window entry at most) and the per-row tables
+  b += part_doubles * (int64_t)sizeof(double) * nbd;
+  b += std::min<int64_t>(part_doubles, nq * 48) * (int64_t)sizeof(int64_t) +
+       (int64_t)(P.n_cart + npairs + 2) * (int64_t)(2 * sizeof(int64_t) + sizeof(int));
   b += (int64_t)P.seg.size() * (sizeof(ShellXYZ) + 3 * sizeof(int));         // shells, seg_ao, seg_atom
   b += (int64_t)std::max<size_t>(P.prims.size(), 1) * (sizeof(PrimPair) + sizeof(PrimAux));
   b += (int64_t)(P.seg.size() + 1) * sizeof(int);
@@ -1590,7 +1731,7 @@ static int64_t plan_bytes(const PlanData& P, int64_t nbd) {
       extra += (int64_t)P.nbas * (sizeof(ShellXYZ) + 2 * sizeof(int)) +
                S->nprims * (int64_t)(sizeof(PrimPair) + sizeof(PrimCoef));
   }
-  return plan_bytes(P, np, nq, nt, nc, nbd, extra);
+  return plan_bytes(P, np, nq, nt, nc, nbd, extra, P.options.partial_sums ? P.jk.part_total : 0);
 }
 
 // Host-only upper bound of the device bytes of a plan plus one build of nb
@@ -1602,7 +1743,7 @@ static int64_t plan_bytes(const PlanData& P, int64_t nbd) {
 static int64_t forecast_bytes(PlanData& P, int nb) {
   build_segments(P);
   P.mol_pair0.assign(P.nmol + 1, 0);
-  int64_t nq = 0, npairs = 0;
+  int64_t nq = 0, npairs = 0, part = 0;
   for (int m = 0; m < P.nmol; ++m) {
     int64_t live = 0;
     for (int S = P.mol_seg[m]; S < P.mol_seg[m + 1]; ++S)
@@ -1616,7 +1757,10 @@ static int64_t forecast_bytes(PlanData& P, int nb) {
       }
     nq += live * (live + 1) / 2;
     npairs += live;
+    // slab rows: at most one bra slot (<= 12 rows of n_m) per quartet
+    part += live * (live + 1) / 2 * 12 * P.mols[m].nc;
   }
+  part = P.options.partial_sums ? std::min<int64_t>(part, kPartCapBytes / (int64_t)sizeof(double)) : 0;
   // sub-tasks pair a bra run with a ket run of one molecule: at most
   // (pair classes x column patterns x primitive-count buckets)^2 per molecule;
   // a contracted J/K set adds parent shells and at most as many primitive
@@ -1624,7 +1768,7 @@ static int64_t forecast_bytes(PlanData& P, int nb) {
   const int64_t runs = (int64_t)kNumPairClasses * 2 * 8;
   const int64_t extra = (int64_t)P.nbas * (sizeof(ShellXYZ) + 2 * sizeof(int)) +
                         (int64_t)P.prims.size() * (sizeof(PrimPair) + sizeof(PrimCoef));
-  return plan_bytes(P, npairs, nq, std::min<int64_t>(nq, runs * runs * P.nmol), nq, 2 * (int64_t)nb, extra);
+  return plan_bytes(P, npairs, nq, std::min<int64_t>(nq, runs * runs * P.nmol), nq, 2 * (int64_t)nb, extra, part);
 }
 
 static void free_plan(PlanData& P) {
@@ -1635,7 +1779,7 @@ static void free_plan(PlanData& P) {
   free_pair_set(P.seg_set);
   f(P.d_shells); f(P.d_prims); f(P.d_all_pairs);
   f(P.d_c2s_idx); f(P.d_c2s_coef); f(P.d_s2c_idx); f(P.d_s2c_coef);
-  f(P.d_dc); f(P.d_jp); f(P.d_jpad); f(P.d_kp); f(P.d_flag); f(P.d_incore);
+  f(P.d_dc); f(P.d_jp); f(P.d_jpad); f(P.d_kp); f(P.d_flag); f(P.d_incore); f(P.d_part);
   f(P.d_pairs1e); f(P.d_nuclei);
   f(P.d_dmax); f(P.d_seg_atom);
   f(P.d_molinfo); f(P.d_molao);
@@ -1665,6 +1809,15 @@ static void ensure_workspace(PlanData& P, size_t nbd) {
   P.ws_nbd = nbd;
 }
 
+// Partial slabs for nbd densities; a grown buffer is retired like the workspace.
+static void ensure_part(PlanData& P, size_t nbd) {
+  if (P.part_nbd >= nbd || P.jk.part_total == 0 || P.options.partial_sums == 0) return;
+  if (P.d_part) P.retired.push_back(P.d_part);
+  P.d_part = nullptr;
+  HIP_OK(hipMalloc(&P.d_part, sizeof(double) * (size_t)P.jk.part_total * nbd));
+  P.part_nbd = nbd;
+}
+
 static void jk_build(PlanData& P, const double* density, int nb, double* out, int with_j, int with_k,
                      uint32_t flags, double mixed_threshold, hipStream_t s) {
   const bool mixed = (flags & GSCF_FLAG_MIXED_PRECISION) != 0;
@@ -1685,6 +1838,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
   const bool sym = (flags & GSCF_FLAG_SYMMETRIC_DENSITY) != 0;
   const int nbd = (with_k && !sym) ? 2 * nb : nb;
   ensure_workspace(P, nbd);
+  ensure_part(P, nbd);
   hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, density, P.nn_tot, nb, P.d_dc,
                      P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, nbd);
   if (mixed)
@@ -1731,6 +1885,9 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     BraArgs b;
     b.mode = 0;
     b.cache = nullptr;
+    b.part_mode = 0;
+    b.part_stride = S.part_total;
+    b.part = P.d_part;
     if (P.d_incore && qc.cache_doubles) {
       b.mode = 2;
       b.cache = P.d_incore + qc.cache_off;
@@ -1766,7 +1923,21 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     size_t lds;
     int nket_max, tab_lds;
     layout(nbd_eff, nb, lds, nket_max, tab_lds);
-    if (qc.d_chunks && lds <= kLdsBudget) {
+    // partial slabs: a slab range this build does not write (ket windows not
+    // used, or the class not run by the bra-major kernel) is cleared first,
+    // since the reductions read every listed slab entry
+    const bool bra_path = qc.d_chunks && lds <= kLdsBudget;
+    if (qc.part) {
+      const int pm = P.options.partial_sums;
+      const bool need_k = with_k && (pm & 1), need_j = with_j && (pm & 2);
+      if (bra_path) b.part_mode = (need_k ? 1 : 0) | ((need_j && nket_max > 0) ? 2 : 0);
+      if ((need_k && !(b.part_mode & 1)) || (need_j && !(b.part_mode & 2)))
+        for (int bt = 0; bt < nbd_eff; ++bt)
+          hipLaunchKernelGGL(fill_kernel, dim3(nblocks((size_t)(qc.part_end - qc.part_beg))), dim3(256), 0, s,
+                             P.d_part + (size_t)bt * S.part_total + qc.part_beg,
+                             (size_t)(qc.part_end - qc.part_beg), 0.0);
+    }
+    if (bra_path) {
       b.dens = P.d_dc;
       b.jp = P.d_jp;
       b.jket_pad = P.d_jpad;
@@ -1828,6 +1999,17 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       HIP_OK(hipEventRecord(P.join_ev[l], P.side[l]));
       HIP_OK(hipStreamWaitEvent(s, P.join_ev[l], 0));
     }
+  }
+  if (S.part_total > 0) {
+    const int nbd_k = with_k ? nbd : nb;
+    if (with_k && (P.options.partial_sums & 1))
+      hipLaunchKernelGGL(kpart_reduce_kernel, dim3((unsigned)S.nrows, (unsigned)((S.nc_max + 63) / 64),
+                         (unsigned)nbd_k), dim3(64), 0, s, P.d_part, S.part_total, S.d_krp, S.d_kent,
+                         S.d_row_dst, S.d_row_n, P.ncc_tot, P.d_kp);
+    if (with_j && (P.options.partial_sums & 2))
+      hipLaunchKernelGGL(jpart_reduce_kernel, dim3((unsigned)S.pairs.size(), (unsigned)nb), dim3(64), 0, s,
+                         P.d_part, S.part_total, S.d_jrp, S.d_jent, S.d_pairs, S.d_shells, P.d_molinfo,
+                         P.ncc_tot, P.d_jpad);
   }
   hipLaunchKernelGGL(finalize_kernel, dim3(nblocks((size_t)(with_j + with_k) * nb * nn)), dim3(256), 0, s,
                      P.d_jp, P.d_jpad, P.d_kp, P.ncc_tot, P.nn_tot, nb, sym ? 1 : 0, with_j, with_k, P.d_molao,
@@ -2227,6 +2409,7 @@ int gscf_options_set(const gscf_options* o) {
     if (o->general_contraction != 0 && o->general_contraction != 1)
       throw InvalidArgument("general_contraction must be 0 or 1");
     if (o->streams < 1 || o->streams > kMaxStreams) throw InvalidArgument("streams must be in 1..4");
+    if (o->partial_sums < 0 || o->partial_sums > 3) throw InvalidArgument("partial_sums must be in 0..3");
     g_options = *o;
   });
 }

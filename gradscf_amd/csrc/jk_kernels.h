@@ -939,6 +939,22 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
                                                                       jket, ch.ket0, tab, cq);
   }
   __syncthreads();
+  if (a.part_mode & 1) {
+    // exchange rows to the chunk's partial slab with plain stores (every
+    // row, zeros included: the reduction reads the whole slab)
+    const int nm = mi.n;
+    const int cnt = a.nbd * ch.nbra * ROWS * nm;
+    double* __restrict__ dst = a.part + ch.poff;
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const int col = i % nm;
+      const int rs = i / nm;  // (bt, slot, row)
+      const int r = rs % ROWS;
+      const int slot = (rs / ROWS) % ch.nbra;
+      const int bt = rs / (ROWS * ch.nbra);
+      dst[(size_t)bt * a.part_stride + ((size_t)slot * ROWS + r) * nm + col] =
+          krow[((bt * a.nbra_max + slot) * ROWS + r) * ns + col];
+    }
+  } else
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
   for (int i = threadIdx.x; i < krow_size; i += blockDim.x) {
     const double v = krow[i];
@@ -967,7 +983,16 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     const int row = a.shells[bp.sa].ao + e / W::WB, col = a.shells[bp.sb].ao + e % W::WB;
     atomicAdd(a.jp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
   }
-  if (jket) {
+  if (jket && (a.part_mode & 2)) {
+    // ket window to the partial slab after the exchange rows
+    const int nk = a.nbj * ch.nket * W::NKET;
+    double* __restrict__ dst = a.part + ch.poff + (size_t)ch.nbra * ROWS * mi.n;
+    for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+      const int ew = i % (ch.nket * W::NKET);  // (window entry, element)
+      const int bt = i / (ch.nket * W::NKET);
+      dst[(size_t)bt * a.part_stride + ew] = jket[(size_t)bt * a.nket_max * W::NKET + ew];
+    }
+  } else if (jket) {
     const int nk = a.nbj * ch.nket * W::NKET;
     for (int i = threadIdx.x; i < nk; i += blockDim.x) {
       const int e = i % W::NKET;

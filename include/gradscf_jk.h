@@ -92,6 +92,9 @@ typedef struct {
                                   primitive quartet for both columns; 0 (default): as two segments */
   int32_t streams;             /* 1..4 HIP streams the class launches of one J/K build are
                                   spread over (fork/join on the caller's stream; default 4) */
+  int32_t partial_sums;        /* bit 0: a workgroup hands its exchange rows, bit 1 its ket-side
+                                  J window, to a reduction pass as plain stores into per-build
+                                  slabs instead of device-scope atomics (0..3; default 1) */
 } gscf_options;
 int gscf_options_set(const gscf_options* o);
 int gscf_options_get(gscf_options* o);

@@ -390,6 +390,34 @@ def test_many_nonsymmetric_densities_match_single_builds(with_j, with_k):
         assert float((both[:, b] - one[:, 0]).abs().max()) < 1e-12
 
 
+@pytest.mark.parametrize("pm", [0, 2, 3])
+def test_partial_sum_flushes_do_not_change_results(pm):
+    """(H2O)4 cc-pVDZ spherical, nonsymmetric densities: exchange rows and
+    ket-side J windows handed to the reduction pass as partial slabs (plain
+    stores) instead of flushed with device-scope atomics, for J and K
+    together and alone, evaluated and in-core, change J/K only by the order
+    of FP64 additions."""
+    import torch
+    from gradscf_amd import autotune as at
+    from gradscf_amd import molecule as M
+    _, _, atm, bas, env = systems.tables(M.water_cluster(4, seed=4, box=7.0), "cc-pvdz", cart=False)
+    ref_plan = _plan(atm, bas, env, False, 1e-12)
+    d = _cuda(systems.random_density(ref_plan.nao, nb=2, seed=6, symmetric=False))
+    refs = {(wj, wk): ref_plan.jk(d, with_j=wj, with_k=wk) for wj, wk in ((1, 1), (1, 0), (0, 1))}
+    before = at.get_options()
+    try:
+        at.set_options(partial_sums=pm)
+        plan = _plan(atm, bas, env, False, 1e-12)
+        for incore in (False, True):
+            if incore:
+                plan.incore(True)
+            for (wj, wk), ref in refs.items():
+                got = plan.jk(d, with_j=wj, with_k=wk)
+                assert float((got - ref).abs().max()) < 1e-11, (pm, incore, wj, wk)
+    finally:
+        at.set_options(**before)
+
+
 @pytest.mark.parametrize("gc,streams", [(1, 4), (1, 1), (0, 1)])
 def test_engine_options_do_not_change_results(gc, streams):
     """(H2O)4 cc-pVDZ spherical: native general contraction instead of

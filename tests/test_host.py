@@ -308,12 +308,15 @@ def test_workspace_forecast_without_gpu():
     topo, par = prepare_basis(M.water_cluster(4, seed=4, box=6.0), "6-31g*", cart=True)
     prov = NativeDirectBasis(make_plan(topo), par)
     one = native_jk_workspace_bytes(prov, 1, 1e-12)
+    two = native_jk_workspace_bytes(prov, 1, 1e-12, nb=2)
     four = native_jk_workspace_bytes(prov, 1, 1e-12, nb=4)
     n = topo.nao()
     assert one > 3 * 2 * n * n * 8
     # per extra density two more workspace slots (D and D^T for nonsymmetric K),
-    # each with Cartesian D, Jp, Kp and the ket-side J padded to one line per element
-    assert four - one == 3 * 2 * (3 + 8) * n * n * 8
+    # each with Cartesian D, Jp, Kp, the ket-side J padded to one line per
+    # element and the partial-sum slabs, linear in the density count
+    assert four - one == 3 * (two - one)
+    assert two - one >= 2 * (3 + 8) * n * n * 8
     assert native_jk_workspace_bytes(object(), 1, 0.0) is None
     with pytest.raises(ValueError):
         native_jk_workspace_bytes(prov, 0, 0.0)
@@ -345,14 +348,15 @@ def test_engine_options_api_without_gpu():
         pytest.skip("engine not built")
     from gradscf_amd import autotune as at
     before = at.get_options()
-    assert before == {"general_contraction": 0, "streams": 4}
+    assert before == {"general_contraction": 0, "streams": 4, "partial_sums": 1}
     try:
-        at.set_options(general_contraction=1, streams=2)
-        assert at.get_options() == {"general_contraction": 1, "streams": 2}
-        for bad in ({"streams": 0}, {"streams": 5}, {"general_contraction": 2}):
+        at.set_options(general_contraction=1, streams=2, partial_sums=1)
+        assert at.get_options() == {"general_contraction": 1, "streams": 2, "partial_sums": 1}
+        for bad in ({"streams": 0}, {"streams": 5}, {"general_contraction": 2}, {"partial_sums": 4},
+                    {"partial_sums": -1}):
             with pytest.raises(ValueError):
                 at.set_options(**bad)
-        assert at.get_options() == {"general_contraction": 1, "streams": 2}
+        assert at.get_options() == {"general_contraction": 1, "streams": 2, "partial_sums": 1}
         with pytest.raises(ValueError):
             at.set_options(threads=4)
     finally:
