@@ -1266,24 +1266,28 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             }
             // partial slabs (segmented sets): each chunk's exchange rows
             // [nbra][rows][n_m], then its ket window [nket][wc*wd]
-            if (!S.contracted && qc.nbra_max >= 1 && !qc.pb && !qc.pk) {
+            const int pmode = P.options.partial_sums;
+            if (pmode && !S.contracted && qc.nbra_max >= 1 && !qc.pb && !qc.pk) {
               qc.part = true;
               qc.part_beg = S.part_total;
               for (BraChunk& c : qc.chunks) {
                 const int m = P.seg[S.pairs[c.bra0].sa].mol;
                 const int nm = P.mols[m].nc;
                 c.poff = S.part_total;
-                for (int sl = 0; sl < c.nbra; ++sl) {
-                  const Pair& pr = S.pairs[c.bra0 + sl];
-                  for (int r = 0; r < rows; ++r) {
-                    const int ao = r < wa ? P.seg[pr.sa].ao_cart + r : P.seg[pr.sb].ao_cart + (r - wa);
-                    kent.push_back({(int64_t)P.mols[m].aoc0 + ao, S.part_total + ((int64_t)sl * rows + r) * nm});
+                if (pmode & 1)
+                  for (int sl = 0; sl < c.nbra; ++sl) {
+                    const Pair& pr = S.pairs[c.bra0 + sl];
+                    for (int r = 0; r < rows; ++r) {
+                      const int ao = r < wa ? P.seg[pr.sa].ao_cart + r : P.seg[pr.sb].ao_cart + (r - wa);
+                      kent.push_back({(int64_t)P.mols[m].aoc0 + ao, S.part_total + ((int64_t)sl * rows + r) * nm});
+                    }
                   }
-                }
                 S.part_total += (int64_t)c.nbra * rows * nm;
-                for (int w = 0; w < c.nket; ++w)
-                  jent.push_back({(int64_t)c.ket0 + w, S.part_total + (int64_t)w * wc * wd});
-                S.part_total += (int64_t)c.nket * wc * wd;
+                if (pmode & 2) {
+                  for (int w = 0; w < c.nket; ++w)
+                    jent.push_back({(int64_t)c.ket0 + w, S.part_total + (int64_t)w * wc * wd});
+                  S.part_total += (int64_t)c.nket * wc * wd;
+                }
               }
               qc.part_end = S.part_total;
             }
@@ -2311,6 +2315,203 @@ static void eri_blocks(PlanData& P, const int32_t* shls, int64_t nq, const int64
   run_blocks(P, P.d_all_pairs, tasks, cls, out, s);
 }
 
+// ----------------------------------------------------------------------------
+// Packed ERIs and packed J/K (the reference's compact-integral and packed J/K
+// targets, csrc/packed_eri.cc:5-14, csrc/compact.cc:5-17, csrc/packed_jk.cc:6-79)
+// ----------------------------------------------------------------------------
+struct PackTask {
+  int64_t off;   // first requested-basis element of the block (element enumeration)
+  int64_t coff;  // first element of the staged Cartesian block
+  int ao[4];     // first AO of each shell (requested basis) and its width
+  int d[4];
+  int cao[4];    // first Cartesian AO of each shell and its width
+  int cd[4];
+};
+
+__device__ __forceinline__ int64_t tri_row(int64_t idx) {
+  // largest r with r (r + 1) / 2 <= idx
+  int64_t r = (int64_t)((sqrt(8.0 * (double)idx + 1.0) - 1.0) * 0.5);
+  while (r * (r + 1) / 2 > idx) --r;
+  while ((r + 1) * (r + 2) / 2 <= idx) ++r;
+  return r;
+}
+
+// One thread per block element in the requested basis: unique AO quartets
+// (a >= b, c >= d, ab >= cd) go to the packed layout (0: pair matrix, both
+// triangles; 1: s8 triangle), transformed from the staged Cartesian block by
+// the plan's spherical-to-Cartesian map (up to 3 terms per AO; identity for
+// Cartesian plans).
+__global__ void pack_eri_kernel(const double* __restrict__ blocks, const PackTask* __restrict__ tasks,
+                                int64_t ntask, int64_t total, int layout, int64_t np,
+                                const int* __restrict__ s2c_idx, const double* __restrict__ s2c_coef,
+                                double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  int64_t lo = 0, hi = ntask;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (tasks[mid].off <= e) lo = mid; else hi = mid;
+  }
+  const PackTask t = tasks[lo];
+  int64_t r = e - t.off;  // libcint layout: first index fastest
+  const int p = (int)(r % t.d[0]);
+  r /= t.d[0];
+  const int q = (int)(r % t.d[1]);
+  r /= t.d[1];
+  const int u = (int)(r % t.d[2]);
+  const int w = (int)(r / t.d[2]);
+  const int64_t a = t.ao[0] + p, b = t.ao[1] + q, c = t.ao[2] + u, d = t.ao[3] + w;
+  if (a < b || c < d) return;
+  const int64_t ab = a * (a + 1) / 2 + b, cd = c * (c + 1) / 2 + d;
+  if (ab < cd) return;
+  const int64_t g[4] = {a, b, c, d};
+  int ci[4][3];
+  double cc[4][3];
+  for (int x = 0; x < 4; ++x)
+    for (int m = 0; m < 3; ++m) {
+      ci[x][m] = s2c_idx[g[x] * 3 + m] - t.cao[x];
+      cc[x][m] = s2c_coef[g[x] * 3 + m];
+    }
+  const double* __restrict__ blk = blocks + t.coff;
+  double v = 0.0;
+  for (int m3 = 0; m3 < 3; ++m3) {
+    if (cc[3][m3] == 0.0) continue;
+    for (int m2 = 0; m2 < 3; ++m2) {
+      if (cc[2][m2] == 0.0) continue;
+      for (int m1 = 0; m1 < 3; ++m1) {
+        if (cc[1][m1] == 0.0) continue;
+        const int64_t base = ci[1][m1] + (int64_t)t.cd[1] * (ci[2][m2] + (int64_t)t.cd[2] * ci[3][m3]);
+        const double f = cc[1][m1] * cc[2][m2] * cc[3][m3];
+        for (int m0 = 0; m0 < 3; ++m0)
+          if (cc[0][m0] != 0.0) v += cc[0][m0] * f * blk[ci[0][m0] + (int64_t)t.cd[0] * base];
+      }
+    }
+  }
+  if (layout == 0) {
+    out[ab * np + cd] = v;
+    out[cd * np + ab] = v;
+  } else {
+    out[ab * (ab + 1) / 2 + cd] = v;
+  }
+}
+
+// One thread per stored element (ab >= cd) of the s8 triangle or of the lower
+// triangle of the pair matrix: the 8-fold digestion of the reference's packed
+// J/K into out[op][b][n][n] with device-scope atomics.
+__global__ void packed_jk_kernel(const double* __restrict__ eri, int s8, int64_t np, const double* __restrict__ D,
+                                 int nb, int n, int with_j, int with_k, double* __restrict__ out) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= np * (np + 1) / 2) return;
+  const int64_t ab = tri_row(idx), cd = idx - ab * (ab + 1) / 2;
+  const double v = s8 ? eri[idx] : eri[ab * np + cd];
+  if (v == 0.0) return;
+  const int a = (int)tri_row(ab), b = (int)(ab - (int64_t)a * (a + 1) / 2);
+  const int c = (int)tri_row(cd), d = (int)(cd - (int64_t)c * (c + 1) / 2);
+  const size_t nn = (size_t)n * n;
+  for (int bt = 0; bt < nb; ++bt) {
+    const double* dm = D + (size_t)bt * nn;
+    if (with_j) {
+      double* j = out + (size_t)bt * nn;
+      const double jab = v * (dm[(size_t)c * n + d] + (c != d ? dm[(size_t)d * n + c] : 0.0));
+      atomicAdd(j + (size_t)a * n + b, jab);
+      if (a != b) atomicAdd(j + (size_t)b * n + a, jab);
+      if (ab != cd) {
+        const double jcd = v * (dm[(size_t)a * n + b] + (a != b ? dm[(size_t)b * n + a] : 0.0));
+        atomicAdd(j + (size_t)c * n + d, jcd);
+        if (c != d) atomicAdd(j + (size_t)d * n + c, jcd);
+      }
+    }
+    if (with_k) {
+      double* k = out + (size_t)(with_j * nb + bt) * nn;
+      atomicAdd(k + (size_t)a * n + c, v * dm[(size_t)b * n + d]);
+      if (c != d) atomicAdd(k + (size_t)a * n + d, v * dm[(size_t)b * n + c]);
+      if (a != b) atomicAdd(k + (size_t)b * n + c, v * dm[(size_t)a * n + d]);
+      if (a != b && c != d) atomicAdd(k + (size_t)b * n + d, v * dm[(size_t)a * n + c]);
+      if (ab != cd) {
+        atomicAdd(k + (size_t)c * n + a, v * dm[(size_t)d * n + b]);
+        if (a != b) atomicAdd(k + (size_t)c * n + b, v * dm[(size_t)d * n + a]);
+        if (c != d) atomicAdd(k + (size_t)d * n + a, v * dm[(size_t)c * n + b]);
+        if (a != b && c != d) atomicAdd(k + (size_t)d * n + b, v * dm[(size_t)c * n + a]);
+      }
+    }
+  }
+}
+
+// Every unique AO quartet of the plan's molecule (no screening), staged shell
+// quartet by shell quartet in the reference's visiting order (i, j <= i,
+// k <= i, l <= k; csrc/coulomb_shells.h:98-125) through the block kernels.
+static void packed_eri(PlanData& P, int layout, double* out, int64_t size, hipStream_t s) {
+  if (layout != 0 && layout != 1) throw Internal("Unknown compact integral layout");
+  if (P.nmol != 1) throw Internal("Packed ERIs need a single-molecule plan");
+  const int64_t n = P.n, np = n * (n + 1) / 2;
+  if (!out || size != (layout == 0 ? np * np : np * (np + 1) / 2)) throw Internal("Packed ERI shape mismatch");
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks((size_t)size)), dim3(256), 0, s, out, (size_t)size, 0.0);
+  constexpr int64_t kStage = int64_t(1) << 24;  // doubles of staged blocks per pass
+  double* stage = nullptr;
+  HIP_OK(hipMalloc(&stage, sizeof(double) * kStage));
+  std::vector<int32_t> shls;
+  std::vector<int64_t> offs;
+  std::vector<PackTask> tasks;
+  int64_t tot = 0, ctot = 0;  // requested-basis elements, staged Cartesian doubles
+  auto flush = [&]() {
+    if (tasks.empty()) return;
+    eri_blocks(P, shls.data(), (int64_t)tasks.size(), offs.data(), stage, s);
+    PackTask* d_tasks = dev_upload(tasks, s);
+    hipLaunchKernelGGL(pack_eri_kernel, dim3(nblocks((size_t)tot)), dim3(256), 0, s, stage, d_tasks,
+                       (int64_t)tasks.size(), tot, layout, np, P.d_s2c_idx, P.d_s2c_coef, out);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipFree(d_tasks));
+    shls.clear();
+    offs.clear();
+    tasks.clear();
+    tot = 0;
+    ctot = 0;
+  };
+  const int nbas = P.nbas;
+  for (int i = 0; i < nbas; ++i)
+    for (int j = 0; j <= i; ++j)
+      for (int k = 0; k <= i; ++k)
+        for (int l = 0; l <= k; ++l) {
+          const int sh[4] = {i, j, k, l};
+          PackTask t;
+          int64_t sz = 1, csz = 1;
+          for (int x = 0; x < 4; ++x) {
+            t.ao[x] = P.ao[sh[x]];
+            t.d[x] = P.ao[sh[x] + 1] - P.ao[sh[x]];
+            t.cao[x] = P.ao_cart[sh[x]];
+            t.cd[x] = P.ao_cart[sh[x] + 1] - P.ao_cart[sh[x]];
+            sz *= t.d[x];
+            csz *= t.cd[x];
+          }
+          if (ctot + csz > kStage) flush();
+          t.off = tot;
+          t.coff = ctot;
+          for (int x = 0; x < 4; ++x) shls.push_back(sh[x]);
+          offs.push_back(ctot);
+          tasks.push_back(t);
+          tot += sz;
+          ctot += csz;
+        }
+  flush();
+  HIP_OK(hipFree(stage));
+}
+
+static void packed_jk(const double* eri, int64_t eri_size, int ndim, const double* density, int nb, int n,
+                      double* out, int with_j, int with_k, hipStream_t s) {
+  if ((with_j != 0 && with_j != 1) || (with_k != 0 && with_k != 1) || !(with_j || with_k))
+    throw Internal("Packed J/K requires at least one selected operator");
+  const int64_t np = (int64_t)n * (n + 1) / 2;
+  if (n < 1 || nb < 1 || (ndim != 1 && ndim != 2) || eri_size != (ndim == 1 ? np * (np + 1) / 2 : np * np))
+    throw Internal("Packed J/K dimensions do not match");
+  if (!eri || !density || !out) throw Internal("Packed J/K requires device buffers");
+  const size_t total = (size_t)(with_j + with_k) * nb * n * n;
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(total)), dim3(256), 0, s, out, total, 0.0);
+  hipLaunchKernelGGL(packed_jk_kernel, dim3(nblocks((size_t)(np * (np + 1) / 2))), dim3(256), 0, s, eri,
+                     ndim == 1 ? 1 : 0, np, density, nb, n, with_j, with_k, out);
+  HIP_OK(hipGetLastError());
+}
+
 }  // namespace gscf
 
 // ----------------------------------------------------------------------------
@@ -2508,6 +2709,27 @@ int gscf_plan_get_schwarz(const gscf_plan* plan, double* q) {
     if (P.parent_q.empty()) throw InvalidArgument("Schwarz bounds are computed only for a positive cutoff");
     std::copy(P.parent_q.begin(), P.parent_q.end(), q);
   });
+}
+
+int gscf_packed_eri(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                    int32_t nenv, int32_t cart, int32_t layout, double* out, int64_t size, void* stream) {
+  // every failure is Internal, as in the reference's compact target (compact.cc:16)
+  gscf_plan* plan = nullptr;
+  const int rc = gscf_plan_create(atm, natm, bas, nbas, env, nenv, cart, 0.0, stream, &plan);
+  if (rc != GSCF_OK) return GSCF_INTERNAL;
+  const int rc2 = guarded([&] { packed_eri(plan->d, layout, out, size, (hipStream_t)stream); });
+  if (rc2 != GSCF_OK) {
+    const std::string msg = g_error;
+    (void)gscf_plan_destroy(plan);
+    g_error = msg;
+    return GSCF_INTERNAL;
+  }
+  return gscf_plan_destroy(plan);
+}
+
+int gscf_packed_jk(const double* eri, int64_t eri_size, int32_t ndim, const double* density, int32_t nb, int32_t n,
+                   double* out, int32_t with_j, int32_t with_k, void* stream) {
+  return guarded([&] { packed_jk(eri, eri_size, ndim, density, nb, n, out, with_j, with_k, (hipStream_t)stream); });
 }
 
 int gscf_plan_incore_bytes(const gscf_plan* plan, int64_t* bytes) {

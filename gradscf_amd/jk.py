@@ -383,6 +383,63 @@ def direct_jk(atm, bas, env, density, *, cart, cutoff=0.0, workers=1, with_j=Tru
             out[-1] if with_k else torch.zeros_like(out[0]))
 
 
+def packed_eri_shape(nao: int, ndim: int):
+    """Shape of packed ERIs (molecular/eri.py:21-23): the s8 triangle
+    (ndim 1) or the np x np pair matrix (ndim 2), np = nao (nao + 1) / 2."""
+    npair = nao * (nao + 1) // 2
+    return (npair * (npair + 1) // 2,) if ndim == 1 else (npair, npair)
+
+
+def packed_eri(atm, bas, env, *, cart: bool, aosym: str = "s8") -> torch.Tensor:
+    """Every unique (ab|cd) of one molecule, unscreened, on the GPU: the
+    reference's compact integrals (csrc/packed_eri.cc), aosym "s8" (triangle,
+    layout 1) or "s4" (pair matrix, layout 0)."""
+    if aosym not in ("s8", "s4"):
+        raise ValueError("aosym must be 's8' or 's4'")
+    atm = np.ascontiguousarray(atm, dtype=np.int32)
+    bas = np.ascontiguousarray(bas, dtype=np.int32)
+    env = np.ascontiguousarray(env, dtype=np.float64)
+    if atm.ndim != 2 or atm.shape[1] != 6 or bas.ndim != 2 or bas.shape[1] != 8:
+        raise ValueError("Invalid libcint table dimensions or operator")
+    L = bas[:, 1]
+    nctr = bas[:, 3]
+    n = int(np.sum(nctr * (((L + 1) * (L + 2)) // 2 if cart else 2 * L + 1)))
+    shape = packed_eri_shape(n, 1 if aosym == "s8" else 2)
+    out = torch.empty(shape, dtype=torch.float64, device=_device())
+    rc = _native.lib().gscf_packed_eri(atm.ctypes.data, len(atm), bas.ctypes.data, len(bas), env.ctypes.data,
+                                       len(env), int(bool(cart)), 1 if aosym == "s8" else 0, out.data_ptr(),
+                                       out.numel(), _stream_handle())
+    _native.check(rc)
+    return out
+
+
+def packed_jk(eri, density, *, with_j=True, with_k=True):
+    """J/K from s8 packed ERIs (backends/native/jk.py:522-546): density
+    (..., n, n) float64 or complex128 on the GPU -> (J, K) shaped like it;
+    zeros for an unselected operator."""
+    if eri.dim() != 1:
+        raise ValueError("The native packed J/K operator requires s8 ERIs.")
+    _validate_selection(with_j, with_k)
+    if not (with_j or with_k):
+        return torch.zeros_like(density), torch.zeros_like(density)
+    n = density.shape[-1]
+    flat = density.reshape(-1, n, n)
+    count = flat.shape[0]
+    real = torch.cat((flat.real, flat.imag), 0) if flat.is_complex() else flat
+    real = real.to(torch.float64).contiguous()
+    eri = eri.to(torch.float64).contiguous()
+    nops = int(bool(with_j)) + int(bool(with_k))
+    out = torch.empty((nops,) + tuple(real.shape), dtype=torch.float64, device=real.device)
+    _native.check(_native.lib().gscf_packed_jk(eri.data_ptr(), eri.numel(), 1, real.data_ptr(), real.shape[0], n,
+                                               out.data_ptr(), int(bool(with_j)), int(bool(with_k)),
+                                               _stream_handle()))
+    if flat.is_complex():
+        out = out[:, :count] + 1j * out[:, count:]
+    out = out.reshape(nops, *density.shape)
+    return (out[0] if with_j else torch.zeros_like(density),
+            out[-1] if with_k else torch.zeros_like(density))
+
+
 @dataclass(frozen=True)
 class IntegralPlan:
     """Plan over a basis topology (the native backend only)."""

@@ -160,6 +160,27 @@ int gscf_plan_get_schwarz(const gscf_plan* plan, double* q);
  * allocate; enabling fails with InvalidArgument when they cannot be
  * allocated.  Enabling synchronises `stream` (not graph-capturable); builds
  * afterwards are capturable as before. */
+/* Packed ERIs of every unique AO quartet, unscreened, for one molecule: the
+ * reference's compact-integral target GradSCFCompact with layout 0 or 1
+ * (csrc/compact.cc:5-21 -> packed_eri, csrc/packed_eri.cc:5-14).  out is a
+ * DEVICE buffer of `size` doubles: layout 1 the s8 triangle, np (np + 1) / 2
+ * values at pair_index(ab, cd), ab >= cd; layout 0 the symmetric np x np pair
+ * matrix; np = n (n + 1) / 2, ab = pair_index(a, b) = a (a + 1) / 2 + b for
+ * a >= b.  Cartesian or spherical AOs per `cart`.  Synchronises `stream`.
+ * Every failure (tables included) is GSCF_INTERNAL, as in the reference. */
+int gscf_packed_eri(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas,
+                    const double* env, int32_t nenv, int32_t cart, int32_t layout, double* out,
+                    int64_t size, void* stream);
+
+/* J/K from packed ERIs: the reference's GradSCFPackedJK
+ * (csrc/packed_jk.cc:6-85): eri F64 s8 triangle (ndim 1) or np x np pair
+ * matrix (ndim 2, lower triangle read), density F64[nb,n,n] (any symmetry)
+ * -> out F64[nops,nb,n,n], nops = with_j + with_k (J first).  All DEVICE
+ * pointers; asynchronous on `stream`.  Errors are GSCF_INTERNAL with the
+ * reference's messages. */
+int gscf_packed_jk(const double* eri, int64_t eri_size, int32_t ndim, const double* density,
+                   int32_t nb, int32_t n, double* out, int32_t with_j, int32_t with_k, void* stream);
+
 int gscf_plan_incore_bytes(const gscf_plan* plan, int64_t* bytes);
 int gscf_plan_incore(gscf_plan* plan, int32_t enable, void* stream);
 
@@ -191,7 +212,7 @@ int gscf_direct_jk(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t
 
 /* Contracted ERI blocks for an explicit list of shell quartets (shls int32[nq,4],
  * host), written in libcint layout (first index fastest) at out + offsets[q]
- * (device).  Cartesian or spherical per the plan.  The reference's
+ * (device).  Cartesian components, whatever the plan's basis.  The reference's
  * int2e_cart/int2e_sph shell-block call (csrc/coulomb_shells.h:32-58). */
 int gscf_eri_blocks(gscf_plan* plan, const int32_t* shls, int64_t nq, const int64_t* offsets,
                     double* out, void* stream);
