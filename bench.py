@@ -77,12 +77,18 @@ TUNING_DIR = os.path.join(ROOT, "gradscf_amd", "tuning")
 
 
 def _kernel_class(name: str):
-    """'void gscf::jk_bra_kernel<2, 1, 1, 0>(gscf::BraArgs)' -> ('jk_bra_kernel', 'dpps')."""
+    """'void gscf::jk_bra_kernel<2, 1, 1, 0, ...>(gscf::BraArgs)' -> ('jk_bra_kernel', 'dpps');
+    the in-core variants of the bra kernel (last template argument 1: evaluate
+    and store, 2: digest stored blocks) get '/store' or '/load'."""
     import re
-    m = re.search(r"(\w+)<(\d), (\d), (\d), (\d)(?:, \w+)*>", name)
+    m = re.search(r"(\w+)<(\d), (\d), (\d), (\d)((?:, \w+)*)>", name)
     if not m:
         return None, None
-    return m.group(1), "".join(CLASS_LETTERS[int(v)] for v in m.groups()[1:5])
+    cls = "".join(CLASS_LETTERS[int(v)] for v in m.groups()[1:5])
+    rest = [x.strip() for x in m.group(6).split(",") if x.strip()]
+    if m.group(1) == "jk_bra_kernel" and len(rest) == 7 and rest[-1] in ("1", "2"):
+        cls += "/store" if rest[-1] == "1" else "/load"
+    return m.group(1), cls
 
 
 def copy_calibration() -> int:
@@ -153,7 +159,9 @@ def summarize_profile(paths) -> int:
                 rows = list(csv.DictReader(open(f)))
                 print(f"{'kernel':72s} {'calls':>6s} {'total_ms':>9s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'pct':>6s}")
                 for r in rows:
-                    print(f"{r['Name'][:72]:72s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:9.3f} "
+                    kind, cls = _kernel_class(r["Name"])
+                    label = f"{kind}[{cls}]" if kind else r["Name"][:72]
+                    print(f"{label:72s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:9.3f} "
                           f"{float(r['AverageNs']) / 1e3:9.2f} {float(r['MinNs']) / 1e3:9.2f} "
                           f"{float(r['MaxNs']) / 1e3:9.2f} {float(r['Percentage']):6.2f}")
             elif f.endswith("counter_collection.csv"):
