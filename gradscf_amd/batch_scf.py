@@ -9,9 +9,11 @@ schedule:
   * one packed J/K call per SCF iteration covers every molecule (a batched
     plan: one launch per quartet class for the whole batch), optionally
     replayed from a captured HIP graph;
-  * molecules with the same (AO count, occupied count) form a group whose
-    diagonalisation, DIIS extrapolation and convergence tests run as batched
-    FP64 tensor operations;
+  * every molecule's matrices sit in one (nmol, N, N) batch padded to the
+    largest AO count N (identity overlap and a lifted diagonal in the
+    padding, masks for occupied and virtual orbitals), so diagonalisation,
+    DIIS extrapolation and convergence tests run as single batched FP64
+    tensor operations over all molecules;
   * a molecule that has converged is frozen (its density change is zero from
     then on), so its energy is the one an independent run would stop at.
 
@@ -42,20 +44,33 @@ class BatchRHFResult:
     timings: dict = field(default_factory=dict)
 
 
-class _Group:
-    """Molecules sharing (n, nocc): batched tensors (k, n, n) and gather indices."""
+class _Padded:
+    """Every molecule's n_m x n_m matrices in one (nmol, N, N) batch, N the
+    largest AO count, zeros outside each molecule's block; gather / scatter
+    between that batch and the packed layout of a batched plan."""
 
-    def __init__(self, mols, n, nocc, offsets, device):
-        self.mols = np.asarray(mols)
-        self.k, self.n, self.nocc = len(mols), n, nocc
-        base = torch.as_tensor(np.asarray(offsets)[self.mols], dtype=torch.int64, device=device)
-        self.idx = (base[:, None] + torch.arange(n * n, device=device)[None, :]).reshape(-1)
+    def __init__(self, mol_n, offsets, device):
+        n = np.asarray(mol_n, dtype=np.int64)
+        self.k, self.N = len(n), int(n.max())
+        src, dst = [], []
+        for m, (nm, off) in enumerate(zip(n, np.asarray(offsets, dtype=np.int64))):
+            ii, jj = np.meshgrid(np.arange(nm), np.arange(nm), indexing="ij")
+            src.append(off + (ii * nm + jj).ravel())
+            dst.append(m * self.N * self.N + (ii * self.N + jj).ravel())
+        self.src = torch.as_tensor(np.concatenate(src), device=device)
+        self.dst = torch.as_tensor(np.concatenate(dst), device=device)
+        ar = torch.arange(self.N, device=device)
+        nt = torch.as_tensor(n, device=device)
+        self.valid = (ar[None, :] < nt[:, None]).to(torch.float64)      # (k, N) AO rows of each molecule
+        self.n = nt.to(torch.float64)
 
     def gather(self, packed):
-        return packed[self.idx].view(self.k, self.n, self.n)
+        out = torch.zeros(self.k * self.N * self.N, dtype=packed.dtype, device=packed.device)
+        out[self.dst] = packed[self.src]
+        return out.view(self.k, self.N, self.N)
 
     def scatter(self, packed, mats):
-        packed[self.idx] = mats.reshape(-1)
+        packed[self.src] = mats.reshape(-1)[self.dst]
 
 
 def _orthogonalizer(s, eps=1e-10):
@@ -63,9 +78,13 @@ def _orthogonalizer(s, eps=1e-10):
     return (v * torch.clamp(w, min=eps).rsqrt()[:, None, :]) @ v.transpose(1, 2)
 
 
-def _diagonalize(f, x):
+def _diagonalize(f, x, pad):
+    """Eigenpairs of x^T f x; pad (k, N) marks padding rows, whose diagonal is
+    lifted above every eigenvalue so that each molecule's own eigenpairs come
+    first (the off-diagonal padding blocks are zero)."""
     fo = x.transpose(1, 2) @ f @ x
     fo = 0.5 * (fo + fo.transpose(1, 2))
+    fo = fo + torch.diag_embed(pad * (fo.abs().sum(dim=-1).amax() + 1.0))
     e, c = eigh(fo)
     return e, x @ c
 
@@ -86,7 +105,7 @@ def _diis_solve(fock_hist, gram, count):
     b[:, m, :m] = -valid
     rhs = torch.zeros((k, m + 1), dtype=dt, device=dev)
     rhs[:, m] = -1.0
-    coeff = torch.linalg.solve(b, rhs)[:, :m] * valid
+    coeff = torch.linalg.solve_ex(b, rhs)[0][:, :m] * valid
     k_, m_, n_ = fock_hist.shape[0], fock_hist.shape[1], fock_hist.shape[2]
     return torch.bmm(coeff[:, None, :], fock_hist.reshape(k_, m_, n_ * n_)).reshape(k_, n_, n_)
 
@@ -151,11 +170,16 @@ def _rhf_batch(plan, nelectrons, enucs, conv_tol, conv_tol_density, conv_tol_gra
     s, t, v = plan.int1e()
     h = t + v
     dev = h.device
-    enucs = enucs.to(dev)
-    keys = {}
-    for m in range(nmol):
-        keys.setdefault((int(plan.mol_n[m]), int(nelectrons[m]) // 2), []).append(m)
-    groups = [_Group(ms, n, nocc, plan.mol_offset, dev) for (n, nocc), ms in sorted(keys.items())]
+    enuc = enucs.to(dev)
+    B = _Padded(plan.mol_n, plan.mol_offset, dev)
+    k, N = B.k, B.N
+    ar = torch.arange(N, device=dev)
+    nocc = torch.as_tensor(nelectrons // 2, device=dev)
+    occ = (ar[None, :] < nocc[:, None]).to(torch.float64)          # (k, N) occupied MO columns
+    vir = B.valid - occ                                             # virtual MO columns
+    pad = 1.0 - B.valid
+    S = B.gather(s) + torch.diag_embed(pad)                         # identity in the padding
+    H = B.gather(h)
     torch.cuda.synchronize()
     t_int = time.perf_counter()
 
@@ -169,101 +193,74 @@ def _rhf_batch(plan, nelectrons, enucs, conv_tol, conv_tol_density, conv_tol_gra
             out = plan.jk(packed[None], symmetric=True, validate=False, mixed_threshold=mixed_threshold)
             return out[0, 0], out[1, 0]
 
-    state = []
+    X = _orthogonalizer(S)
+    _, C0 = _diagonalize(H, X, pad)
+    Co = C0 * occ[:, None, :]
+    D = 2.0 * Co @ Co.transpose(1, 2)
     d_packed = torch.zeros(plan.nn_total, dtype=torch.float64, device=dev)
-    for g in groups:
-        S, H = g.gather(s), g.gather(h)
-        X = _orthogonalizer(S)
-        _, C = _diagonalize(H, X)
-        Co = C[:, :, :g.nocc]
-        D = 2.0 * Co @ Co.transpose(1, 2)
-        g.scatter(d_packed, D)
-        state.append({"S": S, "H": H, "X": X, "C0": C, "D": D,
-                      "fh": torch.zeros((g.k, DIIS_SPACE, g.n, g.n), dtype=torch.float64, device=dev),
-                      "eh": torch.zeros((g.k, DIIS_SPACE, g.n * g.n), dtype=torch.float64, device=dev),
-                      "gram": torch.zeros((g.k, DIIS_SPACE, DIIS_SPACE), dtype=torch.float64, device=dev),
-                      "energy": torch.zeros(g.k, dtype=torch.float64, device=dev),
-                      "active": torch.ones(g.k, dtype=torch.bool, device=dev),
-                      "cycles": np.zeros(g.k, dtype=np.int64),
-                      "enuc": enucs[torch.as_tensor(g.mols, device=dev)]})
+    B.scatter(d_packed, D)
+    fh = torch.zeros((k, DIIS_SPACE, N, N), dtype=torch.float64, device=dev)
+    eh = torch.zeros((k, DIIS_SPACE, N * N), dtype=torch.float64, device=dev)
+    gram = torch.zeros((k, DIIS_SPACE, DIIS_SPACE), dtype=torch.float64, device=dev)
+    energy = torch.zeros(k, dtype=torch.float64, device=dev)
+    active = torch.ones(k, dtype=torch.bool, device=dev)
+    cycles = np.zeros(k, dtype=np.int64)
     j_packed, k_packed = jk(d_packed)
-    j_packed, k_packed = j_packed.clone(), k_packed.clone()
-    for g, st in zip(groups, state):
-        st["J"], st["K"] = g.gather(j_packed), g.gather(k_packed)
-        st["F"] = st["H"] + st["J"] - 0.5 * st["K"]
+    J, K = B.gather(j_packed), B.gather(k_packed)
+    F = H + J - 0.5 * K
     head = count = 0
     iteration_s = []
     n_jk = 1
     torch.cuda.synchronize()
     for cycle in range(max_cycle):
         ti = time.perf_counter()
+        f_eff = F
+        if cycle >= 1:
+            err = C0.transpose(1, 2) @ (F @ D @ S - S @ D @ F) @ C0
+            fh[:, head] = F
+            eh[:, head] = err.reshape(k, -1)
+            row = torch.bmm(eh, eh[:, head, :, None])[:, :, 0]
+            gram[:, head, :] = row
+            gram[:, :, head] = row
+        if cycle >= 1 and min(count + 1, DIIS_SPACE) >= 2:
+            f_eff = _diis_solve(fh, gram, min(count + 1, DIIS_SPACE))
+        _, C = _diagonalize(f_eff, X, pad)
+        Co = C * occ[:, None, :]
+        D_new = 2.0 * Co @ Co.transpose(1, 2)
+        D_new = torch.where(active[:, None, None], D_new, D)
         dd_packed = torch.zeros_like(d_packed)
-        d_new_all = []
-        # every group's DIIS step is queued before the first diagonalisation,
-        # whose host transfer or error check synchronises with the device
-        f_effs = []
-        for g, st in zip(groups, state):
-            F, D, S, C0 = st["F"], st["D"], st["S"], st["C0"]
-            f_eff = F
-            if cycle >= 1:
-                err = C0.transpose(1, 2) @ (F @ D @ S - S @ D @ F) @ C0
-                st["fh"][:, head] = F
-                st["eh"][:, head] = err.reshape(g.k, -1)
-                row = torch.bmm(st["eh"], st["eh"][:, head, :, None])[:, :, 0]
-                st["gram"][:, head, :] = row
-                st["gram"][:, :, head] = row
-            if cycle >= 1 and min(count + 1, DIIS_SPACE) >= 2:
-                f_eff = _diis_solve(st["fh"], st["gram"], min(count + 1, DIIS_SPACE))
-            f_effs.append(f_eff)
-        for (g, st), f_eff in zip(zip(groups, state), f_effs):
-            D = st["D"]
-            mo_e, C = _diagonalize(f_eff, st["X"])
-            Co = C[:, :, :g.nocc]
-            D_new = 2.0 * Co @ Co.transpose(1, 2)
-            D_new = torch.where(st["active"][:, None, None], D_new, D)
-            g.scatter(dd_packed, D_new - D)
-            d_new_all.append((D_new, C))
+        B.scatter(dd_packed, D_new - D)
         if cycle >= 1:
             head = (head + 1) % DIIS_SPACE
             count = min(count + 1, DIIS_SPACE)
         dj_packed, dk_packed = jk(dd_packed)
         n_jk += 1
-        flags = []
-        for (g, st), (D_new, C) in zip(zip(groups, state), d_new_all):
-            st["J"] = st["J"] + g.gather(dj_packed)
-            st["K"] = st["K"] + g.gather(dk_packed)
-            F_new = st["H"] + st["J"] - 0.5 * st["K"]
-            e_new = (D_new * (st["H"] + 0.5 * st["J"] - 0.25 * st["K"])).sum(dim=(1, 2)) + st["enuc"]
-            rms = torch.sqrt(torch.mean((D_new - st["D"]).reshape(g.k, -1) ** 2, dim=1))
-            grad = torch.linalg.norm((2.0 * C[:, :, g.nocc:].transpose(1, 2) @ F_new @ C[:, :, :g.nocc])
-                                     .reshape(g.k, -1), dim=1)
-            done = ((torch.abs(e_new - st["energy"]) < conv_tol) & (rms < conv_tol_density)
-                    & (grad < conv_tol_grad) & (cycle > 0))
-            act = st["active"]
-            st["energy"] = torch.where(act, e_new, st["energy"])
-            st["D"] = D_new
-            st["F"] = torch.where(act[:, None, None], F_new, st["F"])
-            st["active"] = act & ~done
-            flags.append((act, st["active"]))
+        J = J + B.gather(dj_packed)
+        K = K + B.gather(dk_packed)
+        F_new = H + J - 0.5 * K
+        e_new = (D_new * (H + 0.5 * J - 0.25 * K)).sum(dim=(1, 2)) + enuc
+        rms = torch.sqrt(((D_new - D) ** 2).sum(dim=(1, 2)) / (B.n * B.n))
+        grad = torch.linalg.norm((2.0 * (C * vir[:, None, :]).transpose(1, 2) @ F_new @ Co).reshape(k, -1), dim=1)
+        done = ((torch.abs(e_new - energy) < conv_tol) & (rms < conv_tol_density)
+                & (grad < conv_tol_grad) & (cycle > 0))
+        energy = torch.where(active, e_new, energy)
+        D = D_new
+        F = torch.where(active[:, None, None], F_new, F)
+        was = active
+        active = active & ~done
         # one host synchronisation per iteration: which molecules are still running
-        still = torch.cat([a for _, a in flags]).any()
-        was = [a.cpu().numpy() for a, _ in flags]
-        for (g, st), w in zip(zip(groups, state), was):
-            st["cycles"][w] = cycle + 1
+        was_h = was.cpu().numpy()
+        cycles[was_h] = cycle + 1
         iteration_s.append(time.perf_counter() - ti)
-        if not bool(still):
+        if not bool(active.any()):
             break
     torch.cuda.synchronize()
     t_end = time.perf_counter()
-    energies = np.zeros(nmol)
-    converged = np.zeros(nmol, dtype=bool)
-    cycles = np.zeros(nmol, dtype=np.int64)
+    energies = energy.cpu().numpy()
+    converged = ~active.cpu().numpy()
     d_final = torch.zeros_like(d_packed)
-    for g, st in zip(groups, state):
-        energies[g.mols] = st["energy"].cpu().numpy()
-        converged[g.mols] = ~st["active"].cpu().numpy()
-        cycles[g.mols] = st["cycles"]
-        g.scatter(d_final, st["D"])
+    B.scatter(d_final, D)
+    groups = sorted({(int(n), int(o)) for n, o in zip(plan.mol_n, nelectrons // 2)})
     return BatchRHFResult(energies, converged, cycles, d_final, iteration_s,
                           {"integrals_s": t_int - t0, "scf_s": t_end - t_int, "jk_builds": n_jk,
-                           "groups": [(g.n, g.nocc, g.k) for g in groups]})
+                           "padded_order": N, "shapes": groups})

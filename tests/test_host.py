@@ -194,17 +194,27 @@ def test_boys_moment_quadrature_against_mpmath():
 
 
 def test_eigh_placement_and_host_path():
-    """Small eigen-problems go to host LAPACK, large ones stay on the GPU; the
-    host path returns the same decomposition as torch's."""
+    """Small single eigen-problems go to host LAPACK, batches and large ones
+    stay on the GPU; the host path returns the same decomposition as torch's,
+    and the padded form of small batches the same eigenpairs."""
     import torch
     from gradscf_amd import linalg
     assert linalg.on_host((13, 13)) and linalg.on_host((384, 384)) and not linalg.on_host((608, 608))
-    assert linalg.on_host((32, 57, 57)) and not linalg.on_host((32, 76, 76))
+    assert not linalg.on_host((32, 57, 57)) and not linalg.on_host((32, 76, 76))
     assert linalg.on_host((1, 384, 384))
     a = torch.randn(6, 6, dtype=torch.float64, generator=torch.Generator().manual_seed(1))
     a = a + a.T
     e, v = linalg.eigh(a)
     assert torch.allclose(v @ torch.diag(e) @ v.T, a, atol=1e-12)
+    b = torch.randn(5, 19, 19, dtype=torch.float64, generator=torch.Generator().manual_seed(2))
+    b = b + b.transpose(1, 2)
+    ep, vp = linalg.eigh_padded(b, 65)
+    e0, v0 = torch.linalg.eigh(b)
+    assert torch.allclose(ep, e0, atol=1e-12)
+    # eigenvectors up to sign: the reconstruction and the spectral projectors
+    assert torch.allclose(vp @ torch.diag_embed(ep) @ vp.transpose(1, 2), b, atol=1e-11)
+    assert torch.allclose(vp[..., :7] @ vp[..., :7].transpose(1, 2), v0[..., :7] @ v0[..., :7].transpose(1, 2),
+                          atol=1e-10)
 
 
 def test_library_exports_every_declared_symbol():
