@@ -2668,7 +2668,7 @@ int gscf_plan_get_info(const gscf_plan* plan, gscf_plan_info* info) {
     info->nmol = P.nmol;
     info->nc_max = P.nc_max;
     info->nn_total = P.nn_tot;
-    info->device_bytes = plan_bytes(P, (int64_t)P.ws_nbd);
+    info->device_bytes = plan_bytes(P, (int64_t)P.ws_nbd) + (int64_t)(sizeof(double) * P.incore_doubles);
   });
 }
 

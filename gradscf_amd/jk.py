@@ -200,6 +200,12 @@ class DirectJKPlan:
             o += c * c
         return out
 
+    def device_bytes(self) -> int:
+        """Device memory the plan holds now (tables, workspace, in-core integrals)."""
+        info = _native.PlanInfo()
+        _native.check(_native.lib().gscf_plan_get_info(self._handle, ctypes.byref(info)))
+        return int(info.device_bytes)
+
     def incore_bytes(self) -> int:
         """Device bytes that keeping the plan's integrals in core would take."""
         b = ctypes.c_int64()

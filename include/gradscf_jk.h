@@ -64,7 +64,8 @@ typedef struct {
   int32_t nmol;         /* molecules (1 unless built by gscf_plan_create_batch) */
   int32_t nc_max;       /* Cartesian AO count of the widest molecule */
   int64_t nn_total;     /* elements of one packed density: sum over molecules of n_m^2 */
-  int64_t device_bytes; /* device memory held by the plan and its current workspace */
+  int64_t device_bytes; /* device memory held by the plan, its current workspace and
+                           in-core integrals */
 } gscf_plan_info;
 
 /* Device-memory forecast before any GPU work (the reference's

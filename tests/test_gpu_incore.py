@@ -30,8 +30,10 @@ def test_incore_matches_evaluating_build_and_oracle(oracle_mod, basis, cart, cut
     ref_direct = plan.jk(_cuda(d)).cpu().numpy()
     nbytes = plan.incore_bytes()
     assert nbytes > 0 and nbytes % 8 == 0
+    held = plan.device_bytes()
     plan.incore(True)
     plan.incore(True)  # idempotent
+    assert plan.device_bytes() == held + nbytes
     got = plan.jk(_cuda(d)).cpu().numpy()
     assert np.max(np.abs(got - ref_direct)) < 1e-12
     ref = oracle_mod.direct_jk(atm, bas, env, d, cart=cart, cutoff=cutoff, workers=2)
