@@ -209,6 +209,8 @@ struct PairSet {
   int64_t* d_kent = nullptr;
   int64_t* d_jrp = nullptr;
   int64_t* d_jent = nullptr;
+  std::vector<std::pair<int64_t, int64_t>> h_kent;  // (row, slab offset) of every exchange-row slab row
+  std::vector<int> h_kent_cls;                      // its class
   int64_t* d_row_dst = nullptr;   // per global Cartesian row: first K element and length
   int* d_row_n = nullptr;
   int64_t nrows = 0;
@@ -258,6 +260,9 @@ struct PlanData {
   double* d_jp = nullptr;
   double* d_jpad = nullptr;     // ket-side J, one 64-byte line per element
   double* d_part = nullptr;     // partial-sum slabs of the bra-major chunks, [nbd][part_total]
+  // per launch stream: row lists of the exchange-row slabs of its classes, so
+  // that each stream folds its own slabs as soon as its classes are done
+  std::vector<int64_t*> d_lane_krp, d_lane_kent;
   size_t part_nbd = 0;
   double* d_incore = nullptr;   // in-core integrals: stored Cartesian blocks of the J/K quartets
   size_t incore_doubles = 0;
@@ -492,7 +497,7 @@ __global__ __launch_bounds__(64) void kpart_reduce_kernel(const double* __restri
                                                           const int64_t* __restrict__ ent,
                                                           const int64_t* __restrict__ row_dst,
                                                           const int* __restrict__ row_n, int64_t ncc_tot,
-                                                          double* __restrict__ kp) {
+                                                          int atomic, double* __restrict__ kp) {
   const int64_t R = blockIdx.x;
   const int bt = blockIdx.z;
   const int col = blockIdx.y * 64 + threadIdx.x;
@@ -507,7 +512,13 @@ __global__ __launch_bounds__(64) void kpart_reduce_kernel(const double* __restri
     for (int u = 0; u < 8; ++u) v[u] += pb[ent[e + u]];
   }
   for (int u = 0; e < e1; ++e, ++u) v[u] += pb[ent[e]];
-  kp[(size_t)bt * ncc_tot + row_dst[R] + col] += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+  const double sum = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+  double* dst = kp + (size_t)bt * ncc_tot + row_dst[R] + col;
+  if (!atomic) {
+    *dst += sum;
+  } else if (e1 > e0) {
+    atomicAdd(dst, sum);  // another stream may fold into the same row
+  }
 }
 
 // Ket-side J blocks from the partial ket windows: one wavefront per (ket
@@ -1110,6 +1121,7 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
   S.nquartets = 0;
   S.part_total = 0;
   std::vector<std::pair<int64_t, int64_t>> kent, jent;  // (key, slab offset)
+  std::vector<int> kent_cls;                              // class of each kent entry
   for (int cb = 0; cb < kNumPairClasses; ++cb)
     for (int ck = 0; ck <= cb; ++ck)
       for (int pb = 0; pb < 4; ++pb)
@@ -1280,6 +1292,7 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
                     for (int r = 0; r < rows; ++r) {
                       const int ao = r < wa ? P.seg[pr.sa].ao_cart + r : P.seg[pr.sb].ao_cart + (r - wa);
                       kent.push_back({(int64_t)P.mols[m].aoc0 + ao, S.part_total + ((int64_t)sl * rows + r) * nm});
+                      kent_cls.push_back((int)S.classes.size());
                     }
                   }
                 S.part_total += (int64_t)c.nbra * rows * nm;
@@ -1339,6 +1352,8 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
     S.d_row_n = dev_upload(row_n, s);
     csr(kent, nrows, S.d_krp, S.d_kent);
     csr(jent, (int64_t)S.pairs.size(), S.d_jrp, S.d_jent);
+    S.h_kent = std::move(kent);
+    S.h_kent_cls = std::move(kent_cls);
   }
   S.built = true;
 }
@@ -1544,6 +1559,27 @@ static void assign_streams(PlanData& P) {
     P.join_ev.push_back(ev);
   }
   if (ns > 1) HIP_OK(hipEventCreateWithFlags(&P.fork_ev, hipEventDisableTiming));
+  // per-stream row lists of the exchange-row slabs
+  const PairSet& S = P.jk;
+  if (ns > 1 && S.part_total > 0 && !S.h_kent.empty()) {
+    std::vector<int> lane_of(S.classes.size(), 0);
+    for (int l = 0; l < ns; ++l)
+      for (int ci : P.lane_classes[l]) lane_of[ci] = l;
+    for (int l = 0; l < ns; ++l) {
+      std::vector<int64_t> rp(S.nrows + 1, 0), off;
+      for (size_t i = 0; i < S.h_kent.size(); ++i)
+        if (lane_of[S.h_kent_cls[i]] == l) ++rp[S.h_kent[i].first + 1];
+      for (int64_t k = 0; k < S.nrows; ++k) rp[k + 1] += rp[k];
+      off.assign(std::max<int64_t>(1, rp[S.nrows]), 0);
+      std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
+      for (size_t i = 0; i < S.h_kent.size(); ++i)
+        if (lane_of[S.h_kent_cls[i]] == l) off[fill[S.h_kent[i].first]++] = S.h_kent[i].second;
+      P.d_lane_krp.push_back(dev_upload(rp, nullptr));
+      P.d_lane_kent.push_back(dev_upload(off, nullptr));
+    }
+    // the lists are read from the plan's non-blocking streams
+    HIP_OK(hipDeviceSynchronize());
+  }
 }
 
 static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
@@ -1793,6 +1829,8 @@ static void free_plan(PlanData& P) {
   for (hipEvent_t e : P.join_ev) (void)hipEventDestroy(e);
   if (P.fork_ev) (void)hipEventDestroy(P.fork_ev);
   for (hipStream_t st : P.side) (void)hipStreamDestroy(st);
+  for (int64_t* p : P.d_lane_krp) f(p);
+  for (int64_t* p : P.d_lane_kent) f(p);
   if (P.h_flag) (void)hipHostFree(P.h_flag);
 }
 
@@ -1858,6 +1896,11 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
   // that the per-class events time one kernel each), otherwise spread over
   // the plan's streams, forked from and joined back into the caller's stream.
   const bool fork = !P.prof && !P.side.empty();
+  // with several streams each folds the exchange-row slabs of its own
+  // classes right after them (device-scope adds into K), overlapping the
+  // other streams' class kernels; otherwise one reduction after the join
+  const bool lane_reduce = fork && with_k && (P.options.partial_sums & 1) && S.part_total > 0 &&
+                           P.d_lane_krp.size() == P.lane_classes.size();
   if (fork) {
     HIP_OK(hipEventRecord(P.fork_ev, s));
     for (hipStream_t st : P.side) HIP_OK(hipStreamWaitEvent(st, P.fork_ev, 0));
@@ -1998,6 +2041,11 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       P.ev_used += 2;
     }
   }
+  if (lane_reduce)
+    for (size_t l = 0; l < P.lane_classes.size(); ++l)
+      hipLaunchKernelGGL(kpart_reduce_kernel, dim3((unsigned)S.nrows, (unsigned)((S.nc_max + 63) / 64),
+                         (unsigned)nbd), dim3(64), 0, l == 0 ? s : P.side[l - 1], P.d_part, S.part_total,
+                         P.d_lane_krp[l], P.d_lane_kent[l], S.d_row_dst, S.d_row_n, P.ncc_tot, 1, P.d_kp);
   if (fork) {
     for (size_t l = 0; l < P.side.size(); ++l) {
       HIP_OK(hipEventRecord(P.join_ev[l], P.side[l]));
@@ -2006,10 +2054,10 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
   }
   if (S.part_total > 0) {
     const int nbd_k = with_k ? nbd : nb;
-    if (with_k && (P.options.partial_sums & 1))
+    if (with_k && (P.options.partial_sums & 1) && !lane_reduce)
       hipLaunchKernelGGL(kpart_reduce_kernel, dim3((unsigned)S.nrows, (unsigned)((S.nc_max + 63) / 64),
                          (unsigned)nbd_k), dim3(64), 0, s, P.d_part, S.part_total, S.d_krp, S.d_kent,
-                         S.d_row_dst, S.d_row_n, P.ncc_tot, P.d_kp);
+                         S.d_row_dst, S.d_row_n, P.ncc_tot, 0, P.d_kp);
     if (with_j && (P.options.partial_sums & 2))
       hipLaunchKernelGGL(jpart_reduce_kernel, dim3((unsigned)S.pairs.size(), (unsigned)nb), dim3(64), 0, s,
                          P.d_part, S.part_total, S.d_jrp, S.d_jent, S.d_pairs, S.d_shells, P.d_molinfo,
