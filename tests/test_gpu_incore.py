@@ -112,3 +112,23 @@ def test_incore_scf_matches_direct_scf():
     got = rhf_batch(bplan, *args, incore=True)
     assert got.converged.all()
     assert np.max(np.abs(got.energies - ref.energies)) < 1e-10
+
+
+def test_incore_with_general_contraction_matches_evaluating_build():
+    """A plan with native general contraction keeps its single-column classes
+    in core and evaluates the multi-column patterns in the same build."""
+    from gradscf_amd import autotune as at
+    from gradscf_amd.jk import DirectJKPlan
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "cc-pvdz", cart=False)
+    before = at.get_options()
+    try:
+        at.set_options(general_contraction=1)
+        plan = DirectJKPlan(atm, bas, env, cart=False, cutoff=1e-12)
+    finally:
+        at.set_options(**before)
+    d = _cuda(systems.random_density(plan.nao, nb=1, seed=33, symmetric=False))
+    ref = plan.jk(d)
+    assert plan.incore_bytes() > 0
+    plan.incore(True)
+    got = plan.jk(d)
+    assert float((got - ref).abs().max()) < 1e-12
