@@ -92,3 +92,21 @@ def test_packed_entry_points_reject_bad_shapes_as_internal():
     assert L.gscf_last_error().decode() == "Packed ERI shape mismatch"
     with pytest.raises(ValueError):
         packed_jk(eri.reshape(1, -1), d)
+
+
+def test_packed_jk_complex_density_is_linear():
+    """Complex densities are contracted as their real and imaginary parts
+    (backends/native/jk.py:530-546)."""
+    import torch
+    from gradscf_amd.jk import packed_eri, packed_jk
+    topo, _, atm, bas, env = systems.tables(systems.water(), "6-31g", cart=True)
+    eri = packed_eri(atm, bas, env, cart=True)
+    n = topo.nao()
+    re = torch.as_tensor(systems.random_density(n, nb=2, seed=11, symmetric=False), device="cuda")
+    im = torch.as_tensor(systems.random_density(n, nb=2, seed=12, symmetric=False), device="cuda")
+    j, k = packed_jk(eri, torch.complex(re, im))
+    jr, kr = packed_jk(eri, re)
+    ji, ki = packed_jk(eri, im)
+    assert j.is_complex() and j.shape == re.shape
+    assert float((j - torch.complex(jr, ji)).abs().max()) < 1e-12
+    assert float((k - torch.complex(kr, ki)).abs().max()) < 1e-12
