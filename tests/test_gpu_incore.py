@@ -132,3 +132,32 @@ def test_incore_with_general_contraction_matches_evaluating_build():
     plan.incore(True)
     got = plan.jk(d)
     assert float((got - ref).abs().max()) < 1e-12
+
+
+def test_full_256_cluster_batch_properties():
+    """The 256-cluster 6-31G* batch at full size (1.07e8 quartets): in-core
+    builds equal evaluating builds, J/K are linear in the density, and every
+    molecule's J and K of a symmetric density are symmetric."""
+    import torch
+    from gradscf_amd.basis import prepare_basis
+    from gradscf_amd.jk import DirectJKPlan
+    from gradscf_amd.packing import pack
+    specs = M.batch_water_clusters(256, seed=256, base_seed=1000, nmax=8, box=8.0)
+    tabs = [pack(*prepare_basis(s, "6-31g*", cart=True)) for s in specs]
+    plan = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
+    assert plan.nmol == 256 and int(plan.mol_n.sum()) == 22458
+    d1 = plan.pack([torch.as_tensor(systems.random_density(int(n), seed=100 + m)[0], device="cuda")
+                    for m, n in enumerate(plan.mol_n)])
+    d2 = plan.pack([torch.as_tensor(systems.random_density(int(n), seed=900 + m)[0], device="cuda")
+                    for m, n in enumerate(plan.mol_n)])
+    a = plan.jk(d1[None], symmetric=True)
+    b = plan.jk(d2[None], symmetric=True)
+    ab = plan.jk((d1 + 2.0 * d2)[None], symmetric=True)
+    scale = float(ab.abs().max())
+    assert float((ab - (a + 2.0 * b)).abs().max()) < 1e-12 * max(scale, 1.0)
+    for j, k in zip(plan.unpack(a[0, 0]), plan.unpack(a[1, 0])):
+        assert float((j - j.T).abs().max()) < 1e-12 and float((k - k.T).abs().max()) < 1e-12
+    plan.incore(True)
+    a_in = plan.jk(d1[None], symmetric=True)
+    assert float((a_in - a).abs().max()) < 1e-12
+    plan.incore(False)
