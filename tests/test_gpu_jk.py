@@ -418,6 +418,29 @@ def test_partial_sum_flushes_do_not_change_results(pm):
         at.set_options(**before)
 
 
+def test_partial_sum_flushes_on_a_batched_plan():
+    """Both flush kinds as partial slabs on a batched plan (molecules of
+    different widths share the slabs and the per-stream row lists)."""
+    import torch
+    from gradscf_amd import autotune as at
+    from gradscf_amd import molecule as M
+    from gradscf_amd.jk import DirectJKPlan
+    specs = [M.water_monomer(), M.water_cluster(3, seed=1003, box=8.0), M.water_cluster(2, seed=1002, box=8.0)]
+    tabs = [systems.tables(sp, "6-31g*", cart=True)[2:] for sp in specs]
+    before = at.get_options()
+    outs = {}
+    try:
+        for pm in (0, 3):
+            at.set_options(partial_sums=pm)
+            plan = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
+            dens = plan.pack([torch.as_tensor(systems.random_density(int(n), nb=2, seed=60 + m, symmetric=False),
+                                              device="cuda") for m, n in enumerate(plan.mol_n)])
+            outs[pm] = plan.jk(dens)
+    finally:
+        at.set_options(**before)
+    assert float((outs[3] - outs[0]).abs().max()) < 1e-11
+
+
 @pytest.mark.parametrize("gc,streams", [(1, 4), (1, 1), (0, 1)])
 def test_engine_options_do_not_change_results(gc, streams):
     """(H2O)4 cc-pVDZ spherical: native general contraction instead of
