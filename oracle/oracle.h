@@ -9,8 +9,11 @@
  *                     libcint CINTrys_roots (_ref)
  *   ERI blocks        (ss|ss) closed form, 8-fold permutation symmetry,
  *                     libcint int2e_cart / int2e_sph blocks (_ref)
- *   Schwarz, J/K      full-tensor einsum brute force, linearity, symmetry,
- *                     the same driver running on libcint blocks (_ref)
+ *   Schwarz           Q_ij^2 = max diagonal of the (ij|ij) block rebuilt from own and
+ *                     libcint (_ref) blocks; |(ij|kl)| <= Q_ij Q_kl on random quartets
+ *   J/K               full-tensor einsum brute force (unscreened); screened builds equal
+ *                     the full tensor masked by Q_ij Q_kl >= cutoff (1e-12 and 1e-7) and
+ *                     the unmasked one to 1e-10 at 1e-12; the same driver on libcint blocks
  *   one-electron      libcint int1e_ovlp/kin/nuc (_ref), published RHF energies
  *   ERI derivatives   libcint int2e_ip1 (_ref), translation sum rule, finite differences
  */

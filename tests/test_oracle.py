@@ -215,3 +215,114 @@ def test_oracle_geometry_vjp_pins(oracle_mod):
         with oracle_mod.backend("libcint"):
             ref = oracle_mod.jk_geometry_vjp(atm, bas, env, d, cj, ck, cart=True)
         assert np.max(np.abs(own - ref)) < 1e-10
+
+
+# ---- Schwarz bounds and the screened quartet loop -------------------------------------------
+# Reference semantics (csrc/coulomb_shells.h:60-77, 98-110): Q_ij = sqrt(max over AO pairs
+# (p in i, q in j) of |(pq|pq)|); a canonical quartet is skipped when Q_ij*Q_kl < cutoff (the
+# pair prune Q_ij*max(Q) < cutoff only removes quartets the skip removes too).  The pins below
+# rebuild Q from ERI blocks in the test, bound every block by it, and compare the screened
+# driver with the full s1 tensor masked by exactly the quartets the rule keeps.
+
+def _block_fn(oracle_mod, which):
+    if which == "libcint":
+        return lambda q, atm, bas, env, cart: oracle_mod.libcint_block(q, atm, bas, env, cart)
+    return lambda q, atm, bas, env, cart: oracle_mod.eri_block(q, atm, bas, env, cart)
+
+
+def _schwarz_from_blocks(block, atm, bas, env, cart):
+    nb = len(bas)
+    q = np.zeros((nb, nb))
+    for i in range(nb):
+        for j in range(i + 1):
+            b = block((i, j, i, j), atm, bas, env, cart)
+            di, dj = b.shape[0], b.shape[1]
+            diag = np.abs(b.reshape(di * dj, di * dj).diagonal())
+            q[i, j] = q[j, i] = math.sqrt(diag.max())
+    return q
+
+
+def _canonical_quartets(nb):
+    for i in range(nb):
+        for j in range(i + 1):
+            for k in range(i + 1):
+                for l in range(k + 1 if k < i else j + 1):
+                    yield i, j, k, l
+
+
+def _full_tensor_and_mask(block, atm, bas, env, cart, qmat, cutoff):
+    """Full s1 tensor from canonical blocks (8 permutations each) and the 0/1 mask of the AO
+    quartets whose shell quartet survives Q_ij*Q_kl >= cutoff."""
+    ao = np.asarray(__import__("oracle").ao_offsets(bas, cart))
+    n = int(ao[-1])
+    eri = np.zeros((n, n, n, n))
+    keep = np.zeros((n, n, n, n))
+    dropped = 0
+    for i, j, k, l in _canonical_quartets(len(bas)):
+        b = block((i, j, k, l), atm, bas, env, cart)
+        kept = qmat[i, j] * qmat[k, l] >= cutoff
+        dropped += not kept
+        s = [slice(ao[x], ao[x + 1]) for x in (i, j, k, l)]
+        for perm in ((0, 1, 2, 3), (1, 0, 2, 3), (0, 1, 3, 2), (1, 0, 3, 2),
+                     (2, 3, 0, 1), (3, 2, 0, 1), (2, 3, 1, 0), (3, 2, 1, 0)):
+            idx = tuple(s[p] for p in perm)
+            eri[idx] = b.transpose(perm)
+            keep[idx] = 1.0 if kept else 0.0
+    return eri, keep, dropped
+
+
+@pytest.mark.parametrize("which", ["oracle", pytest.param("libcint", marks=ref_only)])
+@pytest.mark.parametrize("cart", [True, False])
+def test_schwarz_matches_block_diagonal_definition(oracle_mod, which, cart):
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "cc-pvdz", cart=cart)
+    mine = oracle_mod.schwarz(atm, bas, env, cart)
+    want = _schwarz_from_blocks(_block_fn(oracle_mod, which), atm, bas, env, cart)
+    low = np.tril_indices(len(bas))
+    # relative 1e-13; the absolute floor covers pairs with Q ~ 1e-11 ((pq|pq) ~ 1e-22), where
+    # libcint's primitive-pair cut changes the block in its tenth digit
+    assert np.all(np.abs(mine[low] - want[low]) <= 1e-13 * want[low] + 1e-18)
+
+
+@pytest.mark.parametrize("cart", [True, False])
+def test_schwarz_bounds_every_block(oracle_mod, cart):
+    """Cauchy-Schwarz: |(ij|kl)| <= Q_ij Q_kl for every element, with equality on (ij|ij)."""
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "cc-pvdz", cart=cart)
+    q = oracle_mod.schwarz(atm, bas, env, cart)
+    q = np.tril(q) + np.tril(q, -1).T
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for _ in range(300):
+        i, j, k, l = (int(v) for v in rng.integers(0, len(bas), 4))
+        b = oracle_mod.eri_block((i, j, k, l), atm, bas, env, cart)
+        worst = max(worst, float(np.abs(b).max() / (q[i, j] * q[k, l])))
+    assert worst <= 1.0 + 1e-12
+    for i in range(len(bas)):
+        b = oracle_mod.eri_block((i, 0, i, 0), atm, bas, env, cart)
+        assert abs(np.abs(b).max() - q[i, 0] ** 2) <= 1e-13 * q[i, 0] ** 2
+
+
+@pytest.mark.parametrize("cart", [True, False])
+@pytest.mark.parametrize("cutoff", [1e-12, 1e-7])
+def test_screened_jk_equals_masked_full_tensor(oracle_mod, cart, cutoff):
+    """The screened driver keeps exactly the quartets with Q_ij Q_kl >= cutoff (masked einsum to
+    1e-13), and at 1e-12 differs from the unscreened full-tensor J/K by <= 1e-10."""
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "cc-pvdz", cart=cart)
+    block = _block_fn(oracle_mod, "libcint" if oracle_mod.have_ref() else "oracle")
+    qmat = _schwarz_from_blocks(block, atm, bas, env, cart)
+    eri, keep, dropped = _full_tensor_and_mask(block, atm, bas, env, cart, qmat, cutoff)
+    assert dropped > 1000  # the screening branch really removes quartets here
+    n = eri.shape[0]
+    d = systems.random_density(n, nb=2, seed=21, symmetric=False)
+    masked = eri * keep
+    jm = np.einsum("pqrs,brs->bpq", masked, d)
+    km = np.einsum("prqs,brs->bpq", masked, d)
+    for backend in (["oracle", "libcint"] if oracle_mod.have_ref() else ["oracle"]):
+        with oracle_mod.backend(backend):
+            out = oracle_mod.direct_jk(atm, bas, env, d, cart=cart, cutoff=cutoff, workers=4)
+        assert np.max(np.abs(out[0] - jm)) < 1e-12
+        assert np.max(np.abs(out[1] - km)) < 1e-12
+        if cutoff == 1e-12:
+            jf = np.einsum("pqrs,brs->bpq", eri, d)
+            kf = np.einsum("prqs,brs->bpq", eri, d)
+            assert np.max(np.abs(out[0] - jf)) < 1e-10
+            assert np.max(np.abs(out[1] - kf)) < 1e-10
