@@ -2731,7 +2731,9 @@ int gscf_jk_build(gscf_plan* plan, const double* density, int32_t nb, double* ou
 int gscf_direct_jk(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
                    int32_t nenv, const double* density, int32_t nb, int32_t n, double* out, int32_t cart,
                    double cutoff, int32_t workers, int32_t with_j, int32_t with_k, void* stream) {
-  return guarded([&] {
+  // the reference target reports every failure, argument checks included, as Internal
+  // (direct_jk.cc:83,93): collapse the InvalidArgument / Internal split here
+  const int rc = guarded([&] {
     if (workers < 1) throw InvalidArgument("Direct J/K workers must be positive");
     if ((with_j != 0 && with_j != 1) || (with_k != 0 && with_k != 1) || !(with_j || with_k))
       throw InvalidArgument("Direct J/K requires at least one selected operator");
@@ -2748,6 +2750,7 @@ int gscf_direct_jk(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t
     }
     free_plan(p.d);
   });
+  return rc == GSCF_OK ? GSCF_OK : GSCF_INTERNAL;
 }
 
 int gscf_plan_get_schwarz(const gscf_plan* plan, double* q) {

@@ -40,6 +40,17 @@ def _stream_handle() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def check_out_buffer(out: torch.Tensor, shape: tuple, device: torch.device) -> None:
+    """A caller-supplied result buffer must be exactly what the native build writes
+    through its pointer: float64, contiguous, on the density's GPU, of `shape`."""
+    if not isinstance(out, torch.Tensor):
+        raise ValueError("Direct J/K output must be a torch tensor")
+    if out.dtype != torch.float64 or not out.is_cuda or out.device != device:
+        raise ValueError("Direct J/K output must be float64 on the density's GPU")
+    if tuple(out.shape) != tuple(shape) or not out.is_contiguous():
+        raise ValueError("Direct J/K output shape mismatch")
+
+
 def concat_tables(tables):
     """Concatenate per-molecule libcint tables [(atm, bas, env), ...] into one
     (atm, bas, env) plus the molecule offsets (mol_atm, mol_bas) of a batched
@@ -169,6 +180,8 @@ class DirectJKPlan:
         nops = int(bool(with_j)) + int(bool(with_k))
         if out is None:
             out = torch.empty((nops, nb) + tail, dtype=torch.float64, device=d.device)
+        else:
+            check_out_buffer(out, (nops, nb) + tail, d.device)
         if symmetric is None:
             if self.nmol == 1:
                 symmetric = bool(torch.equal(d, d.transpose(1, 2)))
@@ -429,11 +442,16 @@ def packed_jk(eri, density, *, with_j=True, with_k=True):
     if not (with_j or with_k):
         return torch.zeros_like(density), torch.zeros_like(density)
     n = density.shape[-1]
-    flat = density.reshape(-1, n, n)
+    if density.dim() < 2 or tuple(density.shape[-2:]) != (n, n):
+        raise ValueError("Packed J/K dimensions do not match")
+    # both operands go to the engine's GPU before their pointers are taken (a host
+    # pointer handed to the kernels would be dereferenced on the device)
+    dev = _device()
+    flat = density.reshape(-1, n, n).to(dev)
     count = flat.shape[0]
     real = torch.cat((flat.real, flat.imag), 0) if flat.is_complex() else flat
     real = real.to(torch.float64).contiguous()
-    eri = eri.to(torch.float64).contiguous()
+    eri = eri.to(dev, torch.float64).contiguous()
     nops = int(bool(with_j)) + int(bool(with_k))
     out = torch.empty((nops,) + tuple(real.shape), dtype=torch.float64, device=real.device)
     _native.check(_native.lib().gscf_packed_jk(eri.data_ptr(), eri.numel(), 1, real.data_ptr(), real.shape[0], n,

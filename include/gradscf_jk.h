@@ -16,7 +16,13 @@
  * validates its arguments on the host before any launch and returns
  *   GSCF_OK, GSCF_INVALID_ARGUMENT (-> ffi::Error::InvalidArgument) or
  *   GSCF_INTERNAL (-> ffi::Error::Internal), with the message in
- *   gscf_last_error() (thread-local), mirroring direct_jk.cc:78-89.
+ *   gscf_last_error() (thread-local).  The drop-ins follow their reference
+ * targets' error classes: gscf_direct_jk, gscf_packed_eri and gscf_packed_jk
+ * report every failure as GSCF_INTERNAL (direct_jk.cc:78-93, compact.cc:16,
+ * packed_jk.cc); gscf_jk_geometry_vjp keeps the split of the geometry target
+ * (std::invalid_argument -> InvalidArgument, else Internal,
+ * direct_jk_geometry.cc:155-179); the plan-based calls, which have no
+ * reference counterpart, use the same split.
  */
 #ifndef GRADSCF_JK_H
 #define GRADSCF_JK_H
@@ -204,8 +210,11 @@ int gscf_jk_build(gscf_plan* plan, const double* density, int32_t nb, double* ou
                   void* stream);
 
 /* One-shot form with the argument list of GradSCFDirectJK (direct_jk.cc:97-102):
- * builds a plan, runs gscf_jk_build, synchronises `stream`, frees the plan.
- * `workers` is validated (>= 1) and otherwise unused on the GPU. */
+ * builds a plan, runs gscf_jk_build (nonfinite densities rejected, as
+ * direct_jk.cc:24), synchronises `stream`, frees the plan.  `workers` is
+ * validated (>= 1) and otherwise unused on the GPU.  Every failure --
+ * argument checks, table validation, shape mismatch, HIP errors -- returns
+ * GSCF_INTERNAL with the reference's message, as direct_jk.cc:83,93 does. */
 int gscf_direct_jk(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas,
                    const double* env, int32_t nenv, const double* density, int32_t nb, int32_t n,
                    double* out, int32_t cart, double cutoff, int32_t workers, int32_t with_j,

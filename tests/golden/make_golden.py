@@ -11,9 +11,13 @@ compiled from the reference sources), i.e. the reference arithmetic:
                            C = default_rng(16).normal(size=(384, 80)) / sqrt(384), cutoff
                            1e-12: 4000 sampled J and K elements,
                            tr(DJ), tr(DK) and Frobenius norms
-  scf_energies.json        oracle RHF energies (water / 6-31G, the config-1 geometry)
+  scf_energies.json        oracle RHF energies: water / 6-31G (Cartesian, the config-1
+                           geometry, cutoff 0), and with --large the reference-libcint
+                           FP64 RHF energies of C20H42 (all-trans) and (H2O)32 (seed 32,
+                           15 A cube) in 6-31G* Cartesian and (H2O)16 (seed 16, 12 A cube)
+                           in cc-pVDZ spherical, Schwarz cutoff 1e-12
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [--large]
 """
 import json
 import sys
@@ -74,5 +78,41 @@ def main():
         indent=1))
 
 
+LARGE = {
+    # key: (molecule, basis, cart)
+    "c20h42_631gs_cart": (lambda: M.linear_alkane(20), "6-31g*", True),
+    "water32_631gs_cart": (lambda: M.water_cluster(32, seed=32, box=15.0), "6-31g*", True),
+    "water16_ccpvdz_sph": (lambda: M.water_cluster(16, seed=16, box=12.0), "cc-pvdz", False),
+}
+
+
+def large(workers: int = 8):
+    """FP64 RHF energies of the larger workloads on the reference libcint (cutoff 1e-12)."""
+    import time
+    path = HERE / "scf_energies.json"
+    table = json.loads(path.read_text())
+    for key, (make, basis, cart) in LARGE.items():
+        spec = make()
+        _, _, atm, bas, env = systems.tables(spec, basis, cart=cart)
+        with oracle.backend("libcint"):
+            s, t, v = oracle.libcint_int1e(atm, bas, env, cart)
+
+            def jk(dm):
+                o = oracle.direct_jk(atm, bas, env, dm, cart=cart, cutoff=1e-12, workers=workers)
+                return o[0, 0], o[1, 0]
+
+            t0 = time.perf_counter()
+            r = oscf.rhf(t + v, s, spec.nelectron, spec.nuclear_repulsion(), jk)
+        print(f"{key}: E={r['energy']:.12f} cycles={r['cycles']} converged={r['converged']} "
+              f"{time.perf_counter() - t0:.1f} s", flush=True)
+        table[key] = {"energy": r["energy"], "cycles": r["cycles"], "converged": r["converged"],
+                      "basis": basis, "cart": cart, "cutoff": 1e-12, "nao": int(s.shape[0]),
+                      "eri": "reference libcint 6.1.3 (oracle/_ref)"}
+        path.write_text(json.dumps(table, indent=1))
+
+
 if __name__ == "__main__":
-    main()
+    if "--large" in sys.argv[1:]:
+        large()
+    else:
+        main()

@@ -252,9 +252,20 @@ def test_c_abi_rejects_bad_tables_without_gpu():
     rc = L.gscf_plan_create(atm.ctypes.data, len(atm), bas.ctypes.data, len(bas), env2.ctypes.data, len(env2),
                             1, ctypes.c_double(0.0), None, ctypes.byref(h))
     assert rc == _native.GSCF_INVALID_ARGUMENT
-    rc = L.gscf_direct_jk(atm.ctypes.data, len(atm), bas.ctypes.data, len(bas), env_f.ctypes.data, len(env_f),
-                          None, 1, 13, None, 1, ctypes.c_double(0.0), 0, 1, 1, None)
-    assert rc == _native.GSCF_INVALID_ARGUMENT and b"workers" in L.gscf_last_error()
+    # the one-shot drop-in reports every failure as Internal, like the reference's
+    # GradSCFDirectJK (direct_jk.cc:83,93), with the reference's messages
+    def one_shot(b=bas, e=env_f, n=13, workers=1, with_j=1, with_k=1):
+        return L.gscf_direct_jk(atm.ctypes.data, len(atm), b.ctypes.data, len(b), e.ctypes.data, len(e),
+                                None, 1, n, None, 1, ctypes.c_double(0.0), workers, with_j, with_k, None)
+
+    for kwargs, msg in (({"workers": 0}, b"workers must be positive"),
+                        ({"with_j": 0, "with_k": 0}, b"at least one selected operator"),
+                        ({"with_j": 2}, b"at least one selected operator"),
+                        ({"n": 12}, b"shape mismatch"),
+                        ({"b": bad}, b"l <= 2"),
+                        ({"e": env2}, b"Range-separated")):
+        assert one_shot(**kwargs) == _native.GSCF_INTERNAL, kwargs
+        assert msg in L.gscf_last_error(), (kwargs, L.gscf_last_error())
     for blocks, iters in ((0, 10), (4, 0)):
         assert L.gscf_fma_probe(blocks, iters, ctypes.c_void_p(8), None) == _native.GSCF_INVALID_ARGUMENT
     assert L.gscf_fma_probe(4, 10, None, None) == _native.GSCF_INVALID_ARGUMENT
@@ -452,3 +463,21 @@ def test_padded_batch_algebra_matches_per_molecule():
         assert torch.allclose(co @ co.T, c0[:, :no] @ c0[:, :no].T, atol=1e-10)
         if n < B.N:   # real orbitals have no padding rows
             assert float(C[m, n:, :n].abs().max()) < 1e-12
+
+
+def test_out_buffer_and_packed_jk_rejections_without_gpu():
+    """A caller's result buffer is checked before its pointer reaches the engine; packed
+    J/K moves its operands to the GPU first, so on a host without one it fails before
+    any pointer is taken."""
+    import torch
+    from gradscf_amd.jk import check_out_buffer, packed_jk
+    dev = torch.device("cuda", 0)
+    for bad in (np.zeros((2, 1, 3, 3)), torch.zeros((2, 1, 3, 3), dtype=torch.float64),
+                torch.zeros((2, 1, 3, 3), dtype=torch.float32)):
+        with pytest.raises(ValueError):
+            check_out_buffer(bad, (2, 1, 3, 3), dev)
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError, match="needs a GPU"):
+            packed_jk(torch.zeros(21, dtype=torch.float64), torch.zeros((3, 3), dtype=torch.float64))
+    with pytest.raises(ValueError, match="s8"):
+        packed_jk(torch.zeros((6, 6), dtype=torch.float64), torch.zeros((3, 3), dtype=torch.float64))
