@@ -135,6 +135,23 @@ def normalized_shell_coefficients(l: int, exponents, coefficients) -> np.ndarray
     return w / np.sqrt(norm2)[None, :]
 
 
+def normalized_shell_coefficients_torch(l: int, exponents, coefficients):
+    """normalized_shell_coefficients in torch operations, so that gradients
+    with respect to the normalised values chain to raw exponents and
+    coefficients (the reference differentiates the same map,
+    src/gradscf/integrals/basis/normalization.py:57-63)."""
+    import torch
+    a = torch.as_tensor(exponents, dtype=torch.float64)
+    c = torch.as_tensor(coefficients, dtype=torch.float64, device=a.device)
+    if c.dim() == 1:
+        c = c[:, None]
+    n = l + 1.5
+    w = c * (1.0 / torch.sqrt(math.gamma(n) / (2.0 * (2.0 * a) ** n)))[:, None]
+    metric = math.gamma(n) / (2.0 * (a[:, None] + a[None, :]) ** n)
+    norm2 = torch.einsum("pi,pq,qi->i", w, metric, w)
+    return w / torch.sqrt(norm2)[None, :]
+
+
 def ncart(l: int) -> int:
     return (l + 1) * (l + 2) // 2
 

@@ -17,6 +17,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
+from .basis import normalized_shell_coefficients_torch
 from .jk import DirectJKPlan
 
 
@@ -69,17 +70,6 @@ def rhf_gradient(plan: DirectJKPlan, density, mo_coeff, mo_energy, nocc: int, ch
     return fold_rows(rows, nnuc, shell_atom) + nuclear_repulsion_gradient(charges, coords)
 
 
-def _normalized_torch(l: int, a, c):
-    """torch version of basis.normalized_shell_coefficients (for autograd)."""
-    import math
-    n = l + 1.5
-    prim = 1.0 / torch.sqrt(math.gamma(n) / (2.0 * (2.0 * a) ** n))
-    w = c * prim[:, None]
-    metric = math.gamma(n) / (2.0 * (a[:, None] + a[None, :]) ** n)
-    norm2 = torch.einsum("pi,pq,qi->i", w, metric, w)
-    return w / torch.sqrt(norm2)[None, :]
-
-
 def env_to_basis_gradient(env_grad, bas, topology, parameters):
     """Chain an env-slot gradient (exponent slots and normalised-coefficient
     slots) through the basis normalisation: returns (dE/d exponents,
@@ -95,7 +85,7 @@ def env_to_basis_gradient(env_grad, bas, topology, parameters):
         gc = torch.as_tensor(g[int(row[6]):int(row[6]) + npr * nct].reshape(nct, npr).T.copy())
         at = torch.tensor(np.asarray(a, dtype=np.float64), requires_grad=True)
         ct = torch.tensor(np.asarray(c, dtype=np.float64).reshape(npr, nct), requires_grad=True)
-        e = _normalized_torch(int(l), at, ct)
+        e = normalized_shell_coefficients_torch(int(l), at, ct)
         loss = (gc * e).sum() + (ge * at).sum()
         loss.backward()
         d_exp.append(at.grad.numpy().copy())

@@ -24,7 +24,7 @@ import torch
 
 from . import _native
 from .basis import BasisParameters, BasisTopology
-from .packing import PTR_COORD, pack
+from .packing import PTR_COORD, pack, pack_torch
 
 _PLAN_CACHE: "OrderedDict[tuple, DirectJKPlan]" = OrderedDict()
 _PLAN_CACHE_SIZE = 8
@@ -365,7 +365,14 @@ def _validate_selection(with_j, with_k):
 
 def direct_jk(atm, bas, env, density, *, cart, cutoff=0.0, workers=1, with_j=True, with_k=True,
               coordinates=None):
-    """Contract J/K for density (..., n, n); returns (J, K) shaped like density."""
+    """Contract J/K for density (..., n, n); returns (J, K) shaped like density.
+
+    Differentiable (torch.autograd, gradscf_amd.autodiff) in the density, in
+    `coordinates` (every atm row, (natm, 3)) and in `env` when they are
+    tensors that require gradients: density-linear and self-adjoint, the
+    coordinate VJP and the env VJP (exponent, coefficient and coordinate
+    slots); geometry and basis derivatives need cutoff 0."""
+    from . import autodiff
     _validate_selection(with_j, with_k)
     if isinstance(workers, bool) or not isinstance(workers, Integral) or workers < 1:
         raise ValueError("workers must be a positive integer.")
@@ -375,13 +382,16 @@ def direct_jk(atm, bas, env, density, *, cart, cutoff=0.0, workers=1, with_j=Tru
     n = d.shape[-1]
     if tuple(d.shape[-2:]) != (n, n) or d.dtype not in (torch.float64, torch.complex128):
         raise ValueError("Direct J/K requires (...,n,n) float64/complex128 density.")
-    env = np.asarray(env)
+    env_t = env if isinstance(env, torch.Tensor) else None
+    env = env.detach().cpu().numpy() if env_t is not None else np.asarray(env)
     if env.dtype != np.float64:
         raise ValueError("Direct J/K requires float64 environment.")
     if not (with_j or with_k):
         return torch.zeros_like(d), torch.zeros_like(d)
+    coords_t = coordinates if isinstance(coordinates, torch.Tensor) else None
     if coordinates is not None:
-        coordinates = np.asarray(coordinates, dtype=np.float64)
+        coordinates = (coords_t.detach().cpu().numpy() if coords_t is not None
+                       else np.asarray(coordinates, dtype=np.float64)).astype(np.float64)
         if coordinates.shape != (len(atm), 3):
             raise ValueError("Direct J/K coordinates must contain every nuclear and basis-center atom.")
         env = env.copy()
@@ -394,9 +404,19 @@ def direct_jk(atm, bas, env, density, *, cart, cutoff=0.0, workers=1, with_j=Tru
     plan = get_plan(atm, bas, env, cart=cart, cutoff=cutoff)
     if plan.nao != n:
         raise ValueError("Direct J/K shape mismatch")
-    out = plan.jk(real.contiguous(), with_j=with_j, with_k=with_k)
+    if autodiff.needs_grad(real, env_t, coords_t):
+        op = autodiff.JKOperator(plan, bool(with_j), bool(with_k))
+        if env_t is not None and coords_t is not None:
+            # the env's coordinate slots are replaced by `coordinates`
+            keep = torch.ones(len(env), dtype=torch.float64, device=env_t.device)
+            for row in np.asarray(atm):
+                keep[int(row[PTR_COORD]):int(row[PTR_COORD]) + 3] = 0.0
+            env_t = env_t * keep
+        out = autodiff.apply_jk(op, real.contiguous(), coords_t, env_t)
+    else:
+        out = plan.jk(real.contiguous(), with_j=with_j, with_k=with_k)
     if flat.is_complex():
-        out = out[:, :count] + 1j * out[:, count:]
+        out = torch.complex(out[:, :count], out[:, count:])
     out = out.reshape(int(with_j) + int(with_k), *d.shape)
     return (out[0] if with_j else torch.zeros_like(out[0]),
             out[-1] if with_k else torch.zeros_like(out[0]))
@@ -476,17 +496,35 @@ class IntegralPlan:
 
     def get_jk(self, parameters: BasisParameters, density, *, screening_threshold=0.0, workers=1,
                with_j=True, with_k=True, omega=0.0):
+        """J/K of density (..., n, n) with the reference's argument checks.
+
+        When exponents, coefficients, centres or nuclear coordinates of
+        `parameters` are tensors that require gradients, the environment is
+        packed with differentiable operations (packing.pack_torch), so J/K
+        gradients reach them (unscreened plans)."""
         if omega:
             raise NotImplementedError("Range-separated (omega != 0) J/K is not supported by the MI355X engine.")
         if self.backend != "native":
             raise NotImplementedError("get_jk requires the native backend.")
-        atm, bas, env = self.pack(parameters)
-        coordinates = np.concatenate([np.asarray(parameters.nuclear_coords).reshape(-1, 3),
-                                      np.asarray(parameters.centers).reshape(-1, 3)])
         if tuple(density.shape[-2:]) != (self.topology.nao(),) * 2:
             raise ValueError("Density AO dimensions do not match the plan.")
+        leaves = [*parameters.exponents, *parameters.coefficients, parameters.centers, parameters.nuclear_coords]
+        if any(isinstance(x, torch.Tensor) and x.requires_grad for x in leaves) and torch.is_grad_enabled():
+            atm, bas, env_t = pack_torch(self.topology, parameters)
+            return direct_jk(atm, bas, env_t, density, cart=self.topology.cart, cutoff=screening_threshold,
+                             workers=workers, with_j=with_j, with_k=with_k)
+        params = BasisParameters(tuple(_host(a) for a in parameters.exponents),
+                                 tuple(_host(c) for c in parameters.coefficients),
+                                 _host(parameters.centers), _host(parameters.nuclear_coords))
+        atm, bas, env = self.pack(params)
+        coordinates = np.concatenate([np.asarray(params.nuclear_coords).reshape(-1, 3),
+                                      np.asarray(params.centers).reshape(-1, 3)])
         return direct_jk(atm, bas, env, density, cart=self.topology.cart, cutoff=screening_threshold,
                          workers=workers, with_j=with_j, with_k=with_k, coordinates=coordinates)
+
+
+def _host(x):
+    return x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else np.asarray(x)
 
 
 def make_plan(topology: BasisTopology, backend: str = "native") -> IntegralPlan:

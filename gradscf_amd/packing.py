@@ -12,7 +12,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from .basis import BasisParameters, BasisTopology, normalized_shell_coefficients
+from .basis import (BasisParameters, BasisTopology, normalized_shell_coefficients,
+                    normalized_shell_coefficients_torch)
 
 # libcint slot constants (ATM_SLOTS, BAS_SLOTS and ENV pointers)
 ATM_SLOTS = 6
@@ -66,6 +67,29 @@ def pack(topology: BasisTopology, parameters: BasisParameters, *, origin=None):
     coefficients = tuple(normalized_shell_coefficients(l, a, c) for l, a, c in
                          zip(topology.angular_momenta, parameters.exponents, parameters.coefficients))
     return pack_environment(topology, parameters, coefficients, origin=origin)
+
+
+def pack_torch(topology: BasisTopology, parameters: BasisParameters, *, device=None):
+    """(atm, bas, env) like pack, with env a float64 torch tensor assembled by
+    differentiable operations from parameters whose exponents, coefficients,
+    centres or nuclear coordinates may be tensors that require gradients."""
+    import torch
+    atm, bas, env_size = build_tables(topology)
+
+    def t(x):
+        return torch.as_tensor(x, dtype=torch.float64, device=device).reshape(-1)
+
+    pieces = [torch.zeros(PTR_ENV_START, dtype=torch.float64, device=device),
+              t(parameters.nuclear_coords), t(parameters.centers)]
+    for l, a, c in zip(topology.angular_momenta, parameters.exponents, parameters.coefficients):
+        a = t(a)
+        c = torch.as_tensor(c, dtype=torch.float64, device=a.device).reshape(a.numel(), -1)
+        pieces.append(a)
+        pieces.append(normalized_shell_coefficients_torch(int(l), a, c).T.reshape(-1))
+    env = torch.cat([p.to(pieces[0].device) for p in pieces])
+    if env.numel() != env_size:
+        raise ValueError("basis parameters do not match the topology")
+    return atm, bas, env
 
 
 def ao_offsets(bas: np.ndarray, cart: bool) -> np.ndarray:
