@@ -31,7 +31,8 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("n_cart", ctypes.c_int32), ("nbas", ctypes.c_int32),
                 ("nseg", ctypes.c_int32), ("npairs", ctypes.c_int64), ("nquartets", ctypes.c_int64),
                 ("cutoff", ctypes.c_double), ("plan_ms", ctypes.c_double), ("nmol", ctypes.c_int32),
-                ("nc_max", ctypes.c_int32), ("nn_total", ctypes.c_int64), ("device_bytes", ctypes.c_int64)]
+                ("nc_max", ctypes.c_int32), ("nn_total", ctypes.c_int64), ("device_bytes", ctypes.c_int64),
+                ("lds_max", ctypes.c_int64), ("classes_global_atomic", ctypes.c_int32)]
 
 
 class ClassTuning(ctypes.Structure):
@@ -42,7 +43,8 @@ class ClassTuning(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("general_contraction", ctypes.c_int32), ("streams", ctypes.c_int32),
-                ("partial_sums", ctypes.c_int32)]
+                ("partial_sums", ctypes.c_int32), ("lds_limit", ctypes.c_int32),
+                ("deterministic", ctypes.c_int32)]
 
 
 class NativeError(RuntimeError):

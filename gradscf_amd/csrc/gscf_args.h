@@ -90,6 +90,8 @@ struct JKArgs {
   const MolInfo* mols;
   int64_t bstride;      // elements per density batch
   int n, nbd, nbj, with_j, with_k;
+  // deterministic digestion: [nmol][2] (scale, 1/scale); NULL for FP64 atomics
+  const double* det_scale;
 };
 
 // Bra-major chunk: quartets [q0, q0+nq) of one sub-task whose bras are the
@@ -150,6 +152,9 @@ struct BraArgs {
   int part_mode;
   int64_t part_stride;
   double* part;
+  // deterministic digestion: [nmol][2] (scale, 1/scale) of the build; NULL
+  // for FP64 atomics (tab_lds 0, mixed 0 and part_mode 0 with it)
+  const double* det_scale;
 };
 
 struct ParamArgs {

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -141,7 +142,7 @@ static void ensure_tuning() {
 
 // defaults: segmented general contractions (faster on the measured cc-pVDZ
 // workloads; a tuning table may switch it on), four launch streams
-static gscf_options g_options = {0, 4, 1};
+static gscf_options g_options = {0, 4, 1, 0, 0};
 constexpr int kMaxMoleculeCartAO = 16384;  // per molecule (32-bit in-molecule offsets)
 constexpr int kMaxStreams = 4;
 
@@ -173,9 +174,14 @@ struct QuartetClass {
 };
 
 // LDS of the bra-major kernel per workgroup: slots are packed up to the
-// class's lds_pack so that several workgroups stay resident per CU; a single
-// slot (with the table, when staged) may use up to 64 KB.
+// class's lds_pack so that several workgroups stay resident per CU, within
+// kLdsBudget for the densities of one launch.  Exchange rows span the widest
+// molecule, so a wide molecule's slot can exceed that: such a class runs one
+// density per launch, and a slot may then take up to the device's LDS per
+// workgroup (160 KB on gfx950, one workgroup per CU) before the class falls
+// back to the global-atomic kernels.
 constexpr size_t kLdsBudget = 64 * 1024;
+constexpr size_t kLdsDeviceMax = 160 * 1024;
 constexpr size_t kKetLdsBytes = 16 * 1024;  // LDS ket-J window per density
 // Partial-sum slabs per density at most; a plan whose slabs would be larger
 // flushes with atomics instead.
@@ -220,6 +226,8 @@ struct PairSet {
 struct PlanData {
   gscf_class_tuning tuning[kNumQuartetClasses];
   gscf_options options;
+  size_t lds_budget = kLdsBudget;  // LDS of a launch over all densities (packing target)
+  size_t lds_max = kLdsBudget;     // LDS of a one-density launch (device limit)
   int cart = 1, n = 0, n_cart = 0, nbas = 0, natm = 0;
   double cutoff = 0.0, plan_ms = 0.0;
   std::vector<int> atm, bas, ao, ao_cart;  // ao/ao_cart: global AO offsets of every shell
@@ -283,6 +291,12 @@ struct PlanData {
   std::vector<int64_t> qbase;      // start of each molecule's parent_q block
   // mixed precision: per-build density block maxima over the J/K shell blocks
   double* d_dmax = nullptr;
+  // deterministic digestion: per density-block Cartesian Schwarz weight
+  // Q(S,T) nS nT [nd_tot], per molecule (Qmax, max_S sum_T Q(S,T) nT), and
+  // the fixed-point (scale, 1/scale) of the current build, per molecule
+  double* d_det_qblk = nullptr;
+  double* d_det_mol = nullptr;
+  double* d_det_scale = nullptr;
   int* d_seg_atom = nullptr; // atm row of each segmented shell
   // basis-parameter derivatives
   PrimAux* d_aux = nullptr;
@@ -683,6 +697,61 @@ __global__ void dmax_kernel(const double* __restrict__ D, int64_t ncc_tot, int n
   dmax[i] = m;
 }
 
+// Fixed-point scale of a deterministic build, one workgroup per molecule m:
+// every J/K partial element of m is bounded by
+//   B_J = Qmax sum_{S,T} Q(S,T) nS nT max|D_ST|       (Coulomb rows)
+//   B_K = (max_S sum_T Q(S,T) nT)^2 max|D|            (exchange rows)
+// (Cauchy-Schwarz |(ab|cd)| <= Q_ab Q_cd over Cartesian blocks); with 2^e >
+// 8 (B_J + B_K) the scale 2^(60-e) keeps every final integer below 2^60.  The
+// sums run in a fixed order (strided partial sums, then a fixed tree), so the
+// scale, like the build, does not depend on scheduling.
+__global__ __launch_bounds__(256) void det_scale_kernel(const double* __restrict__ dmax,
+                                                        const double* __restrict__ qblk,
+                                                        const MolAO* __restrict__ M,
+                                                        const double* __restrict__ mol,
+                                                        double* __restrict__ scale) {
+  __shared__ double sj[256], sm[256];
+  const int m = blockIdx.x;
+  const MolAO mo = M[m];
+  const int64_t cnt = (int64_t)mo.nseg * mo.nseg;
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+    const double d = dmax[mo.dbase + i];
+    a += qblk[mo.dbase + i] * d;
+    b = fmax(b, d);
+  }
+  sj[threadIdx.x] = a;
+  sm[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sj[threadIdx.x] += sj[threadIdx.x + w];
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double bound = 8.0 * (mol[2 * m] * sj[0] + mol[2 * m + 1] * mol[2 * m + 1] * sm[0]);
+    int e = (bound > 0.0 && isfinite(bound)) ? ilogb(bound) + 1 : 0;
+    e = max(-900, min(900, e));
+    scale[2 * m] = ldexp(1.0, 60 - e);
+    scale[2 * m + 1] = ldexp(1.0, e - 60);
+  }
+}
+
+// Deterministic builds: p[i] (a 64-bit integer, element `i` of a packed
+// buffer whose density stride is ncc_tot, every `step`-th slot used) back to
+// FP64 with its molecule's 1/scale.
+__global__ void fixed_to_double_kernel(double* __restrict__ p, int64_t count, int step, int64_t ncc_tot,
+                                       const MolAO* __restrict__ M, int nmol, const double* __restrict__ scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int64_t e = (i % (ncc_tot * step)) / step;
+  const int m = find_mol(M, nmol, e, 1);
+  const long long v = __double_as_longlong(p[i]);
+  p[i] = (double)v * scale[2 * m + 1];
+}
+
 static unsigned nblocks(size_t count) { return (unsigned)((count + 255) / 256); }
 
 // env_grad[exponent slot] += gexp; env_grad[coefficient slot] += fsp * gcoef
@@ -905,7 +974,8 @@ static void run_blocks(PlanData& P, const Pair* d_pairs, const std::vector<Block
   HIP_OK(hipFree(d_tasks));
 }
 
-static void compute_schwarz(PlanData& P, std::vector<double>& seg_q, hipStream_t s) {
+static void compute_schwarz(PlanData& P, std::vector<double>& seg_q, hipStream_t s,
+                            std::vector<double>* cart_q = nullptr) {
   const int npair = (int)P.all_pairs.size();
   // tasks sorted by pair class
   std::vector<int> order(npair);
@@ -939,6 +1009,7 @@ static void compute_schwarz(PlanData& P, std::vector<double>& seg_q, hipStream_t
   HIP_OK(hipMemcpy(h.data(), d_out, sizeof(double) * total, hipMemcpyDeviceToHost));
   HIP_OK(hipFree(d_out));
   seg_q.assign(npair, 0.0);
+  if (cart_q) cart_q->assign(npair, 0.0);
   for (int i = 0; i < npair; ++i) {
     const Pair& pr = P.all_pairs[i];
     const int la = P.seg[pr.sa].l, lb = P.seg[pr.sb].l;
@@ -946,6 +1017,12 @@ static void compute_schwarz(PlanData& P, std::vector<double>& seg_q, hipStream_t
     const double* B = h.data() + base[i];
     auto at = [&](int a, int b, int c, int d) { return B[((d * na + c) * nb + b) * na + a]; };
     double largest = 0.0;
+    if (cart_q) {
+      double lc = 0.0;
+      for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b) lc = std::max(lc, std::fabs(at(a, b, a, b)));
+      (*cart_q)[i] = lc;
+    }
     if (P.cart) {
       for (int a = 0; a < na; ++a)
         for (int b = 0; b < nb; ++b) largest = std::max(largest, std::fabs(at(a, b, a, b)));
@@ -1205,14 +1282,16 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             const int split = (wa * wb * wc * wd > 36) ? wc * wd : 1;
             const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb * kJBraCopies);
             const int nr = (qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1;
-            size_t pack = (size_t)tu.lds_pack;
-            if (tu.rys_lds && per_slot + rys_slice_bytes(nr) <= kLdsBudget) {
+            size_t pack = std::min<size_t>((size_t)tu.lds_pack, P.lds_budget);
+            if (tu.rys_lds && per_slot + rys_slice_bytes(nr) <= P.lds_budget) {
               qc.tab_lds = 1;
               qc.tab_bytes = rys_slice_bytes(nr);
               pack = pack > qc.tab_bytes + per_slot ? pack - qc.tab_bytes : per_slot;
             }
             qc.per_slot = per_slot;
-            qc.nbra_max = per_slot > kLdsBudget ? 0
+            // a slot wider than the packing budget still gets chunks of one
+            // bra when one density's slot fits the device's LDS
+            qc.nbra_max = per_slot > P.lds_max ? 0
                           : (int)std::max<size_t>(1, std::min<size_t>((size_t)kChunkBrasMax, pack / per_slot));
             for (size_t ti = 0; qc.nbra_max >= 1 && ti < pend.size(); ++ti) {
               Pending& pd = pend[ti];
@@ -1582,9 +1661,71 @@ static void assign_streams(PlanData& P) {
   }
 }
 
+// Schwarz tables of the deterministic scale (det_scale_kernel): Cartesian
+// bounds Q(S,T) = sqrt(max |(st|st)|) over the J/K shell set's blocks (a
+// contracted set's parent shell takes the maximum over its segments).
+static void build_det_tables(PlanData& P, hipStream_t s) {
+  std::vector<double> seg_q, cart_q;
+  compute_schwarz(P, seg_q, s, &cart_q);
+  const bool con = P.jk.contracted;
+  std::vector<double> qblk((size_t)std::max<int64_t>(P.nd_tot, 1), 0.0), mol(2 * (size_t)P.nmol, 0.0);
+  std::vector<std::vector<double>> q(P.nmol);
+  for (int m = 0; m < P.nmol; ++m) q[m].assign((size_t)P.mols[m].nseg * P.mols[m].nseg, 0.0);
+  for (size_t i = 0; i < P.all_pairs.size(); ++i) {
+    const Pair& pr = P.all_pairs[i];
+    const int m = P.seg[pr.sa].mol;
+    const MolAO& mo = P.mols[m];
+    const int a = (con ? P.seg[pr.sa].parent : pr.sa) - mo.seg0;
+    const int b = (con ? P.seg[pr.sb].parent : pr.sb) - mo.seg0;
+    const double v = std::sqrt(cart_q[i]);
+    double& x = q[m][(size_t)a * mo.nseg + b];
+    double& y = q[m][(size_t)b * mo.nseg + a];
+    x = std::max(x, v);
+    y = std::max(y, v);
+  }
+  for (int m = 0; m < P.nmol; ++m) {
+    const MolAO& mo = P.mols[m];
+    std::vector<double> w(mo.nseg);
+    for (int k = 0; k < mo.nseg; ++k) {
+      if (con) {
+        const int* row = &P.bas[(size_t)(mo.seg0 + k) * BAS_SLOTS];
+        w[k] = (double)ncart_h(row[ANG_OF]) * row[NCTR_OF];
+      } else {
+        w[k] = (double)ncart_h(P.seg[mo.seg0 + k].l);
+      }
+    }
+    double qmax = 0.0, rmax = 0.0;
+    for (int a = 0; a < mo.nseg; ++a) {
+      double row = 0.0;
+      for (int b = 0; b < mo.nseg; ++b) {
+        const double v = q[m][(size_t)a * mo.nseg + b];
+        qblk[(size_t)mo.dbase + (size_t)a * mo.nseg + b] = v * w[a] * w[b];
+        qmax = std::max(qmax, v);
+        row += v * w[b];
+      }
+      rmax = std::max(rmax, row);
+    }
+    mol[2 * m] = qmax;
+    mol[2 * m + 1] = rmax;
+  }
+  P.d_det_qblk = dev_upload(qblk, s);
+  P.d_det_mol = dev_upload(mol, s);
+  HIP_OK(hipMalloc(&P.d_det_scale, sizeof(double) * 2 * P.nmol));
+}
+
 static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
   if (!std::isfinite(cutoff) || cutoff < 0) throw InvalidArgument("Invalid Schwarz cutoff");
   P.cutoff = cutoff;
+  {
+    // LDS a one-density launch may take: the device's limit per workgroup
+    // (at most kLdsDeviceMax), lowered by the lds_limit option
+    int dev = 0, lmax = 0;
+    HIP_OK(hipGetDevice(&dev));
+    HIP_OK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    P.lds_max = std::min<size_t>(kLdsDeviceMax, (size_t)std::max(lmax, 0));
+    if (P.options.lds_limit > 0) P.lds_max = std::min<size_t>(P.lds_max, (size_t)P.options.lds_limit);
+    P.lds_budget = std::min(kLdsBudget, P.lds_max);
+  }
   build_segments(P);
   const int nseg = (int)P.seg.size();
   // shells on device
@@ -1701,6 +1842,7 @@ static void build_plan(PlanData& P, double cutoff, hipStream_t s) {
     P.d_molao = dev_upload(P.mols, s);
     HIP_OK(hipMalloc(&P.d_dmax, sizeof(double) * std::max<int64_t>(P.nd_tot, 1)));
   }
+  if (P.options.deterministic) build_det_tables(P, s);
   build_c2s_maps(P, s);
   {
     std::vector<int> order(npair);
@@ -1822,6 +1964,7 @@ static void free_plan(PlanData& P) {
   f(P.d_dc); f(P.d_jp); f(P.d_jpad); f(P.d_kp); f(P.d_flag); f(P.d_incore); f(P.d_part);
   f(P.d_pairs1e); f(P.d_nuclei);
   f(P.d_dmax); f(P.d_seg_atom);
+  f(P.d_det_qblk); f(P.d_det_mol); f(P.d_det_scale);
   f(P.d_molinfo); f(P.d_molao);
   f(P.d_aux); f(P.d_seg_prim0); f(P.d_prim_slots); f(P.d_prim_fsp); f(P.d_gparam);
   for (double* p : P.retired) f(p);
@@ -1860,8 +2003,26 @@ static void ensure_part(PlanData& P, size_t nbd) {
   P.part_nbd = nbd;
 }
 
+// Device buffers of one build: Cartesian densities, partial J and K, the
+// padded ket-side J and the partial slabs (NULL: flushes by atomics).
+struct Workspace {
+  double* dc;
+  double* jp;
+  double* jpad;
+  double* kp;
+  double* part;
+};
+
+// Workspace doubles of a build of nb densities without partial slabs, laid out
+// [dc | jp | kp] over nbd densities and jpad over nb (8 slots per element).
+static size_t workspace_doubles(const PlanData& P, int nb, int nbd) {
+  return (size_t)P.ncc_tot * (3 * (size_t)nbd + 8 * (size_t)nb);
+}
+
+// fixed: buffers supplied by the caller (no allocation in the build); NULL
+// grows the plan's own workspace and slabs as needed.
 static void jk_build(PlanData& P, const double* density, int nb, double* out, int with_j, int with_k,
-                     uint32_t flags, double mixed_threshold, hipStream_t s) {
+                     uint32_t flags, double mixed_threshold, hipStream_t s, const Workspace* fixed = nullptr) {
   const bool mixed = (flags & GSCF_FLAG_MIXED_PRECISION) != 0;
   if (mixed && !(std::isfinite(mixed_threshold) && mixed_threshold >= 0))
     throw InvalidArgument("mixed-precision threshold must be finite and non-negative");
@@ -1869,6 +2030,10 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     throw InvalidArgument("Direct J/K requires at least one selected operator");
   if (nb < 1) throw InvalidArgument("Direct J/K shape mismatch");
   if (!density || !out) throw InvalidArgument("Direct J/K requires device density and output buffers");
+  // deterministic digestion (engine option chosen at plan creation): integer
+  // accumulation in FP64 only, every flush by atomics, no partial slabs
+  const bool det = P.options.deterministic != 0;
+  if (det && mixed) throw InvalidArgument("mixed precision is not available in deterministic mode");
   const size_t nn = (size_t)P.nn_tot, ncc = (size_t)P.ncc_tot;
   if (!(flags & GSCF_FLAG_NO_VALIDATE)) {
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, P.d_flag, 1, 0);
@@ -1879,17 +2044,26 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
   }
   const bool sym = (flags & GSCF_FLAG_SYMMETRIC_DENSITY) != 0;
   const int nbd = (with_k && !sym) ? 2 * nb : nb;
-  ensure_workspace(P, nbd);
-  ensure_part(P, nbd);
-  hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, density, P.nn_tot, nb, P.d_dc,
+  Workspace W;
+  if (fixed) {
+    W = *fixed;
+  } else {
+    ensure_workspace(P, nbd);
+    ensure_part(P, nbd);
+    W = {P.d_dc, P.d_jp, P.d_jpad, P.d_kp, P.d_part};
+  }
+  hipLaunchKernelGGL(dens_to_cart_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, density, P.nn_tot, nb, W.dc,
                      P.ncc_tot, P.d_molao, P.nmol, P.d_c2s_idx, P.d_c2s_coef, nbd);
-  if (mixed)
-    hipLaunchKernelGGL(dmax_kernel, dim3(nblocks((size_t)P.nd_tot)), dim3(256), 0, s, P.d_dc, P.ncc_tot, nbd,
+  if (mixed || det)
+    hipLaunchKernelGGL(dmax_kernel, dim3(nblocks((size_t)P.nd_tot)), dim3(256), 0, s, W.dc, P.ncc_tot, nbd,
                        P.d_molao, P.nmol, P.nd_tot, P.jk.d_blk, P.d_dmax);
-  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_jp, ncc * nbd, 0.0);
+  if (det)
+    hipLaunchKernelGGL(det_scale_kernel, dim3((unsigned)P.nmol), dim3(256), 0, s, P.d_dmax, P.d_det_qblk,
+                       P.d_molao, P.d_det_mol, P.d_det_scale);
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, W.jp, ncc * nbd, 0.0);
   if (with_j)
-    hipLaunchKernelGGL(fill_kernel, dim3(nblocks(8 * ncc * nb)), dim3(256), 0, s, P.d_jpad, 8 * ncc * nb, 0.0);
-  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, P.d_kp, ncc * nbd, 0.0);
+    hipLaunchKernelGGL(fill_kernel, dim3(nblocks(8 * ncc * nb)), dim3(256), 0, s, W.jpad, 8 * ncc * nb, 0.0);
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, W.kp, ncc * nbd, 0.0);
   const PairSet& S = P.jk;
   if (P.prof) ++P.prof_builds;
   // Class launches: in list order on the caller's stream while profiling (so
@@ -1899,7 +2073,8 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
   // with several streams each folds the exchange-row slabs of its own
   // classes right after them (device-scope adds into K), overlapping the
   // other streams' class kernels; otherwise one reduction after the join
-  const bool lane_reduce = fork && with_k && (P.options.partial_sums & 1) && S.part_total > 0 &&
+  const int psums = (det || !W.part) ? 0 : P.options.partial_sums;
+  const bool lane_reduce = fork && with_k && (psums & 1) && S.part_total > 0 &&
                            P.d_lane_krp.size() == P.lane_classes.size();
   if (fork) {
     HIP_OK(hipEventRecord(P.fork_ev, s));
@@ -1930,11 +2105,13 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     const int wa = ncart_h(qc.la) * (1 + (qc.pb & 1)), wb = ncart_h(qc.lb) * (1 + (qc.pb >> 1));
     const int wc = ncart_h(qc.lc) * (1 + (qc.pk & 1)), wd = ncart_h(qc.ld) * (1 + (qc.pk >> 1));
     BraArgs b;
+    std::memset(&b, 0, sizeof(b));
+    b.det_scale = det ? P.d_det_scale : nullptr;
     b.mode = 0;
     b.cache = nullptr;
     b.part_mode = 0;
     b.part_stride = S.part_total;
-    b.part = P.d_part;
+    b.part = W.part;
     if (P.d_incore && qc.cache_doubles) {
       b.mode = 2;
       b.cache = P.d_incore + qc.cache_off;
@@ -1956,47 +2133,74 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     b.pair_q = S.d_pair_q;
     b.dmax = P.d_dmax;
     b.threads = qc.threads;
-    // LDS of one launch over nbd_l exchange densities and nbj_l Coulomb densities
-    auto layout = [&](int nbd_l, int nbj_l, size_t& lds, int& nket_max, int& tab_lds) {
+    // LDS of one launch over nbd_l exchange densities and nbj_l Coulomb
+    // densities; the ket window and the Rys table only where they fit `cap`
+    auto layout = [&](int nbd_l, int nbj_l, size_t cap, size_t& lds, int& nket_max, int& tab_lds) {
       lds = sizeof(double) * ((size_t)nbd_l * qc.nbra_max * (wa + wb) * P.nc_max +
                               (size_t)nbj_l * qc.nbra_max * wa * wb * kJBraCopies);
       nket_max = with_j ? qc.nket_max : 0;
       const size_t jket_bytes = sizeof(double) * (size_t)nbj_l * nket_max * wc * wd;
-      if (nket_max && lds + jket_bytes <= kLdsBudget) lds += jket_bytes; else nket_max = 0;
-      tab_lds = b.mode == 2 ? 0 : qc.tab_lds;  // stored blocks need no Rys table
-      if (tab_lds && lds + qc.tab_bytes <= kLdsBudget) lds += qc.tab_bytes; else tab_lds = 0;
+      if (nket_max && lds + jket_bytes <= cap) lds += jket_bytes; else nket_max = 0;
+      tab_lds = (b.mode == 2 || det) ? 0 : qc.tab_lds;  // stored blocks need no Rys table
+      if (tab_lds && lds + qc.tab_bytes <= cap) lds += qc.tab_bytes; else tab_lds = 0;
     };
     const int nbd_eff = with_k ? nbd : nb;
     size_t lds;
     int nket_max, tab_lds;
-    layout(nbd_eff, nb, lds, nket_max, tab_lds);
+    layout(nbd_eff, nb, P.lds_budget, lds, nket_max, tab_lds);
+    // every density in one launch within the packing budget; otherwise one
+    // launch per density (up to the device's LDS); otherwise, for segmented
+    // classes only, the global-atomic kernels
+    const bool bra_path = qc.d_chunks && lds <= P.lds_budget;
+    size_t lds1 = 0;
+    int nket1 = 0, tab1 = 0;
+    if (!bra_path && qc.d_chunks) layout(1, 1, P.lds_max, lds1, nket1, tab1);
+    const bool per_density = !bra_path && qc.d_chunks && lds1 <= P.lds_max;
     // partial slabs: a slab range this build does not write (ket windows not
     // used, or the class not run by the bra-major kernel) is cleared first,
     // since the reductions read every listed slab entry
-    const bool bra_path = qc.d_chunks && lds <= kLdsBudget;
     if (qc.part) {
-      const int pm = P.options.partial_sums;
+      const int pm = psums;
       const bool need_k = with_k && (pm & 1), need_j = with_j && (pm & 2);
       if (bra_path) b.part_mode = (need_k ? 1 : 0) | ((need_j && nket_max > 0) ? 2 : 0);
       if ((need_k && !(b.part_mode & 1)) || (need_j && !(b.part_mode & 2)))
         for (int bt = 0; bt < nbd_eff; ++bt)
           hipLaunchKernelGGL(fill_kernel, dim3(nblocks((size_t)(qc.part_end - qc.part_beg))), dim3(256), 0, s,
-                             P.d_part + (size_t)bt * S.part_total + qc.part_beg,
+                             W.part + (size_t)bt * S.part_total + qc.part_beg,
                              (size_t)(qc.part_end - qc.part_beg), 0.0);
     }
     if (bra_path) {
-      b.dens = P.d_dc;
-      b.jp = P.d_jp;
-      b.jket_pad = P.d_jpad;
-      b.kp = P.d_kp;
+      b.dens = W.dc;
+      b.jp = W.jp;
+      b.jket_pad = W.jpad;
+      b.kp = W.kp;
       b.bstride = P.ncc_tot;
       b.nbd = nbd_eff;
       b.nbj = nb;
       b.tab_lds = tab_lds;
       b.nket_max = nket_max;
       launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, lds, s);
+    } else if (per_density) {
+      // one launch per density (J and K for D_b, K only for D_b^T of a
+      // nonsymmetric K), each within the device's LDS per workgroup
+      for (int bt = 0; bt < nbd_eff; ++bt) {
+        b.dens = W.dc + (size_t)bt * ncc;
+        b.jp = W.jp + (size_t)bt * ncc;
+        b.jket_pad = W.jpad + 8 * (size_t)bt * ncc;
+        b.kp = W.kp + (size_t)bt * ncc;
+        b.bstride = P.ncc_tot;
+        b.nbd = 1;
+        b.nbj = 1;
+        b.with_j = with_j && bt < nb;
+        b.tab_lds = tab1;
+        b.nket_max = b.with_j ? nket1 : 0;
+        if (!b.with_j && !b.with_k) continue;
+        launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, lds1, s);
+      }
     } else if (!qc.pb && !qc.pk) {
       JKArgs a;
+      std::memset(&a, 0, sizeof(a));
+      a.det_scale = det ? P.d_det_scale : nullptr;
       a.pairs = S.d_pairs;
       a.prims = S.d_prims;
       a.shells = S.d_shells;
@@ -2004,9 +2208,9 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       a.tasks = qc.d_tasks;
       a.ntasks = (int)qc.tasks.size();
       a.nitems = qc.nitems;
-      a.dens = P.d_dc;
-      a.jp = P.d_jp;
-      a.kp = P.d_kp;
+      a.dens = W.dc;
+      a.jp = W.jp;
+      a.kp = W.kp;
       a.mols = P.d_molinfo;
       a.bstride = P.ncc_tot;
       a.n = P.nc_max;
@@ -2016,25 +2220,9 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       a.with_k = with_k;
       launch_jk_class(qc.la, qc.lb, qc.lc, qc.ld, a, s);
     } else {
-      // general-contraction class whose slabs do not fit for all densities at
-      // once: one launch per density (J and K for D_b, K only for D_b^T of a
-      // nonsymmetric K), which the plan guarantees fits in LDS
-      layout(1, 1, lds, nket_max, tab_lds);
-      if (!qc.d_chunks || lds > kLdsBudget) throw Internal("general-contraction class does not fit in LDS");
-      for (int bt = 0; bt < nbd_eff; ++bt) {
-        b.dens = P.d_dc + (size_t)bt * ncc;
-        b.jp = P.d_jp + (size_t)bt * ncc;
-        b.jket_pad = P.d_jpad + 8 * (size_t)bt * ncc;
-        b.kp = P.d_kp + (size_t)bt * ncc;
-        b.bstride = P.ncc_tot;
-        b.nbd = 1;
-        b.nbj = 1;
-        b.with_j = with_j && bt < nb;
-        b.tab_lds = tab_lds;
-        b.nket_max = b.with_j ? nket_max : 0;
-        if (!b.with_j && !b.with_k) continue;
-        launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, lds, s);
-      }
+      // general-contraction classes have no global-atomic kernel (the plan
+      // keeps them segmented when one density's slot does not fit)
+      throw Internal("general-contraction class does not fit in LDS");
     }
     if (P.prof) {
       HIP_OK(hipEventRecord(P.ev_pool[P.ev_used + 1], s));
@@ -2044,8 +2232,8 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
   if (lane_reduce)
     for (size_t l = 0; l < P.lane_classes.size(); ++l)
       hipLaunchKernelGGL(kpart_reduce_kernel, dim3((unsigned)S.nrows, (unsigned)((S.nc_max + 63) / 64),
-                         (unsigned)nbd), dim3(64), 0, l == 0 ? s : P.side[l - 1], P.d_part, S.part_total,
-                         P.d_lane_krp[l], P.d_lane_kent[l], S.d_row_dst, S.d_row_n, P.ncc_tot, 1, P.d_kp);
+                         (unsigned)nbd), dim3(64), 0, l == 0 ? s : P.side[l - 1], W.part, S.part_total,
+                         P.d_lane_krp[l], P.d_lane_kent[l], S.d_row_dst, S.d_row_n, P.ncc_tot, 1, W.kp);
   if (fork) {
     for (size_t l = 0; l < P.side.size(); ++l) {
       HIP_OK(hipEventRecord(P.join_ev[l], P.side[l]));
@@ -2054,17 +2242,27 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
   }
   if (S.part_total > 0) {
     const int nbd_k = with_k ? nbd : nb;
-    if (with_k && (P.options.partial_sums & 1) && !lane_reduce)
+    if (with_k && (psums & 1) && !lane_reduce)
       hipLaunchKernelGGL(kpart_reduce_kernel, dim3((unsigned)S.nrows, (unsigned)((S.nc_max + 63) / 64),
-                         (unsigned)nbd_k), dim3(64), 0, s, P.d_part, S.part_total, S.d_krp, S.d_kent,
-                         S.d_row_dst, S.d_row_n, P.ncc_tot, 0, P.d_kp);
-    if (with_j && (P.options.partial_sums & 2))
+                         (unsigned)nbd_k), dim3(64), 0, s, W.part, S.part_total, S.d_krp, S.d_kent,
+                         S.d_row_dst, S.d_row_n, P.ncc_tot, 0, W.kp);
+    if (with_j && (psums & 2))
       hipLaunchKernelGGL(jpart_reduce_kernel, dim3((unsigned)S.pairs.size(), (unsigned)nb), dim3(64), 0, s,
-                         P.d_part, S.part_total, S.d_jrp, S.d_jent, S.d_pairs, S.d_shells, P.d_molinfo,
-                         P.ncc_tot, P.d_jpad);
+                         W.part, S.part_total, S.d_jrp, S.d_jent, S.d_pairs, S.d_shells, P.d_molinfo,
+                         P.ncc_tot, W.jpad);
+  }
+  if (det) {
+    // integers back to FP64 (Jp / Kp over nbd densities, ket-side J over nb)
+    hipLaunchKernelGGL(fixed_to_double_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, W.jp,
+                       (int64_t)(ncc * nbd), 1, P.ncc_tot, P.d_molao, P.nmol, P.d_det_scale);
+    hipLaunchKernelGGL(fixed_to_double_kernel, dim3(nblocks(ncc * nbd)), dim3(256), 0, s, W.kp,
+                       (int64_t)(ncc * nbd), 1, P.ncc_tot, P.d_molao, P.nmol, P.d_det_scale);
+    if (with_j)
+      hipLaunchKernelGGL(fixed_to_double_kernel, dim3(nblocks(8 * ncc * nb)), dim3(256), 0, s, W.jpad,
+                         (int64_t)(8 * ncc * nb), 8, P.ncc_tot, P.d_molao, P.nmol, P.d_det_scale);
   }
   hipLaunchKernelGGL(finalize_kernel, dim3(nblocks((size_t)(with_j + with_k) * nb * nn)), dim3(256), 0, s,
-                     P.d_jp, P.d_jpad, P.d_kp, P.ncc_tot, P.nn_tot, nb, sym ? 1 : 0, with_j, with_k, P.d_molao,
+                     W.jp, W.jpad, W.kp, P.ncc_tot, P.nn_tot, nb, sym ? 1 : 0, with_j, with_k, P.d_molao,
                      P.nmol, P.d_s2c_idx, P.d_s2c_coef, out);
   HIP_OK(hipGetLastError());
 }
@@ -2659,6 +2857,9 @@ int gscf_options_set(const gscf_options* o) {
       throw InvalidArgument("general_contraction must be 0 or 1");
     if (o->streams < 1 || o->streams > kMaxStreams) throw InvalidArgument("streams must be in 1..4");
     if (o->partial_sums < 0 || o->partial_sums > 3) throw InvalidArgument("partial_sums must be in 0..3");
+    if (o->lds_limit != 0 && (o->lds_limit < 4096 || o->lds_limit > (int32_t)kLdsDeviceMax))
+      throw InvalidArgument("lds_limit must be 0 (the device's limit) or in 4096..163840 bytes");
+    if (o->deterministic != 0 && o->deterministic != 1) throw InvalidArgument("deterministic must be 0 or 1");
     g_options = *o;
   });
 }
@@ -2717,6 +2918,10 @@ int gscf_plan_get_info(const gscf_plan* plan, gscf_plan_info* info) {
     info->nc_max = P.nc_max;
     info->nn_total = P.nn_tot;
     info->device_bytes = plan_bytes(P, (int64_t)P.ws_nbd) + (int64_t)(sizeof(double) * P.incore_doubles);
+    info->lds_max = (int64_t)P.lds_max;
+    int fallback = 0;
+    for (const QuartetClass& qc : P.jk.classes) fallback += qc.d_chunks ? 0 : 1;
+    info->classes_global_atomic = fallback;
   });
 }
 

@@ -209,9 +209,54 @@ __device__ __forceinline__ void load_geom(const ShellXYZ& a, const ShellXYZ& b, 
 }
 
 // ----------------------------------------------------------------------------
+// Digestion adds.  FP64 form: atomic adds (LDS or device scope).
+// Deterministic form: the value, already multiplied by its molecule's
+// power-of-two scale, is rounded to a 64-bit integer and added as one;
+// integer sums do not depend on the order of the additions, so every build
+// of the same density gives the same bits.  Buffers hold the integers until
+// a conversion pass turns them back into FP64.
+// ----------------------------------------------------------------------------
+template <bool DET>
+__device__ __forceinline__ void acc_add(double* p, double v) {
+  if constexpr (DET)
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(__double2ll_rn(v)));
+  else
+    atomicAdd(p, v);
+}
+
+// workgroup-scope add into the LDS ket window
+template <bool DET>
+__device__ __forceinline__ void acc_add_wg(double* p, double v) {
+  if constexpr (DET)
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long*>(p),
+                           static_cast<unsigned long long>(__double2ll_rn(v)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  else
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// an accumulated value (FP64, or the deterministic form's integer) is zero
+template <bool DET>
+__device__ __forceinline__ bool acc_zero(const double* p) {
+  if constexpr (DET)
+    return *reinterpret_cast<const unsigned long long*>(p) == 0ull;
+  else
+    return *p == 0.0;
+}
+
+// adds an accumulated value into global memory with a device-scope atomic
+template <bool DET>
+__device__ __forceinline__ void acc_flush(double* dst, const double* src) {
+  if constexpr (DET)
+    atomicAdd(reinterpret_cast<unsigned long long*>(dst), *reinterpret_cast<const unsigned long long*>(src));
+  else
+    atomicAdd(dst, *src);
+}
+
+// ----------------------------------------------------------------------------
 // J/K kernels
 // ----------------------------------------------------------------------------
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool DET = false>
 __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
   using K = Class<LA, LB, LC, LD>;
   const int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -244,7 +289,7 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
       accumulate_full<LA, LB, LC, LD>(q, AB, CD, acc);
     }
   }
-  double f = 1.0;
+  double f = DET ? a.det_scale[2 * sA.mol] : 1.0;
   if (bp.sa == bp.sb) f *= 0.5;
   if (kp.sa == kp.sb) f *= 0.5;
   if (qk.x == qk.y) f *= 0.5;
@@ -275,7 +320,7 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
           for (int ic = 0; ic < K::NC; ++ic)
 #pragma unroll
             for (int id = 0; id < K::ND; ++id) s += acc[ia][ib][ic][id] * dcd[ic][id];
-          atomicAdd(J + (size_t)(oa + ia) * n + ob + ib, f * s);
+          acc_add<DET>(J + (size_t)(oa + ia) * n + ob + ib, f * s);
         }
 #pragma unroll
       for (int ic = 0; ic < K::NC; ++ic)
@@ -286,7 +331,7 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
           for (int ia = 0; ia < K::NA; ++ia)
 #pragma unroll
             for (int ib = 0; ib < K::NB; ++ib) s += acc[ia][ib][ic][id] * dab[ia][ib];
-          atomicAdd(J + (size_t)(oc + ic) * n + od + id, f * s);
+          acc_add<DET>(J + (size_t)(oc + ic) * n + od + id, f * s);
         }
     }
     if (a.with_k) {
@@ -315,7 +360,7 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
           for (int ib = 0; ib < K::NB; ++ib)
 #pragma unroll
             for (int id = 0; id < K::ND; ++id) s += acc[ia][ib][ic][id] * dbd[ib][id];
-          atomicAdd(Kp + (size_t)(oa + ia) * n + oc + ic, f * s);
+          acc_add<DET>(Kp + (size_t)(oa + ia) * n + oc + ic, f * s);
         }
 #pragma unroll
       for (int ia = 0; ia < K::NA; ++ia)
@@ -326,7 +371,7 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
           for (int ib = 0; ib < K::NB; ++ib)
 #pragma unroll
             for (int ic = 0; ic < K::NC; ++ic) s += acc[ia][ib][ic][id] * dbc[ib][ic];
-          atomicAdd(Kp + (size_t)(oa + ia) * n + od + id, f * s);
+          acc_add<DET>(Kp + (size_t)(oa + ia) * n + od + id, f * s);
         }
 #pragma unroll
       for (int ib = 0; ib < K::NB; ++ib)
@@ -337,7 +382,7 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
           for (int ia = 0; ia < K::NA; ++ia)
 #pragma unroll
             for (int id = 0; id < K::ND; ++id) s += acc[ia][ib][ic][id] * dad[ia][id];
-          atomicAdd(Kp + (size_t)(ob + ib) * n + oc + ic, f * s);
+          acc_add<DET>(Kp + (size_t)(ob + ib) * n + oc + ic, f * s);
         }
 #pragma unroll
       for (int ib = 0; ib < K::NB; ++ib)
@@ -348,13 +393,13 @@ __global__ __launch_bounds__(256) void jk_full_kernel(JKArgs a) {
           for (int ia = 0; ia < K::NA; ++ia)
 #pragma unroll
             for (int ic = 0; ic < K::NC; ++ic) s += acc[ia][ib][ic][id] * dac[ia][ic];
-          atomicAdd(Kp + (size_t)(ob + ib) * n + od + id, f * s);
+          acc_add<DET>(Kp + (size_t)(ob + ib) * n + od + id, f * s);
         }
     }
   }
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, bool DET = false>
 __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
   using K = Class<LA, LB, LC, LD>;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -392,7 +437,7 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
       accumulate_split<LA, LB, LC, LD>(q, AB, wk, acc);
     }
   }
-  double f = 1.0;
+  double f = DET ? a.det_scale[2 * sA.mol] : 1.0;
   if (bp.sa == bp.sb) f *= 0.5;
   if (kp.sa == kp.sb) f *= 0.5;
   if (qk.x == qk.y) f *= 0.5;
@@ -410,10 +455,10 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
       for (int ia = 0; ia < K::NA; ++ia)
 #pragma unroll
         for (int ib = 0; ib < K::NB; ++ib) {
-          atomicAdd(J + (size_t)(oa + ia) * n + ob + ib, f * acc[ia][ib] * dcd);
+          acc_add<DET>(J + (size_t)(oa + ia) * n + ob + ib, f * acc[ia][ib] * dcd);
           s2 += acc[ia][ib] * (Dm[(size_t)(oa + ia) * n + ob + ib] + Dm[(size_t)(ob + ib) * n + oa + ia]);
         }
-      atomicAdd(J + (size_t)c * n + d, f * s2);
+      acc_add<DET>(J + (size_t)c * n + d, f * s2);
     }
     if (a.with_k) {
       double* Kp = a.kp + (size_t)bt * a.bstride + mi.base;
@@ -431,8 +476,8 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
           sc += acc[ia][ib] * db_d[ib];
           sd += acc[ia][ib] * db_c[ib];
         }
-        atomicAdd(Kp + (size_t)(oa + ia) * n + c, f * sc);
-        atomicAdd(Kp + (size_t)(oa + ia) * n + d, f * sd);
+        acc_add<DET>(Kp + (size_t)(oa + ia) * n + c, f * sc);
+        acc_add<DET>(Kp + (size_t)(oa + ia) * n + d, f * sd);
       }
       double da_d[K::NA], da_c[K::NA];
 #pragma unroll
@@ -448,8 +493,8 @@ __global__ __launch_bounds__(256) void jk_split_kernel(JKArgs a) {
           sc += acc[ia][ib] * da_d[ia];
           sd += acc[ia][ib] * da_c[ia];
         }
-        atomicAdd(Kp + (size_t)(ob + ib) * n + c, f * sc);
-        atomicAdd(Kp + (size_t)(ob + ib) * n + d, f * sd);
+        acc_add<DET>(Kp + (size_t)(ob + ib) * n + c, f * sc);
+        acc_add<DET>(Kp + (size_t)(ob + ib) * n + d, f * sd);
       }
     }
   }
@@ -502,11 +547,14 @@ struct Wide {
 // with its coefficient product c_A c_B c_C c_D.
 // MODE 1 also stores the block (after the lane-group sum) at cq, MODE 2 reads
 // it from cq instead of evaluating it (in-core integrals).
-template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool MOMENTS, typename R, int MODE = 0>
+// DET: deterministic digestion (integer adds of values scaled by dscale).
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool MOMENTS, typename R, int MODE = 0,
+          bool DET = false>
 __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, const Pair& bp, const Pair& kp,
                                          const int2 qk, int slot, int s0, int S, int kc, double* __restrict__ krow,
                                          double* __restrict__ jrow, double* __restrict__ jket, int kwin0,
-                                         const double* __restrict__ tab, double* __restrict__ cq) {
+                                         const double* __restrict__ tab, double* __restrict__ cq,
+                                         double dscale = 1.0) {
   using K = Class<LA, LB, LC, LD>;
   using W = Wide<LA, LB, LC, LD, CA, CB, CC, CD>;
   constexpr int WA = W::WA, WB = W::WB, WC = W::WC, WD = W::WD;
@@ -525,7 +573,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
   load_geom(sA, sB, sC, sD, geo, ABd, CDd);
   const R AB[3] = {(R)ABd[0], (R)ABd[1], (R)ABd[2]};
   const R CD_[3] = {(R)CDd[0], (R)CDd[1], (R)CDd[2]};
-  double f = 1.0;
+  double f = DET ? dscale : 1.0;
   if (bp.sa == bp.sb) f *= 0.5;
   if (kp.sa == kp.sb) f *= 0.5;
   if (qk.x == qk.y) f *= 0.5;
@@ -625,7 +673,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             for (int ic = 0; ic < WC; ++ic)
 #pragma unroll
               for (int id = 0; id < WD; ++id) sj += (double)acc[ia][ib][ic][id] * dcd[ic][id];
-            atomicAdd(jr + ia * WB + ib, f * sj);
+            acc_add<DET>(jr + ia * WB + ib, f * sj);
           }
 #pragma unroll
         for (int ic = 0; ic < WC; ++ic)
@@ -639,10 +687,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             // (workgroup scope for the LDS window keeps the two atomics from
             // being merged into one generic-address atomic)
             if (jket)
-              __hip_atomic_fetch_add(jket + (bt * a.nket_max + (qk.y - kwin0)) * W::NKET + ic * WD + id, f * sj,
-                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+              acc_add_wg<DET>(jket + (bt * a.nket_max + (qk.y - kwin0)) * W::NKET + ic * WD + id, f * sj);
             else
-              atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base) + 8u * (rc + ic * n + od + id), f * sj);
+              acc_add<DET>(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base) + 8u * (rc + ic * n + od + id), f * sj);
           }
       }
       if (a.with_k) {
@@ -671,7 +718,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             for (int ib = 0; ib < WB; ++ib)
 #pragma unroll
               for (int id = 0; id < WD; ++id) sk += (double)acc[ia][ib][ic][id] * dbd[ib][id];
-            atomicAdd(kr + ia * ns + oc + ic, f * sk);
+            acc_add<DET>(kr + ia * ns + oc + ic, f * sk);
           }
 #pragma unroll
           for (int id = 0; id < WD; ++id) {
@@ -680,7 +727,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             for (int ib = 0; ib < WB; ++ib)
 #pragma unroll
               for (int ic = 0; ic < WC; ++ic) sk += (double)acc[ia][ib][ic][id] * dbc[ib][ic];
-            atomicAdd(kr + ia * ns + od + id, f * sk);
+            acc_add<DET>(kr + ia * ns + od + id, f * sk);
           }
         }
 #pragma unroll
@@ -692,7 +739,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
               for (int id = 0; id < WD; ++id) sk += (double)acc[ia][ib][ic][id] * dad[ia][id];
-            atomicAdd(kr + (WA + ib) * ns + oc + ic, f * sk);
+            acc_add<DET>(kr + (WA + ib) * ns + oc + ic, f * sk);
           }
 #pragma unroll
           for (int id = 0; id < WD; ++id) {
@@ -701,7 +748,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
               for (int ic = 0; ic < WC; ++ic) sk += (double)acc[ia][ib][ic][id] * dac[ia][ic];
-            atomicAdd(kr + (WA + ib) * ns + od + id, f * sk);
+            acc_add<DET>(kr + (WA + ib) * ns + od + id, f * sk);
           }
         }
       }
@@ -779,14 +826,13 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
         for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
           for (int ib = 0; ib < WB; ++ib) {
-            atomicAdd(jr + ia * WB + ib, f * (double)acc[ia][ib] * dcd);
+            acc_add<DET>(jr + ia * WB + ib, f * (double)acc[ia][ib] * dcd);
             s2 += (double)acc[ia][ib] * (ld_u(Dm, ra + ia * n + ob + ib) + ld_u(Dm, rb + ib * n + oa + ia));
           }
         if (jket)
-          __hip_atomic_fetch_add(jket + (bt * a.nket_max + (qk.y - kwin0)) * W::NKET + kc, f * s2, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          acc_add_wg<DET>(jket + (bt * a.nket_max + (qk.y - kwin0)) * W::NKET + kc, f * s2);
         else
-          atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base) + 8u * (c * n + d), f * s2);
+          acc_add<DET>(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base) + 8u * (c * n + d), f * s2);
       }
       if (a.with_k) {
         double* kr = krow + (bt * a.nbra_max + slot) * ROWS * ns;
@@ -804,8 +850,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             sc += (double)acc[ia][ib] * db_d[ib];
             sd += (double)acc[ia][ib] * db_c[ib];
           }
-          atomicAdd(kr + ia * ns + c, f * sc);
-          atomicAdd(kr + ia * ns + d, f * sd);
+          acc_add<DET>(kr + ia * ns + c, f * sc);
+          acc_add<DET>(kr + ia * ns + d, f * sd);
         }
         double da_d[WA], da_c[WA];
 #pragma unroll
@@ -821,8 +867,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             sc += (double)acc[ia][ib] * da_d[ia];
             sd += (double)acc[ia][ib] * da_c[ia];
           }
-          atomicAdd(kr + (WA + ib) * ns + c, f * sc);
-          atomicAdd(kr + (WA + ib) * ns + d, f * sd);
+          acc_add<DET>(kr + (WA + ib) * ns + c, f * sc);
+          acc_add<DET>(kr + (WA + ib) * ns + d, f * sd);
         }
       }
     }
@@ -848,8 +894,13 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
 // evaluation read LDS instead of global memory.
 // MODE (in-core integrals): 0 evaluates, 1 evaluates and stores the blocks,
 // 2 digests stored blocks (one lane per quartet component: no primitive split).
-template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool TLDS, bool MIX, int MODE = 0>
+// DET: deterministic digestion -- every LDS and global accumulator holds a
+// 64-bit integer (value x the molecule's scale, a.det_scale[2 mol]) added by
+// integer atomics, so the result does not depend on the order of the adds.
+template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD, bool TLDS, bool MIX, int MODE = 0,
+          bool DET = false>
 __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
+  static_assert(!(MIX && DET), "the deterministic form is FP64 only");
   using K = Class<LA, LB, LC, LD>;
   using W = Wide<LA, LB, LC, LD, CA, CB, CC, CD>;
   constexpr int ROWS = W::WA + W::WB;
@@ -859,8 +910,11 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   // every quartet of a chunk belongs to one molecule (runs never cross
   // molecules); its layout is workgroup-uniform, kept in scalar registers
   MolInfo mi;
+  double dscale = 1.0;
   {
-    const MolInfo m = a.mols[a.shells[a.pairs[ch.bra0].sa].mol];
+    const int mol = a.shells[a.pairs[ch.bra0].sa].mol;
+    if constexpr (DET) dscale = a.det_scale[2 * mol];
+    const MolInfo m = a.mols[mol];
     mi.base = uniform_i64(m.base);
     mi.dbase = uniform_i64(m.dbase);
     mi.n = __builtin_amdgcn_readfirstlane(m.n);
@@ -935,8 +989,8 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
         continue;
       }
     }
-    bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double, MODE>(a, mi, bp, kp, qk, slot, s0, S, kc, krow, jrow,
-                                                                      jket, ch.ket0, tab, cq);
+    bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double, MODE, DET>(a, mi, bp, kp, qk, slot, s0, S, kc, krow,
+                                                                           jrow, jket, ch.ket0, tab, cq, dscale);
   }
   __syncthreads();
   if (a.part_mode & 1) {
@@ -957,8 +1011,7 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   } else
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
   for (int i = threadIdx.x; i < krow_size; i += blockDim.x) {
-    const double v = krow[i];
-    if (v == 0.0) continue;
+    if (acc_zero<DET>(krow + i)) continue;
     const int col = i % ns;
     const int r = (i / ns) % ROWS;
     const int slot = (i / (ns * ROWS)) % a.nbra_max;
@@ -967,7 +1020,7 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     const Pair bp = a.pairs[ch.bra0 + slot];
     const MolInfo mi = a.mols[a.shells[bp.sa].mol];
     const int row = r < W::WA ? a.shells[bp.sa].ao + r : a.shells[bp.sb].ao + (r - W::WA);
-    atomicAdd(a.kp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
+    acc_flush<DET>(a.kp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, krow + i);
   }
   for (int i = threadIdx.x; i < jrow_size / kJBraCopies; i += blockDim.x) {
     const int e = i % W::NBRA;
@@ -975,13 +1028,23 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     const int bt = i / (W::NBRA * a.nbra_max);
     if (slot >= ch.nbra) continue;
     double v = 0.0;
+    if constexpr (DET) {
+      // the copies' integers add with wrap-around, like the atomics that filled them
+      unsigned long long u = 0;
 #pragma unroll
-    for (int c = 0; c < kJBraCopies; ++c) v += jrow[((bt * a.nbra_max + slot) * kJBraCopies + c) * W::NBRA + e];
-    if (v == 0.0) continue;
+      for (int c = 0; c < kJBraCopies; ++c)
+        u += reinterpret_cast<const unsigned long long*>(jrow)[((bt * a.nbra_max + slot) * kJBraCopies + c) *
+                                                                W::NBRA + e];
+      v = __longlong_as_double((long long)u);
+    } else {
+#pragma unroll
+      for (int c = 0; c < kJBraCopies; ++c) v += jrow[((bt * a.nbra_max + slot) * kJBraCopies + c) * W::NBRA + e];
+    }
+    if (acc_zero<DET>(&v)) continue;
     const Pair bp = a.pairs[ch.bra0 + slot];
     const MolInfo mi = a.mols[a.shells[bp.sa].mol];
     const int row = a.shells[bp.sa].ao + e / W::WB, col = a.shells[bp.sb].ao + e % W::WB;
-    atomicAdd(a.jp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, v);
+    acc_flush<DET>(a.jp + (size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col, &v);
   }
   if (jket && (a.part_mode & 2)) {
     // ket window to the partial slab after the exchange rows
@@ -998,12 +1061,12 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
       const int e = i % W::NKET;
       const int w = (i / W::NKET) % ch.nket;
       const int bt = i / (W::NKET * ch.nket);
-      const double v = jket[((size_t)bt * a.nket_max + w) * W::NKET + e];
-      if (v == 0.0) continue;
+      const double* src = jket + ((size_t)bt * a.nket_max + w) * W::NKET + e;
+      if (acc_zero<DET>(src)) continue;
       const Pair kp = a.pairs[ch.ket0 + w];
       const MolInfo mi = a.mols[a.shells[kp.sa].mol];
       const int row = a.shells[kp.sa].ao + e / W::WD, col = a.shells[kp.sb].ao + e % W::WD;
-      atomicAdd(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col), v);
+      acc_flush<DET>(a.jket_pad + 8 * ((size_t)bt * a.bstride + mi.base + (size_t)row * mi.n + col), src);
     }
   }
 }
@@ -1065,10 +1128,17 @@ void launch_jk_t(const JKArgs& a, hipStream_t s) {
   if (a.nitems <= 0) return;
   const int64_t threads = a.nitems * K::SPLIT;
   const int64_t blocks = (threads + 255) / 256;
-  if constexpr (K::kSplit)
-    hipLaunchKernelGGL((jk_split_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL((jk_full_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+  if constexpr (K::kSplit) {
+    if (a.det_scale)
+      hipLaunchKernelGGL((jk_split_kernel<LA, LB, LC, LD, true>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((jk_split_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+  } else {
+    if (a.det_scale)
+      hipLaunchKernelGGL((jk_full_kernel<LA, LB, LC, LD, true>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((jk_full_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+  }
 }
 
 template <int LA, int LB, int LC, int LD>
@@ -1080,30 +1150,55 @@ void launch_blocks_t(const BlockArgs& a, hipStream_t s) {
   hipLaunchKernelGGL((blocks_kernel<LA, LB, LC, LD>), dim3((unsigned)blocks), dim3(256), 0, s, a);
 }
 
+// Dynamic LDS above 64 KB (one-density launches of wide molecules) is
+// requested explicitly; a runtime that needs no request and rejects it leaves
+// the launch itself to report a size it cannot give.
+template <typename Kernel>
+void request_lds(Kernel kernel, size_t bytes) {
+  if (bytes <= 65536) return;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)bytes) != hipSuccess)
+    (void)hipGetLastError();
+}
+
 template <int LA, int LB, int LC, int LD, int CA, int CB, int CC, int CD>
 void launch_jk_bra_t(const BraArgs& a, size_t lds_bytes, hipStream_t s) {
   if (a.nchunks <= 0) return;
   const dim3 block(a.threads), grid((unsigned)a.nchunks);
+#define GSCF_BRA_LAUNCH(...)                                  \
+  do {                                                        \
+    request_lds(__VA_ARGS__, lds_bytes);                      \
+    hipLaunchKernelGGL((__VA_ARGS__), grid, block, lds_bytes, s, a); \
+  } while (0)
   // in-core integrals (FP64 blocks; segmented patterns only)
   if constexpr (CA * CB * CC * CD == 1) {
     if (a.mode == 1) {
-      hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false, 1>), grid, block, lds_bytes, s, a);
+      GSCF_BRA_LAUNCH(jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false, 1>);
       return;
     }
     if (a.mode == 2) {
-      hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false, 2>), grid, block, lds_bytes, s, a);
+      if (a.det_scale)
+        GSCF_BRA_LAUNCH(jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false, 2, true>);
+      else
+        GSCF_BRA_LAUNCH(jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false, 2>);
       return;
     }
   }
+  // deterministic digestion: FP64, Rys table from global memory
+  if (a.det_scale) {
+    GSCF_BRA_LAUNCH(jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false, 0, true>);
+    return;
+  }
   // variants: Rys table in LDS or global; FP64 only, or with the FP32 quartet path
   if (a.tab_lds && a.mixed)
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, true, true>), grid, block, lds_bytes, s, a);
+    GSCF_BRA_LAUNCH(jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, true, true>);
   else if (a.tab_lds)
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, true, false>), grid, block, lds_bytes, s, a);
+    GSCF_BRA_LAUNCH(jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, true, false>);
   else if (a.mixed)
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, true>), grid, block, lds_bytes, s, a);
+    GSCF_BRA_LAUNCH(jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, true>);
   else
-    hipLaunchKernelGGL((jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false>), grid, block, lds_bytes, s, a);
+    GSCF_BRA_LAUNCH(jk_bra_kernel<LA, LB, LC, LD, CA, CB, CC, CD, false, false>);
+#undef GSCF_BRA_LAUNCH
 }
 
 // Column-pattern dispatch of the bra-major kernel (patterns: columns of the

@@ -72,6 +72,9 @@ typedef struct {
   int64_t nn_total;     /* elements of one packed density: sum over molecules of n_m^2 */
   int64_t device_bytes; /* device memory held by the plan, its current workspace and
                            in-core integrals */
+  int64_t lds_max;      /* LDS per workgroup a one-density launch may use */
+  int32_t classes_global_atomic; /* quartet classes whose exchange rows exceed lds_max even for
+                                    one density (run by the global-atomic kernels) */
 } gscf_plan_info;
 
 /* Device-memory forecast before any GPU work (the reference's
@@ -102,6 +105,17 @@ typedef struct {
   int32_t partial_sums;        /* bit 0: a workgroup hands its exchange rows, bit 1 its ket-side
                                   J window, to a reduction pass as plain stores into per-build
                                   slabs instead of device-scope atomics (0..3; default 1) */
+  int32_t lds_limit;           /* bytes of LDS a workgroup of the bra-major J/K kernels may use:
+                                  0 (default) = the device's limit per workgroup (160 KB on
+                                  gfx950); lower values (4096..) force the per-density and
+                                  global-atomic routes that wide molecules take */
+  int32_t deterministic;       /* 1: J/K builds are reproducible bit for bit -- every digestion
+                                  add is a 64-bit integer add of the value times a per-molecule
+                                  power-of-two scale (from Cauchy-Schwarz bounds of the density
+                                  and the Schwarz table, so every result stays below 2^60), which
+                                  integer atomics sum independently of their order (FP64 only;
+                                  slower: no partial slabs, Rys table read from global memory);
+                                  0 (default): FP64 atomics, equal to rounding */
 } gscf_options;
 int gscf_options_set(const gscf_options* o);
 int gscf_options_get(gscf_options* o);

@@ -148,8 +148,10 @@ def test_invalid_inputs_rejected_before_launch():
         direct_jk(atm, bas, env, torch.zeros((12, 12), dtype=torch.float64, device="cuda"), cart=True)
 
 
-def test_repeat_is_bitwise_reproducible_in_j_structure(oracle_mod):
-    """Same plan, same density: results agree to atomic-reordering level."""
+def test_repeat_agrees_to_rounding(oracle_mod):
+    """Same plan, same density, default FP64 atomics: repeated builds agree to
+    the rounding of a reordered FP64 sum (the deterministic mode below makes
+    them bitwise equal)."""
     from gradscf_amd.jk import direct_jk
     _, _, atm, bas, env = systems.tables(systems.water_dimer(), "6-31g*", cart=True)
     n = oracle_mod.nao(bas, True)
@@ -464,3 +466,127 @@ def test_engine_options_do_not_change_results(gc, streams):
         at.set_options(**before)
     assert float((got - ref).abs().max()) < 1e-11
     assert float((got2 - got).abs().max()) < 1e-11
+
+
+@pytest.mark.parametrize("lds_limit,route", [(0, "all densities"), (8192, "per density"), (4096, "global atomics")])
+def test_lds_routes_match_oracle(oracle_mod, lds_limit, route):
+    """Water dimer cc-pVDZ (Cartesian, nonsymmetric densities): the class
+    launches an LDS limit forces -- every density in one launch, one launch
+    per density (the route of wide molecules), and the global-atomic kernels
+    for classes whose exchange rows do not fit even for one density -- give
+    the oracle's J/K, evaluated and in core, for J and K together and alone."""
+    from gradscf_amd import autotune as at
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "cc-pvdz", cart=True)
+    n = oracle_mod.nao(bas, True)
+    d = systems.random_density(n, nb=2, seed=71, symmetric=False)
+    ref = oracle_mod.direct_jk(atm, bas, env, d, cart=True, cutoff=1e-12, workers=4)
+    before = at.get_options()
+    try:
+        at.set_options(lds_limit=lds_limit)
+        plan = _plan(atm, bas, env, True, 1e-12)
+    finally:
+        at.set_options(**before)
+    assert (plan.info["classes_global_atomic"] > 0) == (route == "global atomics")
+    if lds_limit:
+        assert plan.info["lds_max"] == lds_limit
+    dd = _cuda(d)
+    for incore in (False, True):
+        if incore:
+            plan.incore(True)
+        out = plan.jk(dd).cpu().numpy()
+        assert np.max(np.abs(out - ref)) < 1e-10, (route, incore)
+        j = plan.jk(dd, with_k=False).cpu().numpy()
+        k = plan.jk(dd, with_j=False).cpu().numpy()
+        assert np.max(np.abs(j[0] - ref[0])) < 1e-10 and np.max(np.abs(k[0] - ref[1])) < 1e-10
+
+
+def test_nonsymmetric_density_on_the_bench_workload(oracle_mod):
+    """(H2O)16 cc-pVDZ spherical (400 Cartesian AOs), one nonsymmetric density:
+    the dd classes' exchange rows for D and D^T exceed 64 KB together and run
+    one density per launch; J/K against the reference libcint path."""
+    from gradscf_amd import molecule as M
+    _, _, atm, bas, env = systems.tables(M.water_cluster(16, seed=16, box=12.0), "cc-pvdz", cart=False)
+    plan = _plan(atm, bas, env, False, 1e-12)
+    assert plan.info["classes_global_atomic"] == 0
+    d = systems.random_density(plan.nao, nb=1, seed=72, symmetric=False) * 0.05
+    with oracle_mod.backend("libcint" if oracle_mod.have_ref() else "oracle"):
+        ref = oracle_mod.direct_jk(atm, bas, env, d, cart=False, cutoff=1e-12, workers=16)
+    out = plan.jk(_cuda(d)).cpu().numpy()
+    assert np.max(np.abs(out - ref)) < 1e-10
+
+
+def test_wide_molecule_one_density_slot_above_64kb(oracle_mod):
+    """(H2O)36 6-31G* (684 Cartesian AOs): one bra slot of the dd classes needs
+    more than 64 KB of LDS (12 exchange rows x 684 columns), so those classes
+    run one density per launch with the larger LDS the device allows instead
+    of the global-atomic kernels."""
+    from gradscf_amd import molecule as M
+    _, _, atm, bas, env = systems.tables(M.water_cluster(36, seed=36, box=16.0), "6-31g*", cart=True)
+    plan = _plan(atm, bas, env, True, 1e-12)
+    assert plan.info["nc_max"] == 684
+    assert plan.info["lds_max"] > 65536 and plan.info["classes_global_atomic"] == 0
+    c = np.random.default_rng(73).normal(size=(684, 180)) / np.sqrt(684)
+    d = (c @ c.T)[None]
+    with oracle_mod.backend("libcint" if oracle_mod.have_ref() else "oracle"):
+        ref = oracle_mod.direct_jk(atm, bas, env, d, cart=True, cutoff=1e-12, workers=16)
+    out = plan.jk(_cuda(d)).cpu().numpy()
+    assert np.max(np.abs(out - ref)) < 1e-10
+
+
+@pytest.mark.parametrize("cart,lds_limit", [(True, 0), (False, 0), (True, 8192), (True, 4096)])
+def test_deterministic_mode_is_bitwise_reproducible(oracle_mod, cart, lds_limit):
+    """Engine option deterministic=1 (integer accumulation of scaled values):
+    repeated builds, builds on one launch stream instead of four and in-core
+    builds of the same plan are bitwise equal, on every class route (all
+    densities in one launch, one per density, global atomics), and equal the
+    oracle to 1e-10 (the reference is reproducible at a fixed worker count,
+    tests/integrals/test_native_direct_parallel.py:63-70)."""
+    import torch
+    from gradscf_amd import autotune as at
+    _, _, atm, bas, env = systems.tables(systems.water_dimer(), "cc-pvdz", cart=cart)
+    n = oracle_mod.nao(bas, cart)
+    d = systems.random_density(n, nb=2, seed=802, symmetric=False)
+    ref = oracle_mod.direct_jk(atm, bas, env, d, cart=cart, cutoff=1e-12, workers=4)
+    before = at.get_options()
+    outs = []
+    try:
+        for streams in (4, 1):
+            at.set_options(deterministic=1, lds_limit=lds_limit, streams=streams)
+            plan = _plan(atm, bas, env, cart, 1e-12)
+            dd = _cuda(d)
+            outs += [plan.jk(dd) for _ in range(3)]
+            if streams == 4:
+                plan.incore(True)
+                outs += [plan.jk(dd) for _ in range(2)]
+                plan.incore(False)
+                with pytest.raises(ValueError, match="deterministic"):
+                    plan.jk(dd, mixed_threshold=1e-7)
+    finally:
+        at.set_options(**before)
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+    assert np.max(np.abs(outs[0].cpu().numpy() - ref)) < 1e-10
+
+
+def test_deterministic_mode_batched_plan():
+    """A batched plan (every molecule its own fixed-point scale): bitwise
+    repeatable, and equal to the FP64-atomic build to 1e-11."""
+    import torch
+    from gradscf_amd import autotune as at
+    from gradscf_amd import molecule as M
+    from gradscf_amd.jk import DirectJKPlan
+    specs = [M.water_monomer(), M.water_cluster(3, seed=1003, box=8.0), M.water_cluster(2, seed=1002, box=8.0)]
+    tabs = [systems.tables(sp, "6-31g*", cart=True)[2:] for sp in specs]
+    plan64 = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
+    dens = plan64.pack([torch.as_tensor(systems.random_density(int(n), nb=1, seed=70 + m), device="cuda")
+                        for m, n in enumerate(plan64.mol_n)])
+    ref = plan64.jk(dens)
+    before = at.get_options()
+    try:
+        at.set_options(deterministic=1)
+        plan = DirectJKPlan.batch(tabs, cart=True, cutoff=1e-12)
+    finally:
+        at.set_options(**before)
+    a, b = plan.jk(dens), plan.jk(dens)
+    assert torch.equal(a, b)
+    assert float((a - ref).abs().max()) < 1e-11

@@ -369,15 +369,17 @@ def test_engine_options_api_without_gpu():
         pytest.skip("engine not built")
     from gradscf_amd import autotune as at
     before = at.get_options()
-    assert before == {"general_contraction": 0, "streams": 4, "partial_sums": 1}
+    default = {"general_contraction": 0, "streams": 4, "partial_sums": 1, "lds_limit": 0, "deterministic": 0}
+    assert before == default
+    changed = {"general_contraction": 1, "streams": 2, "partial_sums": 1, "lds_limit": 8192, "deterministic": 1}
     try:
-        at.set_options(general_contraction=1, streams=2, partial_sums=1)
-        assert at.get_options() == {"general_contraction": 1, "streams": 2, "partial_sums": 1}
+        at.set_options(**changed)
+        assert at.get_options() == changed
         for bad in ({"streams": 0}, {"streams": 5}, {"general_contraction": 2}, {"partial_sums": 4},
-                    {"partial_sums": -1}):
+                    {"partial_sums": -1}, {"lds_limit": 100}, {"lds_limit": 200000}, {"deterministic": 2}):
             with pytest.raises(ValueError):
                 at.set_options(**bad)
-        assert at.get_options() == {"general_contraction": 1, "streams": 2, "partial_sums": 1}
+        assert at.get_options() == changed
         with pytest.raises(ValueError):
             at.set_options(threads=4)
     finally:
@@ -481,3 +483,129 @@ def test_out_buffer_and_packed_jk_rejections_without_gpu():
             packed_jk(torch.zeros(21, dtype=torch.float64), torch.zeros((3, 3), dtype=torch.float64))
     with pytest.raises(ValueError, match="s8"):
         packed_jk(torch.zeros((6, 6), dtype=torch.float64), torch.zeros((3, 3), dtype=torch.float64))
+
+
+class _FullTensorPlan:
+    """Stand-in for a device plan: J/K (and the coordinate VJP) from a full
+    ERI tensor on the CPU, so the autograd rules run without a GPU."""
+
+    def __init__(self, eri, dr_eri=None, atm=None, cutoff=0.0):
+        import torch
+        self.eri = torch.as_tensor(eri)
+        self.dr_eri = dr_eri
+        self.atm = atm
+        self.cutoff = cutoff
+
+    def jk(self, d, *, with_j=True, with_k=True):
+        import torch
+        out = []
+        if with_j:
+            out.append(torch.einsum("pqrs,...rs->...pq", self.eri, d))
+        if with_k:
+            out.append(torch.einsum("prqs,...rs->...pq", self.eri, d))
+        return torch.stack(out)
+
+    def jk_geometry_vjp(self, d, cj, ck):
+        import torch
+        # dr_eri[x] = d eri / d (coordinate x); rows = coordinates here
+        g = torch.zeros(self.dr_eri.shape[0], dtype=torch.float64)
+        for x in range(self.dr_eri.shape[0]):
+            e = torch.as_tensor(self.dr_eri[x])
+            if cj is not None:
+                g[x] += torch.sum(cj * torch.einsum("pqrs,rs->pq", e, d))
+            if ck is not None:
+                g[x] += torch.sum(ck * torch.einsum("prqs,rs->pq", e, d))
+        return g.reshape(-1, 3)
+
+
+def _random_eri(n, seed):
+    """A random tensor with the 8-fold permutation symmetry of real ERIs."""
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=(n, n, n, n))
+    a = a + a.transpose(1, 0, 2, 3)
+    a = a + a.transpose(0, 1, 3, 2)
+    return a + a.transpose(2, 3, 0, 1)
+
+
+def test_autodiff_density_rules_without_gpu():
+    """The torch.autograd rules of direct J/K (jk.py:284-307,385-391 of the
+    reference: JVP = the operator on the tangent, VJP = J on the J cotangent and
+    K on the K cotangent) against autograd through explicit einsums, for the
+    gradient and the Hessian-vector product, every operator selection."""
+    import torch
+    import torch.autograd.forward_ad as fwad
+    from gradscf_amd.autodiff import JKOperator, apply_jk
+    n = 5
+    eri = _random_eri(n, 3)
+    plan = _FullTensorPlan(eri)
+    rng = np.random.default_rng(4)
+    d0, t = (torch.as_tensor(x) for x in rng.normal(size=(2, 3, n, n)))
+    for wj, wk in ((True, True), (True, False), (False, True)):
+        op = JKOperator(plan, wj, wk)
+
+        def loss(out):
+            return 0.7 * torch.sum(out[0] ** 2) + 0.3 * torch.sum(out[-1] ** 3)
+
+        d = d0.clone().requires_grad_(True)
+        g = torch.autograd.grad(loss(apply_jk(op, d)), d, create_graph=True)[0]
+        h = torch.autograd.grad(g, d, grad_outputs=t)[0]
+        dr = d0.clone().requires_grad_(True)
+        gr = torch.autograd.grad(loss(plan.jk(dr, with_j=wj, with_k=wk)), dr, create_graph=True)[0]
+        hr = torch.autograd.grad(gr, dr, grad_outputs=t)[0]
+        assert torch.allclose(g, gr, rtol=1e-12, atol=1e-9)
+        assert torch.allclose(h, hr, rtol=1e-12, atol=1e-9)
+        with fwad.dual_level():
+            tan = fwad.unpack_dual(apply_jk(op, fwad.make_dual(d0, t))).tangent
+        assert torch.allclose(tan, plan.jk(t, with_j=wj, with_k=wk), rtol=1e-12, atol=1e-12)
+
+
+def test_autodiff_coordinate_rule_without_gpu():
+    """Coordinate cotangents come from the plan's geometry VJP summed over the
+    density batch; a screened operator refuses, and a second derivative
+    through a coordinate gradient raises instead of returning zeros."""
+    import torch
+    from gradscf_amd.autodiff import JKOperator, apply_jk
+    n = 4
+    eri = _random_eri(n, 5)
+    dr = np.stack([_random_eri(n, 10 + x) for x in range(6)])      # two rows of three coordinates
+    plan = _FullTensorPlan(eri, dr_eri=dr, atm=np.zeros((2, 6), dtype=np.int32))
+    rng = np.random.default_rng(6)
+    d = torch.as_tensor(rng.normal(size=(2, n, n)))
+    cj, ck = (torch.as_tensor(x) for x in rng.normal(size=(2, 2, n, n)))
+    coords = torch.zeros((2, 3), dtype=torch.float64, requires_grad=True)
+    out = apply_jk(JKOperator(plan, True, True), d, coords)
+    (torch.sum(cj * out[0]) + torch.sum(ck * out[1])).backward()
+    ref = sum(plan.jk_geometry_vjp(d[b], cj[b], ck[b]) for b in range(2))
+    assert torch.allclose(coords.grad, ref, rtol=1e-12, atol=1e-12)
+    coords = torch.zeros((2, 3), dtype=torch.float64, requires_grad=True)
+    out = apply_jk(JKOperator(plan, True, True), d, coords)
+    g = torch.autograd.grad(torch.sum(out[0] ** 2), coords, create_graph=True)[0]
+    with pytest.raises(NotImplementedError, match="Second derivatives"):
+        torch.autograd.grad(g.sum(), coords)
+    screened = _FullTensorPlan(eri, dr_eri=dr, atm=plan.atm, cutoff=1e-12)
+    coords = torch.zeros((2, 3), dtype=torch.float64, requires_grad=True)
+    out = apply_jk(JKOperator(screened, True, False), d, coords)
+    with pytest.raises(NotImplementedError, match="cutoff=0"):
+        out.sum().backward()
+
+
+def test_differentiable_packing_matches_pack():
+    """packing.pack_torch gives pack's environment, and its gradient chains a
+    coefficient-slot cotangent to the raw coefficients like grad.env_to_basis_gradient."""
+    import torch
+    from gradscf_amd.basis import BasisParameters
+    from gradscf_amd.grad import env_to_basis_gradient
+    from gradscf_amd.packing import pack_torch
+    topo, par = prepare_basis(M.water_monomer(), "cc-pvdz", cart=False)
+    atm, bas, env = pack(topo, par)
+    exps = tuple(torch.tensor(np.asarray(a, dtype=float), requires_grad=True) for a in par.exponents)
+    coefs = tuple(torch.tensor(np.asarray(c, dtype=float), requires_grad=True) for c in par.coefficients)
+    atm2, bas2, env_t = pack_torch(topo, BasisParameters(exps, coefs, par.centers, par.nuclear_coords))
+    assert np.array_equal(atm, atm2) and np.array_equal(bas, bas2)
+    assert np.allclose(env_t.detach().numpy(), env, rtol=1e-14, atol=0.0)
+    w = np.random.default_rng(7).normal(size=env.shape)
+    (torch.as_tensor(w) * env_t).sum().backward()
+    d_exp, d_coef = env_to_basis_gradient(w, bas, topo, par)
+    for s in range(len(exps)):
+        assert np.allclose(exps[s].grad.numpy(), d_exp[s], rtol=1e-12, atol=1e-12)
+        assert np.allclose(coefs[s].grad.numpy(), d_coef[s], rtol=1e-12, atol=1e-12)
