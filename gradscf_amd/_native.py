@@ -19,7 +19,8 @@ EXPORTS = ("gscf_last_error", "gscf_plan_create", "gscf_plan_create_batch", "gsc
            "gscf_workspace_bytes", "gscf_int1e_gradient", "gscf_tuning_set", "gscf_tuning_get",
            "gscf_tuning_reset", "gscf_jk_param_vjp", "gscf_int1e_param_vjp", "gscf_options_set",
            "gscf_options_get", "gscf_fma_probe", "gscf_plan_incore", "gscf_plan_incore_bytes",
-           "gscf_packed_eri", "gscf_packed_jk")
+           "gscf_packed_eri", "gscf_packed_jk", "gscf_density_fitting",
+           "gscf_compact")
 
 GSCF_OK, GSCF_INVALID_ARGUMENT, GSCF_INTERNAL = 0, 1, 2
 FLAG_SYMMETRIC_DENSITY = 1
@@ -88,6 +89,8 @@ def lib() -> ctypes.CDLL:
     handle.gscf_plan_incore_bytes.argtypes = [vp, vp]
     handle.gscf_packed_eri.argtypes = [vp, i32, vp, i32, vp, i32, i32, i32, vp, i64, vp]
     handle.gscf_packed_jk.argtypes = [vp, i64, i32, vp, i32, i32, vp, i32, i32, vp]
+    handle.gscf_density_fitting.argtypes = [vp, i32, vp, i32, vp, i32, i32, i32, i32, vp, i64, vp]
+    handle.gscf_compact.argtypes = [vp, i32, vp, i32, vp, i32, i32, i32, i32, vp, i64, vp]
     handle.gscf_jk_param_vjp.argtypes = [vp, vp, vp, vp, vp, vp]
     handle.gscf_int1e_param_vjp.argtypes = [vp, vp, vp, vp, vp]
     handle.gscf_workspace_bytes.argtypes = [vp, i32, vp, i32, vp, i32, i32, dbl, i32, vp]

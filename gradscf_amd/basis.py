@@ -10,8 +10,10 @@ Reference semantics followed:
     src/gradscf/integrals/basis/normalization.py:50-63
 
 The basis numbers (exponents and contraction coefficients) are the published
-STO-3G, 6-31G, 6-31G* and cc-pVDZ sets for H, He, C, N, O, stored as JSON
-under basis_data/ in the raw [l, [exponent, c_1, ..., c_nctr], ...] layout.
+STO-3G, 6-31G, 6-31G* and cc-pVDZ sets and the cc-pVDZ-JKFIT and
+def2-universal-JKFIT auxiliary sets for H, He, C, N, O, stored as JSON under
+basis_data/ in the raw [l, [exponent, c_1, ..., c_nctr], ...] layout
+(basis_data/extract.py regenerates them from the reference's snapshot).
 """
 from __future__ import annotations
 
@@ -33,6 +35,8 @@ _ALIASES = {
     "631g*": "6-31gs",
     "631gs": "6-31gs",
     "ccpvdz": "cc-pvdz",
+    "ccpvdzjkfit": "cc-pvdz-jkfit",
+    "def2universaljkfit": "def2-universal-jkfit",
 }
 
 
@@ -49,7 +53,8 @@ def load_basis(name: str, symbol: str) -> list:
     """Raw shells [[l, [exp, c...], ...], ...] of a named basis for one element."""
     key = _format_name(name)
     if key not in _ALIASES:
-        raise KeyError(f"Basis {name!r} is not available (have: sto-3g, 6-31g, 6-31g*, cc-pvdz).")
+        raise KeyError(f"Basis {name!r} is not available (have: sto-3g, 6-31g, 6-31g*, cc-pvdz, "
+                       "cc-pvdz-jkfit, def2-universal-jkfit).")
     data = _load_file(_ALIASES[key])
     sym = std_symbol(symbol)
     if sym not in data:

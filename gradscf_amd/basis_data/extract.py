@@ -36,6 +36,9 @@ SETS = {
     "6-31g": "pople-basis/6-31G.dat",
     "6-31gs": "pople-basis/6-31Gs.dat",
     "cc-pvdz": "cc-pvdz.dat",
+    # auxiliary (density-fitting) sets
+    "cc-pvdz-jkfit": "cc-pvdz-jkfit.dat",
+    "def2-universal-jkfit": "def2-universal-jkfit.dat",
 }
 
 _DELIM = re.compile(r"#\s*BASIS SET.*\n|END\n")

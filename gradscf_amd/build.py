@@ -1,7 +1,8 @@
 """Build the HIP engine in-tree: gradscf_amd/lib/libgradscf_jk.so (gfx950).
 
 Steps: generate the Rys table (csrc/gen_rys_tables.py, cached by source hash),
-compile one translation unit per angular-momentum class plus the host engine
+compile one translation unit per angular-momentum class (and per auxiliary
+angular momentum of the density-fitting kernels) plus the host engine
 with hipcc in parallel, and link with an export map that keeps only gscf_*.
 
 Usage: python -m gradscf_amd.build [--jobs N]
@@ -25,6 +26,9 @@ CLASSES = [(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 0),
            (2, 0, 0, 0), (2, 0, 1, 0), (2, 0, 1, 1), (2, 0, 2, 0), (2, 1, 0, 0), (2, 1, 1, 0),
            (2, 1, 1, 1), (2, 1, 2, 0), (2, 1, 2, 1), (2, 2, 0, 0), (2, 2, 1, 0), (2, 2, 1, 1),
            (2, 2, 2, 0), (2, 2, 2, 1), (2, 2, 2, 2)]
+
+# auxiliary shells of the density-fitting integrals: up to g
+DF_MAX_L = 4
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -55,6 +59,8 @@ def build(jobs: int | None = None) -> Path:
         tag = "".join(map(str, c))
         units.append((CSRC / "jk_class.hip", BUILD / f"jk_class_{tag}.o",
                       (f"GSCF_LA={c[0]}", f"GSCF_LB={c[1]}", f"GSCF_LC={c[2]}", f"GSCF_LD={c[3]}")))
+    for lc in range(DF_MAX_L + 1):
+        units.append((CSRC / "df_class.hip", BUILD / f"df_class_{lc}.o", (f"GSCF_DF_LC={lc}",)))
     jobs = jobs or min(16, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda u: _compile(*u), units))

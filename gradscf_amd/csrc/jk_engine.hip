@@ -75,8 +75,56 @@ static int ncart_h(int l) { return (l + 1) * (l + 2) / 2; }
 
 // Cartesian -> real spherical coefficients for l <= 2 (rows: spherical m in
 // PySCF order; columns: Cartesian components in libcint order).
+// Real solid harmonics r^l Y_lm of l >= 3 (unit-sphere normalisation,
+// m = -l..l) as polynomials in the Cartesian monomials, in libcint's
+// component order (descending x, then descending y): the closed form of
+// Helgaker, Jorgensen and Olsen, Molecular Electronic-Structure Theory,
+// eqs. (6.4.47)-(6.4.48), scaled from Racah's normalisation by
+// sqrt((2l+1)/(4 pi)).  (The d rows below are the same polynomials.)
+static void solid_harmonic_rows(int l, std::vector<std::vector<double>>& m) {
+  const double pi = 3.14159265358979323846;
+  auto fact = [](int k) {
+    double r = 1.0;
+    for (int i = 2; i <= k; ++i) r *= i;
+    return r;
+  };
+  auto choose = [&](int a, int b) { return (b < 0 || b > a) ? 0.0 : fact(a) / (fact(b) * fact(a - b)); };
+  auto comp = [&](int lx, int ly) {
+    int idx = 0;
+    for (int x = l; x >= 0; --x)
+      for (int y = l - x; y >= 0; --y) {
+        if (x == lx && y == ly) return idx;
+        ++idx;
+      }
+    return -1;
+  };
+  m.assign(2 * l + 1, std::vector<double>(ncart_h(l), 0.0));
+  const double unit = std::sqrt((2.0 * l + 1.0) / (4.0 * pi));
+  for (int mm = -l; mm <= l; ++mm) {
+    const int am = std::abs(mm);
+    const int half = mm < 0 ? 1 : 0;  // v runs over half-integers (2v odd) for m < 0
+    const double norm = std::sqrt(2.0 * fact(l + am) * fact(l - am) / (mm == 0 ? 2.0 : 1.0)) /
+                        (std::pow(2.0, am) * fact(l));
+    for (int t = 0; t <= (l - am) / 2; ++t)
+      for (int u = 0; u <= t; ++u)
+        for (int twov = half; twov <= (half ? 2 * ((am - 1) / 2) + 1 : 2 * (am / 2)); twov += 2) {
+          // (-1)^(t + v - v_m): v - v_m = (twov - half) / 2
+          const double sign = ((t + (twov - half) / 2) & 1) ? -1.0 : 1.0;
+          const double c = sign * std::pow(0.25, t) * choose(l, t) * choose(l - t, am + t) * choose(t, u) *
+                           choose(am, twov);
+          const int px = 2 * t + am - 2 * u - twov, py = 2 * u + twov;
+          if (px < 0 || c == 0.0) continue;
+          m[mm + l][comp(px, py)] += norm * c * unit;
+        }
+  }
+}
+
 static void c2s_rows(int l, std::vector<std::vector<double>>& m) {
   const double pi = 3.14159265358979323846;
+  if (l >= 3) {
+    solid_harmonic_rows(l, m);
+    return;
+  }
   if (l <= 1) {
     m.assign(ncart_h(l), std::vector<double>(ncart_h(l), 0.0));
     for (int i = 0; i < ncart_h(l); ++i) m[i][i] = 1.0;
@@ -330,6 +378,13 @@ struct PlanData {
   void launch_blocks_c##A##B##C##D(const BlockArgs& a, hipStream_t s);
 GSCF_CLASSES(X)
 #undef X
+
+// density-fitting block kernels, one translation unit per auxiliary l (df_class.hip)
+bool launch_df_blocks_c0(int la, int lb, const BlockArgs& a, hipStream_t s);
+bool launch_df_blocks_c1(int la, int lb, const BlockArgs& a, hipStream_t s);
+bool launch_df_blocks_c2(int la, int lb, const BlockArgs& a, hipStream_t s);
+bool launch_df_blocks_c3(int la, int lb, const BlockArgs& a, hipStream_t s);
+bool launch_df_blocks_c4(int la, int lb, const BlockArgs& a, hipStream_t s);
 
 void launch_int1e_class(int la, int lb, const Pair* pairs, int npairs, const PrimPair* prims,
                         const ShellXYZ* shells, const double* nuclei, int nnuc, const MolInfo* mols, double* s,
@@ -774,7 +829,7 @@ __global__ void param_to_env_kernel(int nprim, const int2* __restrict__ slots, c
 // own molecule.
 static void validate_tables(PlanData& P, const int32_t* atm, int natm, const int32_t* bas, int nbas,
                             const double* env, int nenv, int cart, int nmol = 1, const int32_t* mol_atm = nullptr,
-                            const int32_t* mol_bas = nullptr) {
+                            const int32_t* mol_bas = nullptr, int max_l = kMaxL) {
   if (!atm || !bas || !env || natm < 0 || nbas < 0 || nenv < 20 || (cart != 0 && cart != 1))
     throw InvalidArgument("Invalid libcint table dimensions or operator");
   ensure_tuning();
@@ -829,8 +884,9 @@ static void validate_tables(PlanData& P, const int32_t* atm, int natm, const int
         throw InvalidArgument("Invalid or unsupported libcint basis table");
       for (int p = 0; p < np; ++p)
         if (P.env[row[PTR_EXP] + p] <= 0.) throw InvalidArgument("Gaussian exponents must be positive");
-      if (l > kMaxL)
-        throw InvalidArgument("The MI355X J/K engine supports shells up to d (l <= 2)");
+      if (l > max_l)
+        throw InvalidArgument(max_l == kMaxL ? "The MI355X J/K engine supports shells up to d (l <= 2)"
+                                             : "Auxiliary shells are supported up to g (l <= 4)");
       P.ao[s + 1] = P.ao[s] + nc * (cart ? ncart_h(l) : 2 * l + 1);
       P.ao_cart[s + 1] = P.ao_cart[s] + nc * ncart_h(l);
     }
@@ -2743,6 +2799,309 @@ static void packed_eri(PlanData& P, int layout, double* out, int64_t size, hipSt
   HIP_OK(hipFree(stage));
 }
 
+// ----------------------------------------------------------------------------
+// Density-fitting integrals: the reference's compact layouts 2 (two-centre
+// metric (P|Q), m x m) and 3 (three-centre (ab|P), m x n(n+1)/2 over AO pairs
+// a >= b at pair_index(a, b)), csrc/density_fitting.cc:5-34.  The tables hold
+// orbital shells [0, split) and auxiliary shells [split, nbas).  A one-centre
+// factor P(r) is evaluated as the pair of P with a unit function (one
+// exponent-0 s primitive, coefficient 1, at P's centre), so the two- and
+// three-centre integrals run through the four-centre block kernels with the
+// libcint conventions of the two- and three-centre drivers.
+// ----------------------------------------------------------------------------
+constexpr int kMaxAuxL = 4;
+constexpr int kDfMaxTerms = 6;  // Cartesian terms of one real solid harmonic up to g
+
+struct DfPack {
+  int64_t off;   // first requested-basis element of the block
+  int64_t coff;  // first element of the staged Cartesian block
+  int ao[3];     // first AO (requested basis) and width of the three slots
+  int d[3];
+  int cao[3];    // first Cartesian AO and width
+  int cd[3];
+};
+
+static void launch_df_blocks(int la, int lb, int lc, const BlockArgs& a, hipStream_t s) {
+  bool ok = false;
+  switch (lc) {
+    case 0: ok = launch_df_blocks_c0(la, lb, a, s); break;
+    case 1: ok = launch_df_blocks_c1(la, lb, a, s); break;
+    case 2: ok = launch_df_blocks_c2(la, lb, a, s); break;
+    case 3: ok = launch_df_blocks_c3(la, lb, a, s); break;
+    case 4: ok = launch_df_blocks_c4(la, lb, a, s); break;
+    default: break;
+  }
+  if (!ok) throw Internal("no density-fitting kernel for this angular-momentum class");
+}
+
+// One thread per requested-basis element of a staged block: the solid-harmonic
+// (or identity) transform of every slot, then the scatter into the layout.
+// layout 3: out[(c - n) np + a (a + 1) / 2 + b] for a >= b (slots: a, b, c);
+// layout 2: out[(a - n) m + (c - n)] and its transpose for a >= c (slots a, -, c).
+__global__ void df_pack_kernel(const double* __restrict__ stage, const DfPack* __restrict__ tasks, int64_t ntask,
+                               int64_t total, int layout, int64_t n, int64_t m, int64_t np,
+                               const int* __restrict__ s2c_idx, const double* __restrict__ s2c_coef,
+                               double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  int64_t lo = 0, hi = ntask;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (tasks[mid].off <= e) lo = mid; else hi = mid;
+  }
+  const DfPack t = tasks[lo];
+  int64_t r = e - t.off;  // first slot fastest
+  const int p = (int)(r % t.d[0]);
+  r /= t.d[0];
+  const int q = (int)(r % t.d[1]);
+  const int u = (int)(r / t.d[1]);
+  const int64_t a = t.ao[0] + p, b = t.ao[1] + q, c = t.ao[2] + u;
+  if (layout == 3 && a < b) return;
+  if (layout == 2 && a < c) return;
+  const double* __restrict__ blk = stage + t.coff;
+  const int64_t g[3] = {a, layout == 3 ? b : -1, c};
+  double v = 0.0;
+  for (int k2 = 0; k2 < kDfMaxTerms; ++k2) {
+    const double c2 = s2c_coef[g[2] * kDfMaxTerms + k2];
+    if (c2 == 0.0) continue;
+    const int64_t i2 = s2c_idx[g[2] * kDfMaxTerms + k2] - t.cao[2];
+    for (int k1 = 0; k1 < (g[1] < 0 ? 1 : kDfMaxTerms); ++k1) {
+      double c1 = 1.0;
+      int64_t i1 = 0;
+      if (g[1] >= 0) {
+        c1 = s2c_coef[g[1] * kDfMaxTerms + k1];
+        if (c1 == 0.0) continue;
+        i1 = s2c_idx[g[1] * kDfMaxTerms + k1] - t.cao[1];
+      }
+      const double f = c1 * c2;
+      const int64_t base = (int64_t)t.cd[0] * (i1 + (int64_t)t.cd[1] * i2);
+      for (int k0 = 0; k0 < kDfMaxTerms; ++k0) {
+        const double c0 = s2c_coef[g[0] * kDfMaxTerms + k0];
+        if (c0 != 0.0) v += c0 * f * blk[base + s2c_idx[g[0] * kDfMaxTerms + k0] - t.cao[0]];
+      }
+    }
+  }
+  if (layout == 3) {
+    out[(c - n) * np + a * (a + 1) / 2 + b] = v;
+  } else {
+    out[(a - n) * m + (c - n)] = v;
+    out[(c - n) * m + (a - n)] = v;
+  }
+}
+
+static void density_fitting(const int32_t* atm, int natm, const int32_t* bas, int nbas, const double* env, int nenv,
+                            int cart, int layout, int split, double* out, int64_t size, hipStream_t s) {
+  if (layout != 2 && layout != 3) throw Internal("Unknown compact integral layout");
+  PlanData P;
+  validate_tables(P, atm, natm, bas, nbas, env, nenv, cart, 1, nullptr, nullptr, kMaxAuxL);
+  if (split < 1 || split >= nbas) throw Internal("Invalid orbital/auxiliary shell split");
+  for (int sh = 0; sh < split; ++sh)
+    if (P.bas[(size_t)sh * BAS_SLOTS + ANG_OF] > kMaxL)
+      throw Internal("Density-fitting orbital shells are supported up to d (l <= 2)");
+  const int64_t n = P.ao[split], m = P.n - n, np = n * (n + 1) / 2;
+  if (!out || size != (layout == 2 ? m * m : m * np)) throw Internal("Auxiliary integral shape mismatch");
+  build_segments(P);
+  // unit functions: one exponent-0 s segment at the centre of every auxiliary segment
+  const int nseg0 = (int)P.seg.size();
+  std::vector<int> unit_of(nseg0, -1);
+  for (int i = 0; i < nseg0; ++i) {
+    if (P.seg[i].parent < split) continue;
+    Seg g;
+    g.parent = -1;
+    g.ctr = 0;
+    g.l = 0;
+    g.np = 1;
+    for (int x = 0; x < 3; ++x) g.r[x] = P.seg[i].r[x];
+    g.ao_cart = 0;
+    g.mol = 0;
+    g.lseg = 0;
+    g.a = {0.0};
+    g.c = {1.0};
+    unit_of[i] = (int)P.seg.size();
+    P.seg.push_back(std::move(g));
+  }
+  std::vector<int> seg_first(nbas, 0);
+  for (int i = nseg0 - 1; i >= 0; --i) seg_first[P.seg[i].parent] = i;
+  // pairs: the orbital segment pairs and every auxiliary segment with its unit
+  // function; make_pair wants the higher angular momentum first
+  std::vector<Pair> pairs;
+  std::map<std::pair<int, int>, int> pair_of;
+  auto get_pair = [&](int a, int b) {
+    if (P.seg[b].l > P.seg[a].l) std::swap(a, b);
+    auto it = pair_of.find({a, b});
+    if (it != pair_of.end()) return it->second;
+    Pair pr;
+    make_pair(P, a, b, pr);
+    pairs.push_back(pr);
+    pair_of[{a, b}] = (int)pairs.size() - 1;
+    return (int)pairs.size() - 1;
+  };
+  struct Item {
+    std::array<int, 3> cls;
+    BlockTask t;
+    int blk;  // index of the requested block (DfPack)
+  };
+  std::vector<Item> items;
+  std::vector<DfPack> packs;
+  auto ncomp = [&](int sh) { return P.ao_cart[sh + 1] - P.ao_cart[sh]; };
+  auto add_block = [&](int i, int j, int k) {
+    // requested block (i, j | k) (layout 3) or (i | k) (layout 2, j < 0)
+    DfPack pk;
+    const int sh[3] = {i, j, k};
+    for (int x = 0; x < 3; ++x) {
+      if (sh[x] < 0) {
+        pk.ao[x] = -1; pk.d[x] = 1; pk.cao[x] = 0; pk.cd[x] = 1;
+        continue;
+      }
+      pk.ao[x] = P.ao[sh[x]];
+      pk.d[x] = P.ao[sh[x] + 1] - P.ao[sh[x]];
+      pk.cao[x] = P.ao_cart[sh[x]];
+      pk.cd[x] = ncomp(sh[x]);
+    }
+    packs.push_back(pk);
+    const int bi = (int)packs.size() - 1;
+    const int st[3] = {1, pk.cd[0], pk.cd[0] * pk.cd[1]};
+    const int nci = P.bas[(size_t)i * BAS_SLOTS + NCTR_OF], ncj = j < 0 ? 1 : P.bas[(size_t)j * BAS_SLOTS + NCTR_OF];
+    const int nck = P.bas[(size_t)k * BAS_SLOTS + NCTR_OF];
+    for (int ci = 0; ci < nci; ++ci)
+      for (int cj = 0; cj < ncj; ++cj)
+        for (int ck = 0; ck < nck; ++ck) {
+          const int si = seg_first[i] + ci, sk = seg_first[k] + ck;
+          const int sj = j < 0 ? unit_of[si] : seg_first[j] + cj;
+          Item it;
+          it.blk = bi;
+          const int bra = get_pair(si, sj);
+          const int ket = get_pair(sk, unit_of[sk]);
+          it.t.bra = bra;
+          it.t.ket = ket;
+          it.t.base = 0;  // set per staging pass
+          const bool swapped = pairs[bra].sa != si;
+          const int off_i = ci * ncart_h(P.seg[si].l), off_j = j < 0 ? 0 : cj * ncart_h(P.seg[sj].l);
+          it.t.stride[0] = swapped ? (j < 0 ? 0 : st[1]) : st[0];
+          it.t.coff[0] = swapped ? off_j : off_i;
+          it.t.stride[1] = swapped ? st[0] : (j < 0 ? 0 : st[1]);
+          it.t.coff[1] = swapped ? off_i : off_j;
+          it.t.stride[2] = st[2];
+          it.t.coff[2] = ck * ncart_h(P.seg[sk].l);
+          it.t.stride[3] = 0;
+          it.t.coff[3] = 0;
+          it.cls = {P.seg[pairs[bra].sa].l, P.seg[pairs[bra].sb].l, P.seg[sk].l};
+          items.push_back(it);
+        }
+  };
+  if (layout == 3) {
+    for (int i = 0; i < split; ++i)
+      for (int j = 0; j <= i; ++j)
+        for (int k = split; k < nbas; ++k) add_block(i, j, k);
+  } else {
+    for (int i = split; i < nbas; ++i)
+      for (int k = split; k <= i; ++k) add_block(i, -1, k);
+  }
+  // device tables
+  std::vector<ShellXYZ> sh(P.seg.size());
+  for (size_t i = 0; i < P.seg.size(); ++i) {
+    sh[i].x = P.seg[i].r[0];
+    sh[i].y = P.seg[i].r[1];
+    sh[i].z = P.seg[i].r[2];
+    sh[i].ao = P.seg[i].ao_cart;
+    sh[i].l = P.seg[i].l;
+    sh[i].mol = 0;
+    sh[i].lseg = 0;
+  }
+  // solid-harmonic (or identity) map of every requested AO: up to six Cartesian terms
+  std::vector<int> sidx((size_t)P.n * kDfMaxTerms, 0);
+  std::vector<double> scoef((size_t)P.n * kDfMaxTerms, 0.0);
+  for (int shl = 0; shl < nbas; ++shl) {
+    const int l = P.bas[(size_t)shl * BAS_SLOTS + ANG_OF], nc = P.bas[(size_t)shl * BAS_SLOTS + NCTR_OF];
+    std::vector<std::vector<double>> rows;
+    if (cart) {
+      rows.assign(ncart_h(l), std::vector<double>(ncart_h(l), 0.0));
+      for (int k = 0; k < ncart_h(l); ++k) rows[k][k] = 1.0;
+    } else {
+      c2s_rows(l, rows);
+    }
+    const int w = (int)rows.size();
+    for (int c = 0; c < nc; ++c)
+      for (int r = 0; r < w; ++r) {
+        const size_t ao = (size_t)P.ao[shl] + (size_t)c * w + r;
+        int t = 0;
+        for (int k = 0; k < ncart_h(l); ++k) {
+          if (rows[r][k] == 0.0) continue;
+          if (t == kDfMaxTerms) throw Internal("solid harmonic with more Cartesian terms than the map holds");
+          sidx[ao * kDfMaxTerms + t] = P.ao_cart[shl] + c * ncart_h(l) + k;
+          scoef[ao * kDfMaxTerms + t] = rows[r][k];
+          ++t;
+        }
+      }
+  }
+  ShellXYZ* d_shells = dev_upload(sh, s);
+  PrimPair* d_prims = dev_upload(P.prims, s);
+  Pair* d_pairs = dev_upload(pairs, s);
+  int* d_sidx = dev_upload(sidx, s);
+  double* d_scoef = dev_upload(scoef, s);
+  hipLaunchKernelGGL(fill_kernel, dim3(nblocks((size_t)size)), dim3(256), 0, s, out, (size_t)size, 0.0);
+  // staging passes of at most kStage doubles, blocks in class order inside a pass
+  constexpr int64_t kStage = int64_t(1) << 24;
+  double* stage = nullptr;
+  HIP_OK(hipMalloc(&stage, sizeof(double) * kStage));
+  std::vector<int64_t> blk_coff(packs.size(), -1);
+  size_t ib = 0;  // items are grouped by block, in block order
+  while (ib < items.size()) {
+    // blocks [b0, b1) of this pass
+    const int b0 = items[ib].blk;
+    int64_t ctot = 0;
+    size_t ie = ib;
+    while (ie < items.size()) {
+      const DfPack& pk = packs[items[ie].blk];
+      const int64_t csz = (int64_t)pk.cd[0] * pk.cd[1] * pk.cd[2];
+      if (ctot > 0 && ctot + csz > kStage) break;
+      if (csz > kStage) throw Internal("auxiliary integral block larger than the staging buffer");
+      const int bk = items[ie].blk;
+      blk_coff[bk] = ctot;
+      ctot += csz;
+      while (ie < items.size() && items[ie].blk == bk) ++ie;
+    }
+    const int b1 = items[ie - 1].blk + 1;
+    std::vector<Item> pass(items.begin() + ib, items.begin() + ie);
+    for (Item& it : pass) it.t.base = blk_coff[it.blk];
+    std::stable_sort(pass.begin(), pass.end(), [](const Item& x, const Item& y) { return x.cls < y.cls; });
+    std::vector<BlockTask> tasks;
+    for (const Item& it : pass) tasks.push_back(it.t);
+    BlockTask* d_tasks = dev_upload(tasks, s);
+    size_t i = 0;
+    while (i < pass.size()) {
+      size_t j = i;
+      while (j < pass.size() && pass[j].cls == pass[i].cls) ++j;
+      BlockArgs a;
+      a.pairs = d_pairs;
+      a.prims = d_prims;
+      a.shells = d_shells;
+      a.tasks = d_tasks + i;
+      a.ntasks = (int64_t)(j - i);
+      a.out = stage;
+      launch_df_blocks(pass[i].cls[0], pass[i].cls[1], pass[i].cls[2], a, s);
+      i = j;
+    }
+    std::vector<DfPack> pp(packs.begin() + b0, packs.begin() + b1);
+    int64_t tot = 0;
+    for (int b = b0; b < b1; ++b) {
+      pp[b - b0].coff = blk_coff[b];
+      pp[b - b0].off = tot;
+      tot += (int64_t)pp[b - b0].d[0] * pp[b - b0].d[1] * pp[b - b0].d[2];
+    }
+    DfPack* d_pp = dev_upload(pp, s);
+    hipLaunchKernelGGL(df_pack_kernel, dim3(nblocks((size_t)tot)), dim3(256), 0, s, stage, d_pp, (int64_t)pp.size(),
+                       tot, layout, n, m, np, d_sidx, d_scoef, out);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipFree(d_tasks));
+    HIP_OK(hipFree(d_pp));
+    ib = ie;
+  }
+  HIP_OK(hipFree(stage));
+  for (void* p : {(void*)d_shells, (void*)d_prims, (void*)d_pairs, (void*)d_sidx, (void*)d_scoef}) (void)hipFree(p);
+}
+
 static void packed_jk(const double* eri, int64_t eri_size, int ndim, const double* density, int nb, int n,
                       double* out, int with_j, int with_k, hipStream_t s) {
   if ((with_j != 0 && with_j != 1) || (with_k != 0 && with_k != 1) || !(with_j || with_k))
@@ -2981,6 +3340,27 @@ int gscf_packed_eri(const int32_t* atm, int32_t natm, const int32_t* bas, int32_
     return GSCF_INTERNAL;
   }
   return gscf_plan_destroy(plan);
+}
+
+int gscf_density_fitting(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                         int32_t nenv, int32_t cart, int32_t layout, int32_t split, double* out, int64_t size,
+                         void* stream) {
+  // the compact target reports every failure as Internal (compact.cc:16)
+  const int rc = guarded([&] {
+    density_fitting(atm, natm, bas, nbas, env, nenv, cart, layout, split, out, size, (hipStream_t)stream);
+  });
+  return rc == GSCF_OK ? rc : GSCF_INTERNAL;
+}
+
+int gscf_compact(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                 int32_t nenv, int32_t layout, int32_t cart, int32_t split, double* out, int64_t size, void* stream) {
+  if (layout < 0 || layout > 3) {
+    g_error = "Unknown compact integral layout";
+    return GSCF_INTERNAL;
+  }
+  if (layout == 2 || layout == 3)
+    return gscf_density_fitting(atm, natm, bas, nbas, env, nenv, cart, layout, split, out, size, stream);
+  return gscf_packed_eri(atm, natm, bas, nbas, env, nenv, cart, layout, out, size, stream);
 }
 
 int gscf_packed_jk(const double* eri, int64_t eri_size, int32_t ndim, const double* density, int32_t nb, int32_t n,

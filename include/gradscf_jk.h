@@ -223,6 +223,28 @@ int gscf_jk_build(gscf_plan* plan, const double* density, int32_t nb, double* ou
                   int32_t with_j, int32_t with_k, uint32_t flags, double mixed_threshold,
                   void* stream);
 
+/* Density-fitting integrals: the reference's compact target layouts 2 and 3
+ * (GradSCFCompact -> density_fitting, csrc/compact.cc:5-21,
+ * csrc/density_fitting.cc:5-34).  The tables hold the orbital shells
+ * [0, split) (up to d) followed by the auxiliary shells [split, nbas) (up to
+ * g); n / m are their AO counts (Cartesian or spherical as cart says).
+ *   layout 2: out[m][m] (device), the two-centre metric (P|Q);
+ *   layout 3: out[m][n(n+1)/2] (device), the three-centre (ab|P) at
+ *             pair index a(a+1)/2 + b for a >= b.
+ * size must equal the element count.  Every failure is GSCF_INTERNAL, as in
+ * the reference target.  Synchronises `stream`. */
+int gscf_density_fitting(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas,
+                         const double* env, int32_t nenv, int32_t cart, int32_t layout, int32_t split,
+                         double* out, int64_t size, void* stream);
+
+/* The whole compact target with its argument list (csrc/compact.cc:5-21):
+ * layouts 0 and 1 go to gscf_packed_eri (split ignored), 2 and 3 to
+ * gscf_density_fitting; any other layout is GSCF_INTERNAL ("Unknown compact
+ * integral layout"). */
+int gscf_compact(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
+                 int32_t nenv, int32_t layout, int32_t cart, int32_t split, double* out, int64_t size,
+                 void* stream);
+
 /* One-shot form with the argument list of GradSCFDirectJK (direct_jk.cc:97-102):
  * builds a plan, runs gscf_jk_build (nonfinite densities rejected, as
  * direct_jk.cc:24), synchronises `stream`, frees the plan.  `workers` is

@@ -252,3 +252,57 @@ def libcint_int1e(atm, bas, env, cart=True):
                 m[ao[i]:ao[i + 1], ao[j]:ao[j + 1]] = out.reshape(dj, di).T
         mats.append(m)
     return tuple(mats)
+
+
+def libcint_aux_block(shls, atm, bas, env, cart=True):
+    """The reference libcint's int2c2e (two shells) or int3c2e (three shells)
+    block as an array [d0, d1(, d2)] (libcint's first-index-fastest layout
+    transposed to C order); needs _ref."""
+    h = ctypes.CDLL(str(REF_LIB))
+    atm, bas, env = _i32(atm), _i32(bas), _f64(env)
+    ao = ao_offsets(bas, cart)
+    d = [int(ao[s + 1] - ao[s]) for s in shls]
+    out = np.zeros(int(np.prod(d)))
+    f = getattr(h, f"{'int3c2e' if len(shls) == 3 else 'int2c2e'}_{'cart' if cart else 'sph'}")
+    sh = _i32(shls)
+    f(_p(out), None, _p(sh, _ip), _p(atm, _ip), len(atm), _p(bas, _ip), len(bas), _p(env), None, None)
+    return out.reshape(d[::-1]).transpose(tuple(range(len(d)))[::-1])
+
+
+def density_fitting(atm, bas, env, *, layout, cart, split):
+    """Auxiliary integrals in the reference's compact layouts, restating the
+    loops of GradSCF's density_fitting (csrc/density_fitting.cc:5-34) over the
+    reference libcint's int2c2e / int3c2e blocks (needs _ref):
+      layout 2: (P|Q), m x m, every shell pair i >= j written to both halves;
+      layout 3: (ab|P), m x n(n+1)/2 at pair_index(a, b) = a(a+1)/2 + b, a >= b,
+    with orbital shells [0, split) and auxiliary shells [split, nbas)."""
+    bas = _i32(bas)
+    nbas = len(bas)
+    if split < 1 or split >= nbas:
+        raise ValueError("Invalid orbital/auxiliary shell split")
+    ao = ao_offsets(bas, cart)
+    n = int(ao[split])
+    m = int(ao[-1]) - n
+    npair = n * (n + 1) // 2
+    if layout == 2:
+        out = np.zeros((m, m))
+        for i in range(split, nbas):
+            for j in range(split, i + 1):
+                blk = libcint_aux_block((i, j), atm, bas, env, cart)
+                for p in range(blk.shape[0]):
+                    for q in range(blk.shape[1]):
+                        a, b = ao[i] + p - n, ao[j] + q - n
+                        out[a, b] = out[b, a] = blk[p, q]
+        return out
+    out = np.zeros((m, npair))
+    for i in range(split):
+        for j in range(i + 1):
+            for k in range(split, nbas):
+                blk = libcint_aux_block((i, j, k), atm, bas, env, cart)
+                for p in range(blk.shape[0]):
+                    for q in range(blk.shape[1]):
+                        a, b = ao[i] + p, ao[j] + q
+                        if a < b:
+                            continue
+                        out[ao[k] - n:ao[k + 1] - n, a * (a + 1) // 2 + b] = blk[p, q, :]
+    return out

@@ -326,3 +326,78 @@ def test_screened_jk_equals_masked_full_tensor(oracle_mod, cart, cutoff):
             kf = np.einsum("prqs,brs->bpq", eri, d)
             assert np.max(np.abs(out[0] - jf)) < 1e-10
             assert np.max(np.abs(out[1] - kf)) < 1e-10
+
+
+def _df_tables(aux="cc-pvdz-jkfit", cart=True, spec=None):
+    """Orbital (6-31G*) and auxiliary shells of water in one table set, like the
+    reference's make_auxiliary_plan (orbital shells first)."""
+    from gradscf_amd import molecule as M
+    from gradscf_amd.basis import BasisParameters, BasisTopology, prepare_basis
+    from gradscf_amd.packing import pack
+    spec = spec or M.water_monomer()
+    top, p = prepare_basis(spec, "6-31g*", cart=cart)
+    at, a = prepare_basis(spec, aux, cart=cart)
+    comb = BasisTopology(top.angular_momenta + at.angular_momenta, top.primitive_counts + at.primitive_counts,
+                         top.contraction_counts + at.contraction_counts, top.nuclear_charges, cart)
+    par = BasisParameters(p.exponents + a.exponents, p.coefficients + a.coefficients,
+                          np.concatenate([p.centers, a.centers]), p.nuclear_coords)
+    atm, bas, env = pack(comb, par)
+    return atm, bas, env, top.nshell
+
+
+def _with_unit(atm, bas, env, atom_row):
+    """Append the unit function: one s primitive of exponent 0 with coefficient
+    1/fac_sp(0) = sqrt(4 pi) (cancelling the s factor the four-centre driver
+    applies), on the atm row of the shell it is paired with."""
+    env = np.concatenate([env, [0.0, np.sqrt(4.0 * np.pi)]])
+    row = np.array([[atom_row, 0, 1, 1, 0, len(env) - 2, len(env) - 1, 0]], dtype=np.int32)
+    return atm, np.concatenate([bas, row]), env
+
+
+@ref_only
+def test_three_and_two_centre_integrals_pin_the_unit_function(oracle_mod):
+    """libcint's int3c2e (ab|P) equals the oracle's own four-centre Rys ERI
+    (ab|P u) and int2c2e (P|Q) equals (P u|Q u), u the unit function: the
+    three- and two-centre integrals the GPU evaluates through its four-centre
+    kernels, pinned on s..f auxiliary shells (Cartesian)."""
+    atm, bas, env, split = _df_tables(cart=True)
+    rng = np.random.default_rng(41)
+    ls = bas[:, 1]
+    aux = np.arange(split, len(bas))
+    for k in [int(aux[np.nonzero(ls[aux] == l)[0][0]]) for l in range(4)]:
+        a2, b2, e2 = _with_unit(atm, bas, env, int(bas[k, 0]))
+        u = len(b2) - 1
+        for _ in range(6):
+            i, j = (int(x) for x in rng.integers(0, split, 2))
+            ref = oracle_mod.libcint_aux_block((i, j, k), atm, bas, env, True)
+            own = oracle_mod.eri_block((i, j, k, u), a2, b2, e2, True)[..., 0]
+            assert np.max(np.abs(own - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+        for kk in [int(aux[np.nonzero(ls[aux] == l)[0][-1]]) for l in range(4)]:
+            a3, b3, e3 = _with_unit(a2, b2, e2, int(bas[kk, 0]))
+            ref = oracle_mod.libcint_aux_block((k, kk), atm, bas, env, True)
+            own = oracle_mod.eri_block((k, u, kk, len(b3) - 1), a3, b3, e3, True)[:, 0, :, 0]
+            assert np.max(np.abs(own - ref)) < 1e-12 * max(1.0, np.max(np.abs(ref)))
+
+
+@ref_only
+def test_density_fitting_oracle_layouts(oracle_mod):
+    """The restated compact layouts 2 and 3: the metric is symmetric positive
+    definite, and (ab|P) contracted with the inverse metric reproduces the
+    fitted Coulomb energy below the exact one (the Coulomb fitting error
+    (rho - rho~|rho - rho~) is non-negative) and within 1e-3 of it."""
+    atm, bas, env, split = _df_tables(cart=False)
+    metric = oracle_mod.density_fitting(atm, bas, env, layout=2, cart=False, split=split)
+    three = oracle_mod.density_fitting(atm, bas, env, layout=3, cart=False, split=split)
+    assert np.array_equal(metric, metric.T)
+    assert np.linalg.eigvalsh(metric).min() > 0
+    n = int(round((np.sqrt(8 * three.shape[1] + 1) - 1) / 2))
+    d = systems.random_density(n, seed=42)[0]
+    d = d @ d.T / n
+    i, j = np.tril_indices(n)
+    w = np.where(i == j, 1.0, 2.0) * d[i, j]
+    rho = three @ w                              # (P|rho)
+    e_fit = rho @ np.linalg.solve(metric, rho)   # (rho~|rho~) = (rho|P) V^-1 (P|rho)
+    jk = oracle_mod.direct_jk(atm, bas[:split], env, d, cart=False)
+    e_exact = float(np.sum(d * jk[0, 0]))        # (rho|rho)
+    assert e_fit <= e_exact + 1e-12
+    assert e_exact - e_fit < 1e-3 * e_exact
