@@ -37,14 +37,19 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(atm, bas, env, dens, cutoff, builds, j_gpu, k_gpu):
-    """Reference libcint J/K (restated DirectJK driver, row-ownership workers) on the host cores."""
+def cpu_baseline(atm, bas, env, dens, cutoff, builds, j_gpu, k_gpu, scf_job=None):
+    """Reference libcint J/K (restated DirectJK driver, row-ownership workers) on
+    the host cores: `builds` builds with every core after a warm-up, one build
+    with one worker, and (scf_job = (spec, gpu_energy)) the full RHF of the
+    same workload with every core, from integrals ready to convergence."""
     import numpy as np
     import oracle
+    from oracle import scf as oscf
     cores = len(os.sched_getaffinity(0))
     kind = "reference" if oracle.have_ref() else "port"
     backend = "libcint" if kind == "reference" else "oracle"
     times = []
+    scf_rec = None
     with oracle.backend(backend):
         for i in range(builds + 1):
             t0 = time.perf_counter()
@@ -53,6 +58,26 @@ def cpu_baseline(atm, bas, env, dens, cutoff, builds, j_gpu, k_gpu):
             log(f"cpu baseline build {i}: {dt:.3f} s ({backend}, {cores} workers)")
             if i:
                 times.append(dt)
+        t0 = time.perf_counter()
+        oracle.direct_jk(atm, bas, env, dens, cart=False, cutoff=cutoff, workers=1)
+        one = time.perf_counter() - t0
+        log(f"cpu baseline build, 1 worker: {one:.3f} s")
+        if scf_job is not None:
+            spec, e_gpu = scf_job
+            s, t, v = oracle.libcint_int1e(atm, bas, env, False) if kind == "reference" else \
+                oracle.int1e(atm, bas, env, False)
+
+            def jk(dm):
+                o = oracle.direct_jk(atm, bas, env, dm, cart=False, cutoff=cutoff, workers=cores)
+                return o[0, 0], o[1, 0]
+
+            t0 = time.perf_counter()
+            r = oscf.rhf(t + v, s, spec.nelectron, spec.nuclear_repulsion(), jk)
+            scf_s = time.perf_counter() - t0
+            log(f"cpu baseline SCF: {scf_s:.1f} s, {r['cycles']} cycles, E {r['energy']:.10f} "
+                f"(GPU - CPU {e_gpu - r['energy']:.2e})")
+            scf_rec = {"wall_s": scf_s, "cycles": r["cycles"], "converged": bool(r["converged"]),
+                       "energy": r["energy"], "energy_diff_gpu_minus_cpu": e_gpu - r["energy"], "workers": cores}
     med = statistics.median(times)
     err_j = float(np.max(np.abs(out[0, 0] - j_gpu)))
     err_k = float(np.max(np.abs(out[1, 0] - k_gpu)))
@@ -68,7 +93,8 @@ def cpu_baseline(atm, bas, env, dens, cutoff, builds, j_gpu, k_gpu):
     return {"value": 1.0 / med, "unit": "J/K builds/s", "cores": cores, "kind": kind,
             "sample": f"{builds} full J/K builds of the same workload (median, after 1 warm-up), "
                       f"Schwarz bounds + pair data rebuilt per call as the reference does; {cpu_model}",
-            "ms_per_build": med * 1e3, "max_abs_diff_vs_gpu": {"J": err_j, "K": err_k}}
+            "ms_per_build": med * 1e3, "ms_per_build_1_worker": one * 1e3,
+            "max_abs_diff_vs_gpu": {"J": err_j, "K": err_k}, "scf": scf_rec}
 
 
 CLASS_LETTERS = "spdf"
@@ -597,6 +623,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-builds", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-scf", action="store_true", help="skip the CPU reference RHF of the workload")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision builds and SCF")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-molecule batched workload")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-class schedules")
@@ -706,6 +733,41 @@ def main() -> int:
     value = args.steps / elapsed
     log(f"J/K: {ms_per_step:.3f} ms/build, {value:.2f} builds/s")
 
+    # a nonsymmetric density (K needs D and D^T): the dd classes' exchange rows
+    # of both exceed 64 KB together and run one density per launch
+    dns = (dens + 1e-3 * torch.triu(dens, 1)).contiguous()
+    out_ns = torch.empty_like(out)
+    for _ in range(args.warmup):
+        plan.jk(dns[None], symmetric=False, validate=False, out=out_ns)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.jk(dns[None], symmetric=False, validate=False, out=out_ns)
+    torch.cuda.synchronize()
+    nonsym_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    log(f"nonsymmetric density: {nonsym_ms:.3f} ms/build")
+
+    # deterministic mode (integer accumulation, bitwise repeatable) on the same workload
+    before_opts = at.get_options()
+    at.set_options(deterministic=1)
+    plan_det = DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff)
+    at.set_options(**before_opts)
+    out_det = torch.empty_like(out)
+    for _ in range(args.warmup):
+        plan_det.jk(dens[None], symmetric=True, validate=False, out=out_det)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan_det.jk(dens[None], symmetric=True, validate=False, out=out_det)
+    torch.cuda.synchronize()
+    det_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    det_again = plan_det.jk(dens[None], symmetric=True, validate=False)
+    det_rec = {"ms_per_step": det_ms, "bitwise_repeatable": bool(torch.equal(det_again, out_det)),
+               "max_abs_jk_diff_vs_fp64_atomics": float((out_det - out).abs().max())}
+    del plan_det
+    log(f"deterministic mode: {det_ms:.3f} ms/build, repeatable {det_rec['bitwise_repeatable']}, "
+        f"max|dJK| vs atomics {det_rec['max_abs_jk_diff_vs_fp64_atomics']:.1e}")
+
     # mixed precision: same workload, FP32 for quartets with Q_ij Q_kl max|D| < threshold
     thr = args.mixed_threshold
     mixed = None
@@ -728,7 +790,11 @@ def main() -> int:
         mixed = {"threshold": thr, "ms_per_step": mixed_ms, "builds_per_s": 1e3 / mixed_ms,
                  "speedup_vs_fp64": ms_per_step / mixed_ms, "max_abs_jk_diff_vs_fp64": mixed_err,
                  "scf_energy": scf_mixed.energy, "scf_energy_err_vs_fp64": scf_mixed.energy - scf.energy,
-                 "scf_wall_s": scf_mixed.timings["scf_s"], "scf_converged": bool(scf_mixed.converged)}
+                 "scf_wall_s": scf_mixed.timings["scf_s"], "scf_converged": bool(scf_mixed.converged),
+                 "pays": ms_per_step / mixed_ms > 1.0,
+                 "note": "non-packed FP32 VALU issues at the FP64 rate on CDNA4 (packed FP32 is excluded "
+                         "from this build), so FP32 quartets do not shorten a build; the batched "
+                         "speed-up over per-molecule FP64 calls comes from batching"}
 
     # per-class kernel durations (HIP events on the launch stream) over a timed pass
     plan.profile(True)
@@ -812,10 +878,12 @@ def main() -> int:
                 "wall_s": scf.timings["scf_s"], "wall_s_first": scf_first.timings["scf_s"],
                 "integrals_s": scf.timings["integrals_s"], "jk_builds": scf.timings["jk_builds"]},
         "mixed": mixed,
+        "nonsymmetric_ms_per_step": nonsym_ms,
+        "deterministic": det_rec,
         "incore": incore,
         "plan_ms": plan_s * 1e3,
         "ms_per_step_including_plan": ms_per_step + plan_s * 1e3,
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": profiled_traffic(dom["class"]),
                      "kernel": f"jk class ({dom['class']}) quartet kernel",
                      "note": "FP64 vector-ALU Rys quartet evaluation (no MFMA shape); algorithmic FLOP_eval "
@@ -843,7 +911,7 @@ def main() -> int:
                                                                incore=not args.no_incore)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(atm, bas, env, dens.cpu().numpy(), cutoff, args.cpu_builds,
-                                              j_gpu, k_gpu)
+                                              j_gpu, k_gpu, None if args.no_cpu_scf else (spec, scf.energy))
         if water is not None:
             result["cpu_baseline"]["water_631g"] = cpu_baseline_water(water)
     else:
