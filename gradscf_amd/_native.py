@@ -19,7 +19,7 @@ EXPORTS = ("gscf_last_error", "gscf_plan_create", "gscf_plan_create_batch", "gsc
            "gscf_workspace_bytes", "gscf_int1e_gradient", "gscf_tuning_set", "gscf_tuning_get",
            "gscf_tuning_reset", "gscf_jk_param_vjp", "gscf_int1e_param_vjp", "gscf_options_set",
            "gscf_options_get", "gscf_fma_probe", "gscf_plan_incore", "gscf_plan_incore_bytes",
-           "gscf_packed_eri", "gscf_packed_jk", "gscf_density_fitting",
+           "gscf_packed_eri", "gscf_packed_jk", "gscf_jk_build_noalloc", "gscf_density_fitting",
            "gscf_compact")
 
 GSCF_OK, GSCF_INVALID_ARGUMENT, GSCF_INTERNAL = 0, 1, 2
@@ -74,6 +74,7 @@ def lib() -> ctypes.CDLL:
     handle.gscf_plan_profile.argtypes = [vp, i32]
     handle.gscf_plan_class_times.argtypes = [vp, i32, vp, vp, vp, vp]
     handle.gscf_jk_build.argtypes = [vp, vp, i32, vp, i32, i32, u32, dbl, vp]
+    handle.gscf_jk_build_noalloc.argtypes = [vp, vp, i32, vp, i32, i32, u32, dbl, vp]
     handle.gscf_direct_jk.argtypes = [vp, i32, vp, i32, vp, i32, vp, i32, i32, vp, i32, dbl, i32, i32, i32, vp]
     handle.gscf_eri_blocks.argtypes = [vp, vp, i64, vp, vp, vp]
     handle.gscf_int1e.argtypes = [vp, vp, vp, vp, vp]

@@ -3292,6 +3292,53 @@ int gscf_jk_build(gscf_plan* plan, const double* density, int32_t nb, double* ou
   });
 }
 
+// ----------------------------------------------------------------------------
+// Allocation-free builds: the workspace lives in a file-scope device array
+// (64 MiB, held by every GPU the library runs on until the process ends).
+// Host code addresses it only through the per-device pointer that
+// hipGetSymbolAddress returns (the array's name is a host-side stand-in).
+// ----------------------------------------------------------------------------
+constexpr size_t kScratchDoubles = size_t(8) << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex g_scratch_mu;
+static std::vector<double*> g_scratch_ptr;  // per device
+
+static double* scratch_ptr() {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Internal("no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+int gscf_jk_build_noalloc(gscf_plan* plan, const double* density, int32_t nb, double* out, int32_t with_j,
+                          int32_t with_k, uint32_t flags, double mixed_threshold, void* stream) {
+  return guarded([&] {
+    if (!plan) throw InvalidArgument("NULL plan");
+    PlanData& P = plan->d;
+    if (nb < 1) throw InvalidArgument("Direct J/K shape mismatch");
+    if (P.prof) throw InvalidArgument("the allocation-free build does not run while the plan is profiling");
+    const bool sym = (flags & GSCF_FLAG_SYMMETRIC_DENSITY) != 0;
+    const int nbd = (with_k && !sym) ? 2 * nb : nb;
+    const size_t need = workspace_doubles(P, nb, nbd);
+    if (need > kScratchDoubles)
+      throw InvalidArgument("the allocation-free J/K build needs " + std::to_string(sizeof(double) * need) +
+                            " bytes of scratch for " + std::to_string(nb) +
+                            " densities; its scratch holds " + std::to_string(sizeof(double) * kScratchDoubles));
+    double* base = scratch_ptr();
+    const size_t c = (size_t)P.ncc_tot * nbd;
+    // [dc | jp | kp] over nbd densities, then the padded ket-side J over nb
+    const Workspace W = {base, base + c, base + 3 * c, base + 2 * c, nullptr};
+    jk_build(P, density, nb, out, with_j, with_k, flags, mixed_threshold, (hipStream_t)stream, &W);
+  });
+}
+
 int gscf_direct_jk(const int32_t* atm, int32_t natm, const int32_t* bas, int32_t nbas, const double* env,
                    int32_t nenv, const double* density, int32_t nb, int32_t n, double* out, int32_t cart,
                    double cutoff, int32_t workers, int32_t with_j, int32_t with_k, void* stream) {

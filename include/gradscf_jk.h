@@ -223,6 +223,24 @@ int gscf_jk_build(gscf_plan* plan, const double* density, int32_t nb, double* ou
                   int32_t with_j, int32_t with_k, uint32_t flags, double mixed_threshold,
                   void* stream);
 
+/* Allocation-free variant of gscf_jk_build.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results; the build's workspace (Cartesian densities, partial
+ * J/K, padded ket-side J: 8 * n_cc * (3 nbd + 8 nb) bytes, n_cc = sum of the
+ * molecules' squared Cartesian AO counts, nbd = 2 nb for a nonsymmetric K,
+ * else nb) lives in one 64 MiB device array per GPU, held until the process
+ * ends; larger requests return GSCF_INVALID_ARGUMENT.  Exchange rows are
+ * flushed by atomics (no partial slabs), so results equal gscf_jk_build's to
+ * FP64 rounding.  Calls on one GPU share that array: calls that overlap on one
+ * GPU (different streams or threads) race and must be serialised by the
+ * caller.  The plan must not be profiling.  Without GSCF_FLAG_NO_VALIDATE the
+ * finite-density check synchronises the stream (pinned host flag, no
+ * allocation). */
+int gscf_jk_build_noalloc(gscf_plan* plan, const double* density, int32_t nb, double* out,
+                          int32_t with_j, int32_t with_k, uint32_t flags, double mixed_threshold,
+                          void* stream);
+
 /* Density-fitting integrals: the reference's compact target layouts 2 and 3
  * (GradSCFCompact -> density_fitting, csrc/compact.cc:5-21,
  * csrc/density_fitting.cc:5-34).  The tables hold the orbital shells
