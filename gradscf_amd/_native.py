@@ -45,7 +45,7 @@ class ClassTuning(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("general_contraction", ctypes.c_int32), ("streams", ctypes.c_int32),
                 ("partial_sums", ctypes.c_int32), ("lds_limit", ctypes.c_int32),
-                ("deterministic", ctypes.c_int32)]
+                ("deterministic", ctypes.c_int32), ("balance", ctypes.c_int32)]
 
 
 class NativeError(RuntimeError):

@@ -190,7 +190,7 @@ static void ensure_tuning() {
 
 // defaults: segmented general contractions (faster on the measured cc-pVDZ
 // workloads; a tuning table may switch it on), four launch streams
-static gscf_options g_options = {0, 4, 1, 0, 0};
+static gscf_options g_options = {0, 4, 1, 0, 0, 1};
 constexpr int kMaxMoleculeCartAO = 16384;  // per molecule (32-bit in-molecule offsets)
 constexpr int kMaxStreams = 4;
 
@@ -356,8 +356,8 @@ struct PlanData {
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
-  struct ProfRec { int cls; size_t ev; int64_t build; };
-  std::vector<ProfRec> prof_recs;  // (canonical class, event pair offset, build number)
+  struct ProfRec { int cls; size_t ev; int64_t build; int qci; };
+  std::vector<ProfRec> prof_recs;  // (canonical class, event pair offset, build number, class list index)
   int64_t prof_builds = 0;
   // class launches of one build spread over the caller's stream and side
   // streams (default priority, non-blocking), joined back by events
@@ -730,6 +730,58 @@ __global__ void expand_kernel(const int64_t* __restrict__ qoff, int nbra, int br
     rr -= c - a;
   }
   out[g] = make_int2(bra0 + lo, ket0 + ket);
+}
+
+// Order of the quartets inside every bra-major chunk: by ket primitive-pair
+// count (descending), then by the squared distance between the bra and ket
+// pair midpoints (ascending).  Lanes of a wavefront then run the same number
+// of ket primitive iterations, and their Boys arguments T = rho |P - Q|^2
+// stay close, so the wavefront mostly takes one branch of the Boys
+// evaluation (grid expansion below the switch, asymptotic form above) and
+// reads neighbouring grid rows.  One workgroup per chunk: 64-bit keys
+// (count, distance bits, position) bitonic-sorted in LDS, then the quartets
+// gathered through a scratch copy.  nq of a chunk is at most kSortMax.
+constexpr int kSortMax = 8192;
+__global__ __launch_bounds__(256) void sort_chunk_kernel(const BraChunk* __restrict__ chunks,
+                                                         const Pair* __restrict__ pairs,
+                                                         const ShellXYZ* __restrict__ shells,
+                                                         const int2* __restrict__ src, int2* __restrict__ dst) {
+  __shared__ uint64_t key[kSortMax];
+  const BraChunk c = chunks[blockIdx.x];
+  const int nq = c.nq;
+  int np2 = 1;
+  while (np2 < nq) np2 <<= 1;
+  for (int i = threadIdx.x; i < np2; i += blockDim.x) {
+    uint64_t k = ~0ull;
+    if (i < nq) {
+      const int2 q = src[c.q0 + i];
+      const Pair b = pairs[q.x], t = pairs[q.y];
+      const ShellXYZ A = shells[b.sa], B = shells[b.sb], C = shells[t.sa], D = shells[t.sb];
+      const double dx = 0.5 * (A.x + B.x - C.x - D.x), dy = 0.5 * (A.y + B.y - C.y - D.y),
+                   dz = 0.5 * (A.z + B.z - C.z - D.z);
+      const float d2 = (float)(dx * dx + dy * dy + dz * dz);
+      const uint32_t fb = __float_as_uint(d2) >> 1;  // non-negative: bits order as the values
+      const uint32_t npk = 65535u - (uint32_t)min(t.np, 65535);
+      k = ((uint64_t)npk << 45) | ((uint64_t)fb << 14) | (uint64_t)i;
+    }
+    key[i] = k;
+  }
+  __syncthreads();
+  for (int size = 2; size <= np2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < np2 / 2; i += blockDim.x) {
+        const int lo = 2 * i - (i & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const uint64_t a = key[lo], b = key[hi];
+        if ((a > b) == up) {
+          key[lo] = b;
+          key[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) dst[c.q0 + i] = src[c.q0 + (int)(key[i] & 0x3fff)];
 }
 
 // dmax[m][S][T] = max over batches and over the Cartesian block (S, T) of |D_m|
@@ -1451,6 +1503,19 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             HIP_OK(hipStreamSynchronize(s));
             HIP_OK(hipFree(d_qoff));
           }
+          int64_t nq_chunk_max = 0;
+          for (const BraChunk& c : qc.chunks) nq_chunk_max = std::max<int64_t>(nq_chunk_max, c.nq);
+          // (tuned chunk sizes above kSortMax keep the expansion order)
+          if (qc.d_chunks && !qc.chunks.empty() && nq_chunk_max <= kSortMax) {
+            int2* tmp = nullptr;
+            HIP_OK(hipMalloc(&tmp, sizeof(int2) * qc.nq));
+            HIP_OK(hipMemcpyAsync(tmp, qc.d_quartets, sizeof(int2) * qc.nq, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(sort_chunk_kernel, dim3((unsigned)qc.chunks.size()), dim3(256), 0, s, qc.d_chunks,
+                               S.d_pairs, S.d_shells, tmp, qc.d_quartets);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipStreamSynchronize(s));
+            HIP_OK(hipFree(tmp));
+          }
           qc.d_tasks = dev_upload(qc.tasks, s);
           S.nquartets += qc.nq;
           S.classes.push_back(std::move(qc));
@@ -1670,10 +1735,15 @@ static bool build_contracted_set(PlanData& P, hipStream_t s) {
 }
 
 // Spread the class launches of a build over the plan's streams: longest
-// estimated class first onto the least-loaded stream (stream 0 is the
-// caller's).  Side streams and the fork/join events are created here.
-static void assign_streams(PlanData& P) {
-  const int ns = std::max(1, std::min<int>(kMaxStreams, P.options.streams));
+// class first onto the least-loaded stream (stream 0 is the caller's), by
+// qc.cost (the plan-time work model, or measured durations after
+// balance_lanes), and build each stream's exchange-row reduction lists.
+static void assign_lanes(PlanData& P) {
+  const int ns = (int)P.side.size() + 1;
+  for (int64_t* p : P.d_lane_krp) HIP_OK(hipFree(p));
+  for (int64_t* p : P.d_lane_kent) HIP_OK(hipFree(p));
+  P.d_lane_krp.clear();
+  P.d_lane_kent.clear();
   P.lane_classes.assign(ns, {});
   std::vector<int> order(P.jk.classes.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
@@ -1685,15 +1755,6 @@ static void assign_streams(PlanData& P) {
     load[l] += P.jk.classes[ci].cost;
     P.lane_classes[l].push_back(ci);
   }
-  for (int l = 1; l < ns; ++l) {
-    hipStream_t st;
-    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    P.side.push_back(st);
-    hipEvent_t ev;
-    HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    P.join_ev.push_back(ev);
-  }
-  if (ns > 1) HIP_OK(hipEventCreateWithFlags(&P.fork_ev, hipEventDisableTiming));
   // per-stream row lists of the exchange-row slabs
   const PairSet& S = P.jk;
   if (ns > 1 && S.part_total > 0 && !S.h_kent.empty()) {
@@ -1715,6 +1776,21 @@ static void assign_streams(PlanData& P) {
     // the lists are read from the plan's non-blocking streams
     HIP_OK(hipDeviceSynchronize());
   }
+}
+
+// Side streams and fork/join events of the class launches, then the lanes.
+static void assign_streams(PlanData& P) {
+  const int ns = std::max(1, std::min<int>(kMaxStreams, P.options.streams));
+  for (int l = 1; l < ns; ++l) {
+    hipStream_t st;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    P.side.push_back(st);
+    hipEvent_t ev;
+    HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    P.join_ev.push_back(ev);
+  }
+  if (ns > 1) HIP_OK(hipEventCreateWithFlags(&P.fork_ev, hipEventDisableTiming));
+  assign_lanes(P);
 }
 
 // Schwarz tables of the deterministic scale (det_scale_kernel): Cartesian
@@ -2154,7 +2230,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
           P.ev_pool.push_back(ev);
         }
       }
-      P.prof_recs.push_back({qc.tuning_index, P.ev_used, P.prof_builds});
+      P.prof_recs.push_back({qc.tuning_index, P.ev_used, P.prof_builds, ci});
       HIP_OK(hipEventRecord(P.ev_pool[P.ev_used], s));
     }
     // widths count every contraction column of a shell
@@ -2321,6 +2397,57 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
                      W.jp, W.jpad, W.kp, P.ncc_tot, P.nn_tot, nb, sym ? 1 : 0, with_j, with_k, P.d_molao,
                      P.nmol, P.d_s2c_idx, P.d_s2c_coef, out);
   HIP_OK(hipGetLastError());
+}
+
+// Stream balance from measured durations: one untimed build (code objects,
+// workspace), then one build with every class launch timed alone by HIP
+// events on the caller's stream; the classes are then spread over the
+// streams by those times.  The plan-time work model weighs primitive
+// quartets and block sizes and misses what bounds the classes with few,
+// long work items (latency of the per-lane item chains), which left streams
+// idle while others still ran such classes back to back.  The probe density
+// is the identity per molecule (the kernels' work does not depend on the
+// density's values).
+static void balance_lanes(PlanData& P, hipStream_t s) {
+  if (P.side.empty() || !P.options.balance || P.jk.classes.empty()) return;
+  const size_t nn = (size_t)P.nn_tot;
+  std::vector<double> eye(nn, 0.0);
+  for (int m = 0; m < P.nmol; ++m)
+    for (int i = 0; i < P.mols[m].n; ++i) eye[(size_t)P.mols[m].sbase + (size_t)i * P.mols[m].n + i] = 1.0;
+  double* d_dens = dev_upload(eye, s);
+  double* d_out = nullptr;
+  HIP_OK(hipMalloc(&d_out, sizeof(double) * 2 * nn));
+  P.prof = true;
+  P.prof_recs.clear();
+  P.prof_builds = 0;
+  P.ev_used = 0;
+  std::vector<double> t(P.jk.classes.size(), 0.0);
+  try {
+    jk_build(P, d_dens, 1, d_out, 1, 1, 0u, 0.0, s);
+    P.prof_recs.clear();
+    P.ev_used = 0;
+    jk_build(P, d_dens, 1, d_out, 1, 1, 0u, 0.0, s);
+    for (const auto& r : P.prof_recs) {
+      HIP_OK(hipEventSynchronize(P.ev_pool[r.ev + 1]));
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, P.ev_pool[r.ev], P.ev_pool[r.ev + 1]));
+      t[r.qci] += ms;
+    }
+    HIP_OK(hipStreamSynchronize(s));
+  } catch (...) {
+    P.prof = false;
+    (void)hipFree(d_dens);
+    (void)hipFree(d_out);
+    throw;
+  }
+  P.prof = false;
+  P.prof_recs.clear();
+  P.prof_builds = 0;
+  P.ev_used = 0;
+  HIP_OK(hipFree(d_dens));
+  HIP_OK(hipFree(d_out));
+  for (size_t i = 0; i < t.size(); ++i) P.jk.classes[i].cost = t[i];
+  assign_lanes(P);
 }
 
 // In-core integrals: the contracted Cartesian blocks of every quartet of the
@@ -3138,6 +3265,7 @@ int gscf_plan_create(const int32_t* atm, int32_t natm, const int32_t* bas, int32
     try {
       validate_tables(p->d, atm, natm, bas, nbas, env, nenv, cart);
       build_plan(p->d, cutoff, (hipStream_t)stream);
+      balance_lanes(p->d, (hipStream_t)stream);
     } catch (...) {
       free_plan(p->d);
       delete p;
@@ -3160,6 +3288,7 @@ int gscf_plan_create_batch(const int32_t* atm, int32_t natm, const int32_t* bas,
     try {
       validate_tables(p->d, atm, natm, bas, nbas, env, nenv, cart, nmol, mol_atm, mol_bas);
       build_plan(p->d, cutoff, (hipStream_t)stream);
+      balance_lanes(p->d, (hipStream_t)stream);
     } catch (...) {
       free_plan(p->d);
       delete p;
@@ -3219,6 +3348,7 @@ int gscf_options_set(const gscf_options* o) {
     if (o->lds_limit != 0 && (o->lds_limit < 4096 || o->lds_limit > (int32_t)kLdsDeviceMax))
       throw InvalidArgument("lds_limit must be 0 (the device's limit) or in 4096..163840 bytes");
     if (o->deterministic != 0 && o->deterministic != 1) throw InvalidArgument("deterministic must be 0 or 1");
+    if (o->balance != 0 && o->balance != 1) throw InvalidArgument("balance must be 0 or 1");
     g_options = *o;
   });
 }

@@ -116,6 +116,10 @@ typedef struct {
                                   integer atomics sum independently of their order (FP64 only;
                                   slower: no partial slabs, Rys table read from global memory);
                                   0 (default): FP64 atomics, equal to rounding */
+  int32_t balance;             /* 1 (default): plan creation times every quartet class of one
+                                  build alone (two builds of an identity density, HIP events)
+                                  and spreads the classes over the streams by those times;
+                                  0: by the plan-time work model (no probe builds) */
 } gscf_options;
 int gscf_options_set(const gscf_options* o);
 int gscf_options_get(gscf_options* o);

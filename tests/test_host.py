@@ -369,14 +369,17 @@ def test_engine_options_api_without_gpu():
         pytest.skip("engine not built")
     from gradscf_amd import autotune as at
     before = at.get_options()
-    default = {"general_contraction": 0, "streams": 4, "partial_sums": 1, "lds_limit": 0, "deterministic": 0}
+    default = {"general_contraction": 0, "streams": 4, "partial_sums": 1, "lds_limit": 0, "deterministic": 0,
+               "balance": 1}
     assert before == default
-    changed = {"general_contraction": 1, "streams": 2, "partial_sums": 1, "lds_limit": 8192, "deterministic": 1}
+    changed = {"general_contraction": 1, "streams": 2, "partial_sums": 1, "lds_limit": 8192, "deterministic": 1,
+               "balance": 0}
     try:
         at.set_options(**changed)
         assert at.get_options() == changed
         for bad in ({"streams": 0}, {"streams": 5}, {"general_contraction": 2}, {"partial_sums": 4},
-                    {"partial_sums": -1}, {"lds_limit": 100}, {"lds_limit": 200000}, {"deterministic": 2}):
+                    {"partial_sums": -1}, {"lds_limit": 100}, {"lds_limit": 200000}, {"deterministic": 2},
+                    {"balance": 2}):
             with pytest.raises(ValueError):
                 at.set_options(**bad)
         assert at.get_options() == changed
