@@ -155,7 +155,13 @@ struct BraArgs {
   // deterministic digestion: [nmol][2] (scale, 1/scale) of the build; NULL
   // for FP64 atomics (tab_lds 0, mixed 0 and part_mode 0 with it)
   const double* det_scale;
+  // bra primitive pairs of the chunk staged in LDS (full-block segmented
+  // classes): capacity in records of kStagedPrim doubles, 0 = not staged
+  int bra_stage;
 };
+
+// Doubles per staged primitive pair: p, px, py, pz, e, 1/p.
+constexpr int kStagedPrim = 6;
 
 struct ParamArgs {
   const Pair* pairs;

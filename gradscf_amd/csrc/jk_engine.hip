@@ -205,6 +205,7 @@ struct QuartetClass {
   int tab_lds = 0;          // Rys table slice staged in LDS
   size_t tab_bytes = 0;
   int nket_max = 0;         // LDS ket-J window capacity (ket pairs), 0 = global atomics
+  int stage_cap = 0;        // staged bra primitive pairs per chunk (full-block segmented classes)
   int threads = 256;
   size_t per_slot = 0;      // LDS bytes of one bra slot for one density
   int64_t nq = 0, nitems = 0;
@@ -1388,7 +1389,15 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             const int wc = ncart_h(qc.lc) * (1 + (pk & 1)), wd = ncart_h(qc.ld) * (1 + (pk >> 1));
             const int rows = wa + wb;
             const int split = (wa * wb * wc * wd > 36) ? wc * wd : 1;
-            const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb * kJBraCopies);
+            // full-block segmented classes stage their chunk's bra primitive
+            // pairs in LDS (kStagedPrim doubles each)
+            const bool stage = split == 1 && !pb && !pk;
+            int bra_np_max = 0;
+            if (stage)
+              for (const Run& r : runs)
+                if (r.cls == cb && r.pat == pb) bra_np_max = std::max(bra_np_max, r.npmax);
+            const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb * kJBraCopies +
+                                                      (size_t)kStagedPrim * bra_np_max + 1);
             const int nr = (qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1;
             size_t pack = std::min<size_t>((size_t)tu.lds_pack, P.lds_budget);
             if (tu.rys_lds && per_slot + rys_slice_bytes(nr) <= P.lds_budget) {
@@ -1452,6 +1461,12 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
                 b = e;
               }
             }
+            if (stage)
+              for (const BraChunk& c : qc.chunks) {
+                int cap = 0;
+                for (int j = 0; j < c.nbra; ++j) cap += S.pairs[c.bra0 + j].np;
+                qc.stage_cap = std::max(qc.stage_cap, cap);
+              }
             // ket-side J in LDS for chunks whose ket window fits kKetLdsBytes per density
             qc.threads = tu.threads;
             if (tu.ket_lds) {
@@ -2267,12 +2282,16 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     b.threads = qc.threads;
     // LDS of one launch over nbd_l exchange densities and nbj_l Coulomb
     // densities; the ket window and the Rys table only where they fit `cap`
+    // staged bra primitive pairs (evaluating launches of full-block segmented classes)
+    const int stage_cap = b.mode == 2 ? 0 : qc.stage_cap;
     auto layout = [&](int nbd_l, int nbj_l, size_t cap, size_t& lds, int& nket_max, int& tab_lds) {
       lds = sizeof(double) * ((size_t)nbd_l * qc.nbra_max * (wa + wb) * P.nc_max +
                               (size_t)nbj_l * qc.nbra_max * wa * wb * kJBraCopies);
       nket_max = with_j ? qc.nket_max : 0;
       const size_t jket_bytes = sizeof(double) * (size_t)nbj_l * nket_max * wc * wd;
       if (nket_max && lds + jket_bytes <= cap) lds += jket_bytes; else nket_max = 0;
+      // (the kernel places the staging area 16-byte aligned after the slabs)
+      if (stage_cap) lds = ((lds + 15) & ~(size_t)15) + sizeof(double) * ((size_t)kStagedPrim * stage_cap + (qc.nbra_max + 2) / 2);
       tab_lds = (b.mode == 2 || det) ? 0 : qc.tab_lds;  // stored blocks need no Rys table
       if (tab_lds && lds + qc.tab_bytes <= cap) lds += qc.tab_bytes; else tab_lds = 0;
     };
@@ -2311,6 +2330,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       b.nbj = nb;
       b.tab_lds = tab_lds;
       b.nket_max = nket_max;
+      b.bra_stage = stage_cap;
       launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, lds, s);
     } else if (per_density) {
       // one launch per density (J and K for D_b, K only for D_b^T of a
@@ -2326,6 +2346,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
         b.with_j = with_j && bt < nb;
         b.tab_lds = tab1;
         b.nket_max = b.with_j ? nket1 : 0;
+        b.bra_stage = stage_cap;
         if (!b.with_j && !b.with_k) continue;
         launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, lds1, s);
       }
@@ -2509,8 +2530,13 @@ static void incore_set(PlanData& P, int enable, hipStream_t s) {
     b.pair_q = S.d_pair_q;
     b.threads = qc.threads;
     b.bstride = P.ncc_tot;
-    // no densities: the pass only evaluates and stores
-    launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, 0, s);
+    // no densities: the pass only evaluates and stores; it stages the bra
+    // primitive pairs like the evaluating builds, so that both sum every
+    // quartet's primitives in one order (the deterministic mode's in-core
+    // builds equal its direct builds bit for bit)
+    b.bra_stage = qc.stage_cap;
+    const size_t lds = qc.stage_cap ? sizeof(double) * ((size_t)kStagedPrim * qc.stage_cap + (qc.nbra_max + 2) / 2) : 0;
+    launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, lds, s);
   }
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(s));

@@ -554,7 +554,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
                                          const int2 qk, int slot, int s0, int S, int kc, double* __restrict__ krow,
                                          double* __restrict__ jrow, double* __restrict__ jket, int kwin0,
                                          const double* __restrict__ tab, double* __restrict__ cq,
-                                         double dscale = 1.0) {
+                                         double dscale = 1.0, const double* __restrict__ bpl = nullptr) {
   using K = Class<LA, LB, LC, LD>;
   using W = Wide<LA, LB, LC, LD, CA, CB, CC, CD>;
   constexpr int WA = W::WA, WB = W::WB, WC = W::WC, WD = W::WD;
@@ -592,6 +592,29 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
         for (int ic = 0; ic < WC; ++ic)
 #pragma unroll
           for (int id = 0; id < WD; ++id) acc[ia][ib][ic][id] = R(0);
+    bool staged = false;
+    if constexpr (!W::GEN) {
+      staged = bpl != nullptr;
+      // ket primitives outermost (one memory read of each per lane), the
+      // chunk's bra primitives from LDS inside
+      for (int kq = 0; staged && kq < kp.np; ++kq) {
+        const PrimPair Q = a.prims[kp.poff + kq];
+        for (int ip = s0; ip < bp.np; ip += S) {
+          const double* r = bpl + ip * kStagedPrim;
+          PrimPair P;
+          P.p = r[0];
+          P.px = r[1];
+          P.py = r[2];
+          P.pz = r[3];
+          P.e = r[4];
+          P.ip = r[5];
+          PrimQ<K::NR> q;
+          prim_setup<K::NR, MOMENTS>(P, Q, geo, q, tab);
+          accumulate_full<LA, LB, LC, LD, R>(q, AB, CD_, acc);
+        }
+      }
+    }
+    if (!staged)
     for (int ip = s0; ip < bp.np; ip += S) {
       const PrimPair P = a.prims[bp.poff + ip];
       // a side with one column already carries its coefficients in e
@@ -929,12 +952,44 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   const int jket_size = a.nbj * a.nket_max * W::NKET;
   const int zero_size = krow_size + jrow_size + (jket ? jket_size : 0);
   for (int i = threadIdx.x; i < zero_size; i += blockDim.x) lds[i] = 0.0;
+  // staged bra primitive pairs [records][kStagedPrim] and the first record
+  // of every slot (full-block segmented classes that evaluate)
+  constexpr bool kStage = !W::kSplit && !W::GEN && MODE != 2;
+  const int stage_base = (krow_size + jrow_size + jket_size + 1) & ~1;  // 16-byte aligned
+  const int stage_size = kStage && a.bra_stage ? kStagedPrim * a.bra_stage + (a.nbra_max + 2) / 2 : 0;
+  double* bstage = lds + stage_base;
+  int* bslot = reinterpret_cast<int*>(bstage + kStagedPrim * a.bra_stage);
+  if constexpr (kStage) {
+    if (a.bra_stage) {
+      if (threadIdx.x == 0) {
+        int o = 0;
+        for (int j = 0; j < ch.nbra; ++j) {
+          bslot[j] = o;
+          o += a.pairs[ch.bra0 + j].np;
+        }
+      }
+      __syncthreads();
+      for (int j = 0; j < ch.nbra; ++j) {
+        const Pair bp = a.pairs[ch.bra0 + j];
+        double* dst = bstage + kStagedPrim * bslot[j];
+        for (int k = threadIdx.x; k < bp.np; k += blockDim.x) {
+          const PrimPair P = a.prims[bp.poff + k];
+          dst[kStagedPrim * k + 0] = P.p;
+          dst[kStagedPrim * k + 1] = P.px;
+          dst[kStagedPrim * k + 2] = P.py;
+          dst[kStagedPrim * k + 3] = P.pz;
+          dst[kStagedPrim * k + 4] = P.e;
+          dst[kStagedPrim * k + 5] = P.ip;
+        }
+      }
+    }
+  }
   const double* tab = rys_tab + rys_table_offset[K::NR - 1];
   // one or two roots: the table staged in LDS, or the Boys-moment quadrature
   // (global Boys grid) when the table stays in global memory
   constexpr bool kMoments = K::NR <= 2 && !TLDS;
   if constexpr (TLDS && MODE != 2) {
-    double* ltab = lds + ((krow_size + jrow_size + jket_size + 1) & ~1);  // 16-byte aligned
+    double* ltab = lds + ((stage_base + stage_size + 1) & ~1);  // 16-byte aligned
     for (int i = threadIdx.x; i < rys_slice_size(K::NR); i += blockDim.x) ltab[i] = tab[i];
     tab = ltab;
   }
@@ -984,13 +1039,15 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
     }
     if constexpr (MIX) {
       if (lowp) {
-        bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, float>(a, mi, bp, kp, qk, slot, s0, S, kc, krow, jrow,
-                                                                 jket, ch.ket0, tab, cq);
+        bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, float>(
+            a, mi, bp, kp, qk, slot, s0, S, kc, krow, jrow, jket, ch.ket0, tab, cq, 1.0,
+            (kStage && a.bra_stage) ? bstage + kStagedPrim * bslot[slot] : nullptr);
         continue;
       }
     }
+    const double* bpl = (kStage && a.bra_stage) ? bstage + kStagedPrim * bslot[slot] : nullptr;
     bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double, MODE, DET>(a, mi, bp, kp, qk, slot, s0, S, kc, krow,
-                                                                           jrow, jket, ch.ket0, tab, cq, dscale);
+                                                                           jrow, jket, ch.ket0, tab, cq, dscale, bpl);
   }
   __syncthreads();
   if (a.part_mode & 1) {
