@@ -1396,7 +1396,7 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             if (stage)
               for (const Run& r : runs)
                 if (r.cls == cb && r.pat == pb) bra_np_max = std::max(bra_np_max, r.npmax);
-            const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb * kJBraCopies +
+            const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb * (kJBraCopies + 1) +
                                                       (size_t)kStagedPrim * bra_np_max + 1);
             const int nr = (qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1;
             size_t pack = std::min<size_t>((size_t)tu.lds_pack, P.lds_budget);
@@ -2290,6 +2290,8 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       nket_max = with_j ? qc.nket_max : 0;
       const size_t jket_bytes = sizeof(double) * (size_t)nbj_l * nket_max * wc * wd;
       if (nket_max && lds + jket_bytes <= cap) lds += jket_bytes; else nket_max = 0;
+      // the slots' bra density blocks D_ab + D_ba of every Coulomb density
+      if (with_j) lds += sizeof(double) * (size_t)nbj_l * qc.nbra_max * wa * wb;
       // (the kernel places the staging area 16-byte aligned after the slabs)
       if (stage_cap) lds = ((lds + 15) & ~(size_t)15) + sizeof(double) * ((size_t)kStagedPrim * stage_cap + (qc.nbra_max + 2) / 2);
       tab_lds = (b.mode == 2 || det) ? 0 : qc.tab_lds;  // stored blocks need no Rys table

@@ -554,7 +554,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
                                          const int2 qk, int slot, int s0, int S, int kc, double* __restrict__ krow,
                                          double* __restrict__ jrow, double* __restrict__ jket, int kwin0,
                                          const double* __restrict__ tab, double* __restrict__ cq,
-                                         double dscale = 1.0, const double* __restrict__ bpl = nullptr) {
+                                         double dscale = 1.0, const double* __restrict__ bpl = nullptr,
+                                         const double* __restrict__ dabs = nullptr) {
   using K = Class<LA, LB, LC, LD>;
   using W = Wide<LA, LB, LC, LD, CA, CB, CC, CD>;
   constexpr int WA = W::WA, WB = W::WB, WC = W::WC, WD = W::WD;
@@ -682,11 +683,12 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
 #pragma unroll
           for (int id = 0; id < WD; ++id)
             dcd[ic][id] = ld_u(Dm, rc + ic * n + od + id) + ld_u(Dm, rd + id * n + oc + ic);
+        // D_ab + D_ba of the bra block: staged per slot in LDS
+        const double* __restrict__ dsl = dabs + (bt * a.nbra_max + slot) * W::NBRA;
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
-          for (int ib = 0; ib < WB; ++ib)
-            dab[ia][ib] = ld_u(Dm, ra + ia * n + ob + ib) + ld_u(Dm, rb + ib * n + oa + ia);
+          for (int ib = 0; ib < WB; ++ib) dab[ia][ib] = dsl[ia * WB + ib];
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
@@ -844,13 +846,15 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
       if (a.with_j && bt < a.nbj) {
         double* jr = jrow + ((bt * a.nbra_max + slot) * kJBraCopies + jcopy) * W::NBRA;
         const double dcd = ld_u(Dm, c * n + d) + ld_u(Dm, d * n + c);
+        // D_ab + D_ba of the bra block: staged per slot in LDS
+        const double* __restrict__ dsl = dabs + (bt * a.nbra_max + slot) * W::NBRA;
         double s2 = 0.0;
 #pragma unroll
         for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
           for (int ib = 0; ib < WB; ++ib) {
             acc_add<DET>(jr + ia * WB + ib, f * (double)acc[ia][ib] * dcd);
-            s2 += (double)acc[ia][ib] * (ld_u(Dm, ra + ia * n + ob + ib) + ld_u(Dm, rb + ib * n + oa + ia));
+            s2 += (double)acc[ia][ib] * dsl[ia * WB + ib];
           }
         if (jket)
           acc_add_wg<DET>(jket + (bt * a.nket_max + (qk.y - kwin0)) * W::NKET + kc, f * s2);
@@ -952,10 +956,27 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   const int jket_size = a.nbj * a.nket_max * W::NKET;
   const int zero_size = krow_size + jrow_size + (jket ? jket_size : 0);
   for (int i = threadIdx.x; i < zero_size; i += blockDim.x) lds[i] = 0.0;
+  // D_ab + D_ba of every slot's bra block, per Coulomb density: read by the
+  // bra-side and ket-side J digestion of every quartet of the slot
+  const int dab_size = a.with_j ? a.nbj * a.nbra_max * W::NBRA : 0;
+  double* dabs = lds + krow_size + jrow_size + jket_size;
+  for (int i = threadIdx.x; i < dab_size; i += blockDim.x) {
+    const int e = i % W::NBRA;
+    const int slot = (i / W::NBRA) % a.nbra_max;
+    const int bt = i / (W::NBRA * a.nbra_max);
+    double v = 0.0;
+    if (slot < ch.nbra) {
+      const Pair bp = a.pairs[ch.bra0 + slot];
+      const unsigned oa = a.shells[bp.sa].ao + e / W::WB, ob = a.shells[bp.sb].ao + e % W::WB;
+      const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
+      v = Dm[(size_t)oa * mi.n + ob] + Dm[(size_t)ob * mi.n + oa];
+    }
+    dabs[i] = v;
+  }
   // staged bra primitive pairs [records][kStagedPrim] and the first record
   // of every slot (full-block segmented classes that evaluate)
   constexpr bool kStage = !W::kSplit && !W::GEN && MODE != 2;
-  const int stage_base = (krow_size + jrow_size + jket_size + 1) & ~1;  // 16-byte aligned
+  const int stage_base = (krow_size + jrow_size + jket_size + dab_size + 1) & ~1;  // 16-byte aligned
   const int stage_size = kStage && a.bra_stage ? kStagedPrim * a.bra_stage + (a.nbra_max + 2) / 2 : 0;
   double* bstage = lds + stage_base;
   int* bslot = reinterpret_cast<int*>(bstage + kStagedPrim * a.bra_stage);
@@ -1041,13 +1062,14 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
       if (lowp) {
         bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, float>(
             a, mi, bp, kp, qk, slot, s0, S, kc, krow, jrow, jket, ch.ket0, tab, cq, 1.0,
-            (kStage && a.bra_stage) ? bstage + kStagedPrim * bslot[slot] : nullptr);
+            (kStage && a.bra_stage) ? bstage + kStagedPrim * bslot[slot] : nullptr, dabs);
         continue;
       }
     }
     const double* bpl = (kStage && a.bra_stage) ? bstage + kStagedPrim * bslot[slot] : nullptr;
     bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double, MODE, DET>(a, mi, bp, kp, qk, slot, s0, S, kc, krow,
-                                                                           jrow, jket, ch.ket0, tab, cq, dscale, bpl);
+                                                                           jrow, jket, ch.ket0, tab, cq, dscale, bpl,
+                                                                           dabs);
   }
   __syncthreads();
   if (a.part_mode & 1) {
