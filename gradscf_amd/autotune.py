@@ -38,8 +38,9 @@ SEARCH = {
     "rys_lds": (0, 1),
     "ket_lds": (0, 1),
     "threads": (256, 512),
-    ("chunk_items", "target_prim"): tuple((c, t) for c in (2048, 4096, 8192) for t in (12, 24, 48, 96)),
+    ("chunk_items", "target_prim"): tuple((c, t) for c in (1024, 2048, 4096, 8192) for t in (12, 24, 48, 96)),
     "bras_max": (8, 16, 32),
+    "lds_pack": (32768, 49152, 65536),
 }
 
 
