@@ -160,8 +160,10 @@ struct BraArgs {
   int bra_stage;
 };
 
-// Doubles per staged primitive pair: p, px, py, pz, e, 1/p.
+// Doubles per staged primitive pair: p, P - A (A: the pair's first centre),
+// e, 1/p; a bra with general-contraction columns adds c_A[2], c_B[2].
 constexpr int kStagedPrim = 6;
+constexpr int kStagedPrimCoef = kStagedPrim + 4;
 
 struct ParamArgs {
   const Pair* pairs;

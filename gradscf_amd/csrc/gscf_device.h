@@ -193,16 +193,26 @@ __host__ __device__ constexpr int binom(int n, int k) {
 //   G(n+1,0) = C00 G(n,0) + n B10 G(n-1,0)
 //   G(n,m+1) = C0p G(n,m) + m B01 G(n,m-1) + n B00 G(n-1,m)
 // ---------------------------------------------------------------------------
+// (terms with a zero factor are left out rather than added as zeros, which
+// the compiler may not drop: x + 0 is not x for x = -0)
 template <int NM, int MM, typename R = double>
 __device__ __forceinline__ void vrr(R c00, R c0p, R b10, R b01, R b00, R base, R (&G)[NM + 1][MM + 1]) {
   G[0][0] = base;
 #pragma unroll
-  for (int n = 0; n < NM; ++n) G[n + 1][0] = c00 * G[n][0] + (n ? R(n) * b10 * G[n - 1][0] : R(0));
+  for (int n = 0; n < NM; ++n) {
+    R t = c00 * G[n][0];
+    if (n) t += R(n) * b10 * G[n - 1][0];
+    G[n + 1][0] = t;
+  }
 #pragma unroll
   for (int m = 0; m < MM; ++m)
 #pragma unroll
-    for (int n = 0; n <= NM; ++n)
-      G[n][m + 1] = c0p * G[n][m] + (m ? R(m) * b01 * G[n][m - 1] : R(0)) + (n ? R(n) * b00 * G[n - 1][m] : R(0));
+    for (int n = 0; n <= NM; ++n) {
+      R t = c0p * G[n][m];
+      if (m) t += R(m) * b01 * G[n][m - 1];
+      if (n) t += R(n) * b00 * G[n - 1][m];
+      G[n][m + 1] = t;
+    }
 }
 
 // Horizontal transfer (x - B) = (x - A) + (A - B) on the bra index of a 1D vector.

@@ -1389,15 +1389,16 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             const int wc = ncart_h(qc.lc) * (1 + (pk & 1)), wd = ncart_h(qc.ld) * (1 + (pk >> 1));
             const int rows = wa + wb;
             const int split = (wa * wb * wc * wd > 36) ? wc * wd : 1;
-            // full-block segmented classes stage their chunk's bra primitive
-            // pairs in LDS (kStagedPrim doubles each)
-            const bool stage = split == 1 && !pb && !pk;
+            // full-block classes stage their chunk's bra primitive pairs in
+            // LDS (kStagedPrim doubles each, kStagedPrimCoef with bra columns)
+            const bool stage = split == 1;
+            const int rec = pb ? kStagedPrimCoef : kStagedPrim;
             int bra_np_max = 0;
             if (stage)
               for (const Run& r : runs)
                 if (r.cls == cb && r.pat == pb) bra_np_max = std::max(bra_np_max, r.npmax);
             const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb * (kJBraCopies + 1) +
-                                                      (size_t)kStagedPrim * bra_np_max + 1);
+                                                      (size_t)rec * bra_np_max + 1);
             const int nr = (qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1;
             size_t pack = std::min<size_t>((size_t)tu.lds_pack, P.lds_budget);
             if (tu.rys_lds && per_slot + rys_slice_bytes(nr) <= P.lds_budget) {
@@ -2284,6 +2285,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     // densities; the ket window and the Rys table only where they fit `cap`
     // staged bra primitive pairs (evaluating launches of full-block segmented classes)
     const int stage_cap = b.mode == 2 ? 0 : qc.stage_cap;
+    const int stage_rec = qc.pb ? kStagedPrimCoef : kStagedPrim;
     auto layout = [&](int nbd_l, int nbj_l, size_t cap, size_t& lds, int& nket_max, int& tab_lds) {
       lds = sizeof(double) * ((size_t)nbd_l * qc.nbra_max * (wa + wb) * P.nc_max +
                               (size_t)nbj_l * qc.nbra_max * wa * wb * kJBraCopies);
@@ -2293,7 +2295,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       // the slots' bra density blocks D_ab + D_ba of every Coulomb density
       if (with_j) lds += sizeof(double) * (size_t)nbj_l * qc.nbra_max * wa * wb;
       // (the kernel places the staging area 16-byte aligned after the slabs)
-      if (stage_cap) lds = ((lds + 15) & ~(size_t)15) + sizeof(double) * ((size_t)kStagedPrim * stage_cap + (qc.nbra_max + 2) / 2);
+      if (stage_cap) lds = ((lds + 15) & ~(size_t)15) + sizeof(double) * ((size_t)stage_rec * stage_cap + (qc.nbra_max + 2) / 2);
       tab_lds = (b.mode == 2 || det) ? 0 : qc.tab_lds;  // stored blocks need no Rys table
       if (tab_lds && lds + qc.tab_bytes <= cap) lds += qc.tab_bytes; else tab_lds = 0;
     };
@@ -2537,7 +2539,8 @@ static void incore_set(PlanData& P, int enable, hipStream_t s) {
     // quartet's primitives in one order (the deterministic mode's in-core
     // builds equal its direct builds bit for bit)
     b.bra_stage = qc.stage_cap;
-    const size_t lds = qc.stage_cap ? sizeof(double) * ((size_t)kStagedPrim * qc.stage_cap + (qc.nbra_max + 2) / 2) : 0;
+    const size_t lds = qc.stage_cap ? sizeof(double) * ((size_t)(qc.pb ? kStagedPrimCoef : kStagedPrim) * qc.stage_cap +
+                                                        (qc.nbra_max + 2) / 2) : 0;
     launch_jk_bra_class(qc.la, qc.lb, qc.lc, qc.ld, qc.pb, qc.pk, b, lds, s);
   }
   HIP_OK(hipGetLastError());

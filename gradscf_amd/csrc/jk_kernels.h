@@ -15,6 +15,8 @@
 // Kp[b,c] += f v D_ad, Kp[b,d] += f v D_ac, with f = 1/2 per coincident
 // shell / pair; J = Jp + Jp^T and K = Kp(D) + Kp(D^T)^T are formed afterwards.
 #pragma once
+#include <type_traits>
+
 #include "gscf_device.h"
 #include "gscf_args.h"
 
@@ -123,6 +125,62 @@ __device__ __forceinline__ void accumulate_full(const PrimQ<Class<LA, LB, LC, LD
             const R vz = H[2][cart_comp(LA, ia, 2)][cart_comp(LB, ib, 2)][cart_comp(LC, ic, 2)][cart_comp(LD, id, 2)];
             acc[ia][ib][ic][id] += vx * vy * vz;
           }
+  }
+}
+
+// One primitive quartet from a staged bra record (p, P - A, e, 1/p) and a
+// ket primitive pair (with QA = Q - A, QC = Q - C), added into blk: the
+// one-root s-ket classes in the Obara-Saika form from F_0 (and F_1),
+//   (s|ss) = pref F0,  (p_x|ss) = pref (PA_x F0 - (q/(p+q)) PQ_x F1),
+// everything else by the Rys quadrature (accumulate_full).
+template <int LA, int LB, int LC, int LD, int MOMENTS, typename R>
+__device__ __forceinline__ void staged_quartet(const double* __restrict__ r, const PrimPair& Q, const double (&QA)[3],
+                                               const double (&QC)[3], const R AB[3], const R CD[3],
+                                               const double* __restrict__ tab,
+                                               R (&blk)[ncart(LA)][ncart(LB)][ncart(LC)][ncart(LD)]) {
+  using K = Class<LA, LB, LC, LD>;
+  const double PA[3] = {r[1], r[2], r[3]};
+  const double PQ[3] = {PA[0] - QA[0], PA[1] - QA[1], PA[2] - QA[2]};
+  const double pq = r[0] + Q.p;
+  const double rs = rsqrt_pos(pq);
+  const double ipq = rs * rs;
+  const double T = r[0] * Q.p * ipq * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+  const double pref = r[4] * Q.e * rs;
+  if constexpr (K::NR == 1 && LB == 0 && std::is_same<R, double>::value) {
+    double F0, F1 = 0.0;
+    if (T < BOYS_TMAX) {
+      double F[LA + 1];
+      boys_taylor<LA + 1>(T, F);
+      F0 = F[0];
+      if constexpr (LA == 1) F1 = F[1];
+    } else {
+      const double sq = rsqrt_pos(T);
+      F0 = 0.88622692545275801365 * sq;  // sqrt(pi) / 2 / sqrt(T)
+      F1 = 0.5 * F0 * sq * sq;
+    }
+    if constexpr (LA == 0) {
+      blk[0][0][0][0] += pref * F0;
+    } else {
+      const double af = pref * F0, bf = pref * Q.p * ipq * F1;
+#pragma unroll
+      for (int x = 0; x < 3; ++x) blk[x][0][0][0] += PA[x] * af - PQ[x] * bf;
+    }
+  } else {
+    PrimQ<K::NR> q;
+    q.ipq = ipq;
+    q.ip = r[5];
+    q.iq = Q.ip;
+    q.qf = Q.p * ipq;
+    q.pf = r[0] * ipq;
+    q.pref = pref;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      q.PA[x] = PA[x];
+      q.QC[x] = QC[x];
+      q.PQ[x] = PQ[x];
+    }
+    rys_roots_tab<K::NR, MOMENTS>(tab, T, q.s, q.w);
+    accumulate_full<LA, LB, LC, LD, R>(q, AB, CD, blk);
   }
 }
 
@@ -593,26 +651,64 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
         for (int ic = 0; ic < WC; ++ic)
 #pragma unroll
           for (int id = 0; id < WD; ++id) acc[ia][ib][ic][id] = R(0);
-    bool staged = false;
-    if constexpr (!W::GEN) {
-      staged = bpl != nullptr;
-      // ket primitives outermost (one memory read of each per lane), the
-      // chunk's bra primitives from LDS inside
-      for (int kq = 0; staged && kq < kp.np; ++kq) {
-        const PrimPair Q = a.prims[kp.poff + kq];
+    const bool staged = bpl != nullptr;
+    // staged bra primitives: ket primitives outermost (one memory read of
+    // each per lane), the chunk's bra primitives from LDS inside
+    for (int kq = 0; staged && kq < kp.np; ++kq) {
+      const PrimPair Q = a.prims[kp.poff + kq];
+      // ket primitive relative to the bra's first centre and its own
+      const double QA[3] = {Q.px - geo.A[0], Q.py - geo.A[1], Q.pz - geo.A[2]};
+      const double QC[3] = {Q.px - geo.C[0], Q.py - geo.C[1], Q.pz - geo.C[2]};
+      if constexpr (!W::GEN) {
+        for (int ip = s0; ip < bp.np; ip += S)
+          staged_quartet<LA, LB, LC, LD, MOMENTS, R>(bpl + ip * kStagedPrim, Q, QA, QC, AB, CD_, tab, acc);
+      } else {
+        // general contraction: coefficient-free blocks summed over the bra
+        // primitives with the bra column products (record: ..., c_A[2],
+        // c_B[2]), then into every ket column with the ket's products
+        constexpr int REC = kStagedPrim + (CA * CB > 1 ? 4 : 0);
+        R tmp[WA][WB][K::NC][K::ND];
+#pragma unroll
+        for (int e = 0; e < WA * WB * K::NC * K::ND; ++e) (&tmp[0][0][0][0])[e] = R(0);
         for (int ip = s0; ip < bp.np; ip += S) {
-          const double* r = bpl + ip * kStagedPrim;
-          PrimPair P;
-          P.p = r[0];
-          P.px = r[1];
-          P.py = r[2];
-          P.pz = r[3];
-          P.e = r[4];
-          P.ip = r[5];
-          PrimQ<K::NR> q;
-          prim_setup<K::NR, MOMENTS>(P, Q, geo, q, tab);
-          accumulate_full<LA, LB, LC, LD, R>(q, AB, CD_, acc);
+          const double* __restrict__ r = bpl + ip * REC;
+          R blk[K::NA][K::NB][K::NC][K::ND];
+#pragma unroll
+          for (int e = 0; e < K::NA * K::NB * K::NC * K::ND; ++e) (&blk[0][0][0][0])[e] = R(0);
+          staged_quartet<LA, LB, LC, LD, MOMENTS, R>(r, Q, QA, QC, AB, CD_, tab, blk);
+#pragma unroll
+          for (int x = 0; x < CA; ++x)
+#pragma unroll
+            for (int y = 0; y < CB; ++y) {
+              const R cf = CA * CB > 1 ? (R)(r[kStagedPrim + x] * r[kStagedPrim + 2 + y]) : R(1);
+#pragma unroll
+              for (int ia = 0; ia < K::NA; ++ia)
+#pragma unroll
+                for (int ib = 0; ib < K::NB; ++ib)
+#pragma unroll
+                  for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+                    for (int id = 0; id < K::ND; ++id)
+                      tmp[x * K::NA + ia][y * K::NB + ib][ic][id] += cf * blk[ia][ib][ic][id];
+            }
         }
+        PrimCoef cQ = {{1.0, 0.0}, {1.0, 0.0}};
+        if constexpr (CC * CD > 1) cQ = a.coef[kp.poff + kq];
+#pragma unroll
+        for (int z = 0; z < CC; ++z)
+#pragma unroll
+          for (int w = 0; w < CD; ++w) {
+            const R cf = (R)(cQ.ca[z] * cQ.cb[w]);
+#pragma unroll
+            for (int ia = 0; ia < WA; ++ia)
+#pragma unroll
+              for (int ib = 0; ib < WB; ++ib)
+#pragma unroll
+                for (int ic = 0; ic < K::NC; ++ic)
+#pragma unroll
+                  for (int id = 0; id < K::ND; ++id)
+                    acc[ia][ib][z * K::NC + ic][w * K::ND + id] += cf * tmp[ia][ib][ic][id];
+          }
       }
     }
     if (!staged)
@@ -975,11 +1071,13 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   }
   // staged bra primitive pairs [records][kStagedPrim] and the first record
   // of every slot (full-block segmented classes that evaluate)
-  constexpr bool kStage = !W::kSplit && !W::GEN && MODE != 2;
+  constexpr bool kStage = !W::kSplit && MODE != 2;
+  // record doubles: p, P - A, e, 1/p (+ the bra's column coefficients)
+  constexpr int kRec = kStagedPrim + (CA * CB > 1 ? 4 : 0);
   const int stage_base = (krow_size + jrow_size + jket_size + dab_size + 1) & ~1;  // 16-byte aligned
-  const int stage_size = kStage && a.bra_stage ? kStagedPrim * a.bra_stage + (a.nbra_max + 2) / 2 : 0;
+  const int stage_size = kStage && a.bra_stage ? kRec * a.bra_stage + (a.nbra_max + 2) / 2 : 0;
   double* bstage = lds + stage_base;
-  int* bslot = reinterpret_cast<int*>(bstage + kStagedPrim * a.bra_stage);
+  int* bslot = reinterpret_cast<int*>(bstage + kRec * a.bra_stage);
   if constexpr (kStage) {
     if (a.bra_stage) {
       if (threadIdx.x == 0) {
@@ -992,15 +1090,24 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
       __syncthreads();
       for (int j = 0; j < ch.nbra; ++j) {
         const Pair bp = a.pairs[ch.bra0 + j];
-        double* dst = bstage + kStagedPrim * bslot[j];
+        double* dst = bstage + kRec * bslot[j];
         for (int k = threadIdx.x; k < bp.np; k += blockDim.x) {
           const PrimPair P = a.prims[bp.poff + k];
-          dst[kStagedPrim * k + 0] = P.p;
-          dst[kStagedPrim * k + 1] = P.px;
-          dst[kStagedPrim * k + 2] = P.py;
-          dst[kStagedPrim * k + 3] = P.pz;
-          dst[kStagedPrim * k + 4] = P.e;
-          dst[kStagedPrim * k + 5] = P.ip;
+          const ShellXYZ A = a.shells[bp.sa];
+          double* rec = dst + kRec * k;
+          rec[0] = P.p;
+          rec[1] = P.px - A.x;
+          rec[2] = P.py - A.y;
+          rec[3] = P.pz - A.z;
+          rec[4] = P.e;
+          rec[5] = P.ip;
+          if constexpr (CA * CB > 1) {
+            const PrimCoef c = a.coef[bp.poff + k];
+            rec[6] = c.ca[0];
+            rec[7] = c.ca[1];
+            rec[8] = c.cb[0];
+            rec[9] = c.cb[1];
+          }
         }
       }
     }
@@ -1062,11 +1169,11 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
       if (lowp) {
         bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, float>(
             a, mi, bp, kp, qk, slot, s0, S, kc, krow, jrow, jket, ch.ket0, tab, cq, 1.0,
-            (kStage && a.bra_stage) ? bstage + kStagedPrim * bslot[slot] : nullptr, dabs);
+            (kStage && a.bra_stage) ? bstage + kRec * bslot[slot] : nullptr, dabs);
         continue;
       }
     }
-    const double* bpl = (kStage && a.bra_stage) ? bstage + kStagedPrim * bslot[slot] : nullptr;
+    const double* bpl = (kStage && a.bra_stage) ? bstage + kRec * bslot[slot] : nullptr;
     bra_item<LA, LB, LC, LD, CA, CB, CC, CD, kMoments, double, MODE, DET>(a, mi, bp, kp, qk, slot, s0, S, kc, krow,
                                                                            jrow, jket, ch.ket0, tab, cq, dscale, bpl,
                                                                            dabs);
