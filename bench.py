@@ -104,16 +104,21 @@ TUNING_DIR = os.path.join(ROOT, "gradscf_amd", "tuning")
 
 def _kernel_class(name: str):
     """'void gscf::jk_bra_kernel<2, 1, 1, 0, ...>(gscf::BraArgs)' -> ('jk_bra_kernel', 'dpps');
-    the in-core variants of the bra kernel (last template argument 1: evaluate
-    and store, 2: digest stored blocks) get '/store' or '/load'."""
+    the in-core variants of the bra kernel (MODE template argument 1: evaluate
+    and store, 2: digest stored blocks) get '/store' or '/load', the
+    deterministic-mode variant '/det'."""
     import re
     m = re.search(r"(\w+)<(\d), (\d), (\d), (\d)((?:, \w+)*)>", name)
     if not m:
         return None, None
     cls = "".join(CLASS_LETTERS[int(v)] for v in m.groups()[1:5])
     rest = [x.strip() for x in m.group(6).split(",") if x.strip()]
-    if m.group(1) == "jk_bra_kernel" and len(rest) == 7 and rest[-1] in ("1", "2"):
-        cls += "/store" if rest[-1] == "1" else "/load"
+    if m.group(1) == "jk_bra_kernel" and len(rest) >= 7:
+        # rest: column counts (4), Rys table in LDS, mixed precision, MODE[, DET]
+        if rest[6] in ("1", "2"):
+            cls += "/store" if rest[6] == "1" else "/load"
+        if len(rest) >= 8 and rest[7] in ("1", "true"):
+            cls += "/det"
     return m.group(1), cls
 
 
@@ -629,6 +634,9 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true", help="keep the default per-class schedules")
     ap.add_argument("--no-incore", action="store_true",
                     help="skip the in-core integral builds and SCF (blocks stored once, digested per build)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the nonsymmetric-density and deterministic-mode builds (profiling runs: "
+                         "every class kernel dispatch then belongs to the headline builds)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the water/6-31G, mixed C20H42/(H2O)32 and gradient sections")
     ap.add_argument("--retune", action="store_true",
@@ -733,40 +741,42 @@ def main() -> int:
     value = args.steps / elapsed
     log(f"J/K: {ms_per_step:.3f} ms/build, {value:.2f} builds/s")
 
-    # a nonsymmetric density (K needs D and D^T): the dd classes' exchange rows
-    # of both exceed 64 KB together and run one density per launch
-    dns = (dens + 1e-3 * torch.triu(dens, 1)).contiguous()
-    out_ns = torch.empty_like(out)
-    for _ in range(args.warmup):
-        plan.jk(dns[None], symmetric=False, validate=False, out=out_ns)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan.jk(dns[None], symmetric=False, validate=False, out=out_ns)
-    torch.cuda.synchronize()
-    nonsym_ms = (time.perf_counter() - t0) / args.steps * 1e3
-    log(f"nonsymmetric density: {nonsym_ms:.3f} ms/build")
+    nonsym_ms, det_rec = None, None
+    if not args.no_variants:
+        # a nonsymmetric density (K needs D and D^T): the dd classes' exchange rows
+        # of both exceed 64 KB together and run one density per launch
+        dns = (dens + 1e-3 * torch.triu(dens, 1)).contiguous()
+        out_ns = torch.empty_like(out)
+        for _ in range(args.warmup):
+            plan.jk(dns[None], symmetric=False, validate=False, out=out_ns)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.jk(dns[None], symmetric=False, validate=False, out=out_ns)
+        torch.cuda.synchronize()
+        nonsym_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        log(f"nonsymmetric density: {nonsym_ms:.3f} ms/build")
 
-    # deterministic mode (integer accumulation, bitwise repeatable) on the same workload
-    before_opts = at.get_options()
-    at.set_options(deterministic=1)
-    plan_det = DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff)
-    at.set_options(**before_opts)
-    out_det = torch.empty_like(out)
-    for _ in range(args.warmup):
-        plan_det.jk(dens[None], symmetric=True, validate=False, out=out_det)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plan_det.jk(dens[None], symmetric=True, validate=False, out=out_det)
-    torch.cuda.synchronize()
-    det_ms = (time.perf_counter() - t0) / args.steps * 1e3
-    det_again = plan_det.jk(dens[None], symmetric=True, validate=False)
-    det_rec = {"ms_per_step": det_ms, "bitwise_repeatable": bool(torch.equal(det_again, out_det)),
-               "max_abs_jk_diff_vs_fp64_atomics": float((out_det - out).abs().max())}
-    del plan_det
-    log(f"deterministic mode: {det_ms:.3f} ms/build, repeatable {det_rec['bitwise_repeatable']}, "
-        f"max|dJK| vs atomics {det_rec['max_abs_jk_diff_vs_fp64_atomics']:.1e}")
+        # deterministic mode (integer accumulation, bitwise repeatable) on the same workload
+        before_opts = at.get_options()
+        at.set_options(deterministic=1)
+        plan_det = DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff)
+        at.set_options(**before_opts)
+        out_det = torch.empty_like(out)
+        for _ in range(args.warmup):
+            plan_det.jk(dens[None], symmetric=True, validate=False, out=out_det)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan_det.jk(dens[None], symmetric=True, validate=False, out=out_det)
+        torch.cuda.synchronize()
+        det_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        det_again = plan_det.jk(dens[None], symmetric=True, validate=False)
+        det_rec = {"ms_per_step": det_ms, "bitwise_repeatable": bool(torch.equal(det_again, out_det)),
+                   "max_abs_jk_diff_vs_fp64_atomics": float((out_det - out).abs().max())}
+        del plan_det
+        log(f"deterministic mode: {det_ms:.3f} ms/build, repeatable {det_rec['bitwise_repeatable']}, "
+            f"max|dJK| vs atomics {det_rec['max_abs_jk_diff_vs_fp64_atomics']:.1e}")
 
     # mixed precision: same workload, FP32 for quartets with Q_ij Q_kl max|D| < threshold
     thr = args.mixed_threshold
