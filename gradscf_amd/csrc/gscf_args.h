@@ -152,6 +152,12 @@ struct BraArgs {
   int part_mode;
   int64_t part_stride;
   double* part;
+  // exchange-row slab columns: per (molecule, ket mask) the first entry in
+  // colmap and the count of the molecule's Cartesian columns whose shell l is
+  // in the mask; this class's mask (1 << lc | 1 << ld)
+  const int* colmap;
+  const int2* cspace;
+  int cmask;
   // deterministic digestion: [nmol][2] (scale, 1/scale) of the build; NULL
   // for FP64 atomics (tab_lds 0, mixed 0 and part_mode 0 with it)
   const double* det_scale;

@@ -268,6 +268,17 @@ struct PairSet {
   std::vector<int> h_kent_cls;                      // its class
   int64_t* d_row_dst = nullptr;   // per global Cartesian row: first K element and length
   int* d_row_n = nullptr;
+  int* d_row_mol = nullptr;       // molecule of every global Cartesian row
+  // exchange-row slab columns (segmented sets): a class's slab rows hold the
+  // Cartesian AO columns of its molecule's shells whose l is in the class's
+  // ket mask (1 << lc | 1 << ld), the only columns its exchange terms reach.
+  // Per (molecule, mask): (first entry in colmap, count); cinv maps a
+  // molecule's column to its compact index in the mask's list, or -1.
+  std::vector<int2> h_cspace;
+  int* d_colmap = nullptr;
+  int2* d_cspace = nullptr;
+  int64_t* d_cinv_off = nullptr;
+  int* d_cinv = nullptr;
   int64_t nrows = 0;
   int nc_max = 0;
 };
@@ -559,34 +570,50 @@ __global__ void dens_to_cart_kernel(const double* __restrict__ D, int64_t nn_tot
 
 // out[op][b][p][q]: J = T (Jp + Jp^T) T^T, K = T (Kp_b + Kp_b'^T) T^T, per molecule
 // K rows += their partial exchange rows: block (x: global Cartesian row R,
-// y: 64-column range, z: density); the row's slab list is block-uniform (scalar
-// loads) and eight slab rows are read per step so that many loads are in
-// flight.  row_dst[R] / row_n[R]: first element and length of row R in K.
+// y: 64-column range, z: density).  The slab rows that add into row R are
+// listed per ket mask (rp over keys R * 8 + mask, block-uniform scalar loads):
+// a slab row of mask `mask` holds the molecule's columns of that mask in
+// compact order, so every thread looks up its column's compact index once
+// per mask (cinv, -1 when the mask does not contain the column) and then
+// reads eight slab rows per step so that many loads are in flight.
+// row_dst[R] / row_n[R] / row_mol[R]: first element, length and molecule of
+// row R of K.
 __global__ __launch_bounds__(64) void kpart_reduce_kernel(const double* __restrict__ part, int64_t stride,
                                                           const int64_t* __restrict__ rp,
                                                           const int64_t* __restrict__ ent,
                                                           const int64_t* __restrict__ row_dst,
-                                                          const int* __restrict__ row_n, int64_t ncc_tot,
+                                                          const int* __restrict__ row_n,
+                                                          const int* __restrict__ row_mol,
+                                                          const int64_t* __restrict__ cinv_off,
+                                                          const int* __restrict__ cinv, int64_t ncc_tot,
                                                           int atomic, double* __restrict__ kp) {
   const int64_t R = blockIdx.x;
   const int bt = blockIdx.z;
   const int col = blockIdx.y * 64 + threadIdx.x;
   const int n = row_n[R];
   if (col >= n) return;
-  const int64_t e0 = rp[R], e1 = rp[R + 1];
-  const double* __restrict__ pb = part + (size_t)bt * stride + col;
+  const int m = row_mol[R];
+  const double* __restrict__ pb = part + (size_t)bt * stride;
   double v[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  int64_t e = e0;
-  for (; e + 8 <= e1; e += 8) {
+  bool any = false;
+  for (int mask = 1; mask < 8; ++mask) {
+    const int64_t e0 = rp[R * 8 + mask], e1 = rp[R * 8 + mask + 1];
+    if (e0 == e1) continue;
+    const int j = cinv[cinv_off[m * 8 + mask] + col];
+    if (j < 0) continue;
+    any = true;
+    int64_t e = e0;
+    for (; e + 8 <= e1; e += 8) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] += pb[ent[e + u]];
+      for (int u = 0; u < 8; ++u) v[u] += pb[ent[e + u] + j];
+    }
+    for (int u = 0; e < e1; ++e, ++u) v[u] += pb[ent[e] + j];
   }
-  for (int u = 0; e < e1; ++e, ++u) v[u] += pb[ent[e]];
   const double sum = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
   double* dst = kp + (size_t)bt * ncc_tot + row_dst[R] + col;
   if (!atomic) {
     *dst += sum;
-  } else if (e1 > e0) {
+  } else if (any) {
     atomicAdd(dst, sum);  // another stream may fold into the same row
   }
 }
@@ -1229,6 +1256,29 @@ struct PairCand {
 // ket pattern); bra chunks for the LDS kernel when with_chunks.
 static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand, bool with_chunks, hipStream_t s) {
   const double cutoff = P.cutoff;
+  // slab column spaces per (molecule, ket l mask); segmented sets only (the
+  // contracted sets flush exchange rows by atomics)
+  std::vector<int> colmap, cinv;
+  std::vector<int64_t> cinv_off((size_t)P.nmol * 8, 0);
+  S.h_cspace.assign((size_t)P.nmol * 8, make_int2(0, 0));
+  if (!S.contracted)
+    for (int m = 0; m < P.nmol; ++m)
+      for (int mask = 1; mask < 8; ++mask) {
+        const int nc = P.mols[m].nc;
+        S.h_cspace[m * 8 + mask].x = (int)colmap.size();
+        cinv_off[m * 8 + mask] = (int64_t)cinv.size();
+        cinv.resize(cinv.size() + nc, -1);
+        int cnt = 0;
+        for (int g = P.mol_seg[m]; g < P.mol_seg[m + 1]; ++g) {
+          const Seg& sg = P.seg[g];
+          if (!((mask >> sg.l) & 1)) continue;
+          for (int c = 0; c < ncart_h(sg.l); ++c) {
+            colmap.push_back(sg.ao_cart + c);
+            cinv[cinv_off[m * 8 + mask] + sg.ao_cart + c] = cnt++;
+          }
+        }
+        S.h_cspace[m * 8 + mask].y = cnt;
+      }
   std::vector<int> keep;
   for (int i = 0; i < (int)cand.size(); ++i) {
     if (cand[i].pr.np == 0) continue;
@@ -1480,21 +1530,24 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
               for (BraChunk& c : qc.chunks) c.nket = 0;
             }
             // partial slabs (segmented sets): each chunk's exchange rows
-            // [nbra][rows][n_m], then its ket window [nket][wc*wd]
+            // [nbra][rows][columns of the class's ket mask], then its ket
+            // window [nket][wc*wd]; row-list keys are (global row, mask)
             const int pmode = P.options.partial_sums;
             if (pmode && !S.contracted && qc.nbra_max >= 1 && !qc.pb && !qc.pk) {
               qc.part = true;
               qc.part_beg = S.part_total;
+              const int kmask = (1 << qc.lc) | (1 << qc.ld);
               for (BraChunk& c : qc.chunks) {
                 const int m = P.seg[S.pairs[c.bra0].sa].mol;
-                const int nm = P.mols[m].nc;
+                const int nm = S.h_cspace[m * 8 + kmask].y;
                 c.poff = S.part_total;
                 if (pmode & 1)
                   for (int sl = 0; sl < c.nbra; ++sl) {
                     const Pair& pr = S.pairs[c.bra0 + sl];
                     for (int r = 0; r < rows; ++r) {
                       const int ao = r < wa ? P.seg[pr.sa].ao_cart + r : P.seg[pr.sb].ao_cart + (r - wa);
-                      kent.push_back({(int64_t)P.mols[m].aoc0 + ao, S.part_total + ((int64_t)sl * rows + r) * nm});
+                      kent.push_back({((int64_t)P.mols[m].aoc0 + ao) * 8 + kmask,
+                                      S.part_total + ((int64_t)sl * rows + r) * nm});
                       kent_cls.push_back((int)S.classes.size());
                     }
                   }
@@ -1556,21 +1609,29 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
     int64_t nrows = 0;
     for (int m = 0; m < P.nmol; ++m) nrows = std::max<int64_t>(nrows, (int64_t)P.mols[m].aoc0 + P.mols[m].nc);
     std::vector<int64_t> row_dst(nrows, 0);
-    std::vector<int> row_n(nrows, 0);
+    std::vector<int> row_n(nrows, 0), row_mol(nrows, 0);
     for (int m = 0; m < P.nmol; ++m)
       for (int a = 0; a < P.mols[m].nc; ++a) {
         row_dst[P.mols[m].aoc0 + a] = P.mols[m].cbase + (int64_t)a * P.mols[m].nc;
         row_n[P.mols[m].aoc0 + a] = P.mols[m].nc;
+        row_mol[P.mols[m].aoc0 + a] = m;
         S.nc_max = std::max(S.nc_max, P.mols[m].nc);
       }
     S.nrows = nrows;
     S.d_row_dst = dev_upload(row_dst, s);
     S.d_row_n = dev_upload(row_n, s);
-    csr(kent, nrows, S.d_krp, S.d_kent);
+    S.d_row_mol = dev_upload(row_mol, s);
+    csr(kent, nrows * 8, S.d_krp, S.d_kent);
     csr(jent, (int64_t)S.pairs.size(), S.d_jrp, S.d_jent);
     S.h_kent = std::move(kent);
     S.h_kent_cls = std::move(kent_cls);
   }
+  if (!colmap.empty()) {
+    S.d_colmap = dev_upload(colmap, s);
+    S.d_cinv = dev_upload(cinv, s);
+  }
+  S.d_cspace = dev_upload(S.h_cspace, s);
+  S.d_cinv_off = dev_upload(cinv_off, s);
   S.built = true;
 }
 
@@ -1583,7 +1644,8 @@ static void free_pair_set(PairSet& S) {
     f(S.d_prims);
   }
   f(S.d_coef); f(S.d_blk); f(S.d_pairs); f(S.d_pair_q);
-  f(S.d_krp); f(S.d_kent); f(S.d_jrp); f(S.d_jent); f(S.d_row_dst); f(S.d_row_n);
+  f(S.d_krp); f(S.d_kent); f(S.d_jrp); f(S.d_jent); f(S.d_row_dst); f(S.d_row_n); f(S.d_row_mol);
+  f(S.d_colmap); f(S.d_cspace); f(S.d_cinv_off); f(S.d_cinv);
   for (auto& c : S.classes) {
     f(c.d_quartets);
     f(c.d_tasks);
@@ -1778,11 +1840,12 @@ static void assign_lanes(PlanData& P) {
     for (int l = 0; l < ns; ++l)
       for (int ci : P.lane_classes[l]) lane_of[ci] = l;
     for (int l = 0; l < ns; ++l) {
-      std::vector<int64_t> rp(S.nrows + 1, 0), off;
+      const int64_t nkeys = S.nrows * 8;  // (row, ket mask)
+      std::vector<int64_t> rp(nkeys + 1, 0), off;
       for (size_t i = 0; i < S.h_kent.size(); ++i)
         if (lane_of[S.h_kent_cls[i]] == l) ++rp[S.h_kent[i].first + 1];
-      for (int64_t k = 0; k < S.nrows; ++k) rp[k + 1] += rp[k];
-      off.assign(std::max<int64_t>(1, rp[S.nrows]), 0);
+      for (int64_t k = 0; k < nkeys; ++k) rp[k + 1] += rp[k];
+      off.assign(std::max<int64_t>(1, rp[nkeys]), 0);
       std::vector<int64_t> fill(rp.begin(), rp.end() - 1);
       for (size_t i = 0; i < S.h_kent.size(); ++i)
         if (lane_of[S.h_kent_cls[i]] == l) off[fill[S.h_kent[i].first]++] = S.h_kent[i].second;
@@ -2260,6 +2323,9 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     b.part_mode = 0;
     b.part_stride = S.part_total;
     b.part = W.part;
+    b.colmap = S.d_colmap;
+    b.cspace = S.d_cspace;
+    b.cmask = (1 << qc.lc) | (1 << qc.ld);
     if (P.d_incore && qc.cache_doubles) {
       b.mode = 2;
       b.cache = P.d_incore + qc.cache_off;
@@ -2390,7 +2456,8 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     for (size_t l = 0; l < P.lane_classes.size(); ++l)
       hipLaunchKernelGGL(kpart_reduce_kernel, dim3((unsigned)S.nrows, (unsigned)((S.nc_max + 63) / 64),
                          (unsigned)nbd), dim3(64), 0, l == 0 ? s : P.side[l - 1], W.part, S.part_total,
-                         P.d_lane_krp[l], P.d_lane_kent[l], S.d_row_dst, S.d_row_n, P.ncc_tot, 1, W.kp);
+                         P.d_lane_krp[l], P.d_lane_kent[l], S.d_row_dst, S.d_row_n, S.d_row_mol, S.d_cinv_off,
+                         S.d_cinv, P.ncc_tot, 1, W.kp);
   if (fork) {
     for (size_t l = 0; l < P.side.size(); ++l) {
       HIP_OK(hipEventRecord(P.join_ev[l], P.side[l]));
@@ -2402,7 +2469,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     if (with_k && (psums & 1) && !lane_reduce)
       hipLaunchKernelGGL(kpart_reduce_kernel, dim3((unsigned)S.nrows, (unsigned)((S.nc_max + 63) / 64),
                          (unsigned)nbd_k), dim3(64), 0, s, W.part, S.part_total, S.d_krp, S.d_kent,
-                         S.d_row_dst, S.d_row_n, P.ncc_tot, 0, W.kp);
+                         S.d_row_dst, S.d_row_n, S.d_row_mol, S.d_cinv_off, S.d_cinv, P.ncc_tot, 0, W.kp);
     if (with_j && (psums & 2))
       hipLaunchKernelGGL(jpart_reduce_kernel, dim3((unsigned)S.pairs.size(), (unsigned)nb), dim3(64), 0, s,
                          W.part, S.part_total, S.d_jrp, S.d_jent, S.d_pairs, S.d_shells, P.d_molinfo,

@@ -1179,20 +1179,26 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
                                                                            dabs);
   }
   __syncthreads();
+  // the chunk's molecule and the columns its exchange terms can reach (the
+  // shells whose l is in the class's ket mask)
+  const int2 cspace = (a.part_mode & 3) ? a.cspace[a.shells[a.pairs[ch.bra0].sa].mol * 8 + a.cmask]
+                                        : make_int2(0, mi.n);
   if (a.part_mode & 1) {
-    // exchange rows to the chunk's partial slab with plain stores (every
-    // row, zeros included: the reduction reads the whole slab)
-    const int nm = mi.n;
+    // exchange rows to the chunk's partial slab with plain stores, over the
+    // mask's columns in compact order (zeros included: the reduction reads
+    // every entry)
+    const int nm = cspace.y;
+    const int* __restrict__ cmap = a.colmap + cspace.x;
     const int cnt = a.nbd * ch.nbra * ROWS * nm;
     double* __restrict__ dst = a.part + ch.poff;
     for (int i = threadIdx.x; i < cnt; i += blockDim.x) {
-      const int col = i % nm;
+      const int cc = i % nm;
       const int rs = i / nm;  // (bt, slot, row)
       const int r = rs % ROWS;
       const int slot = (rs / ROWS) % ch.nbra;
       const int bt = rs / (ROWS * ch.nbra);
-      dst[(size_t)bt * a.part_stride + ((size_t)slot * ROWS + r) * nm + col] =
-          krow[((bt * a.nbra_max + slot) * ROWS + r) * ns + col];
+      dst[(size_t)bt * a.part_stride + ((size_t)slot * ROWS + r) * nm + cc] =
+          krow[((bt * a.nbra_max + slot) * ROWS + r) * ns + cmap[cc]];
     }
   } else
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
@@ -1235,7 +1241,9 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   if (jket && (a.part_mode & 2)) {
     // ket window to the partial slab after the exchange rows
     const int nk = a.nbj * ch.nket * W::NKET;
-    double* __restrict__ dst = a.part + ch.poff + (size_t)ch.nbra * ROWS * mi.n;
+    // (the chunk's exchange rows precede its window whether this build
+    // writes them or not)
+    double* __restrict__ dst = a.part + ch.poff + (size_t)ch.nbra * ROWS * cspace.y;
     for (int i = threadIdx.x; i < nk; i += blockDim.x) {
       const int ew = i % (ch.nket * W::NKET);  // (window entry, element)
       const int bt = i / (ch.nket * W::NKET);
