@@ -158,6 +158,9 @@ struct BraArgs {
   const int* colmap;
   const int2* cspace;
   int cmask;
+  // LDS exchange rows over the same compact columns (segmented sets): per
+  // shell and mask its first compact column; NULL: rows over all n columns
+  const int* shell_cofs;
   // deterministic digestion: [nmol][2] (scale, 1/scale) of the build; NULL
   // for FP64 atomics (tab_lds 0, mixed 0 and part_mode 0 with it)
   const double* det_scale;

@@ -208,6 +208,7 @@ struct QuartetClass {
   int stage_cap = 0;        // staged bra primitive pairs per chunk (full-block segmented classes)
   int threads = 256;
   size_t per_slot = 0;      // LDS bytes of one bra slot for one density
+  int row_len = 0;          // LDS exchange-row length (columns of the ket mask, widest molecule)
   int64_t nq = 0, nitems = 0;
   double cost = 0.0;        // estimated work, for spreading launches over streams
   std::vector<SubTask> tasks;
@@ -275,6 +276,7 @@ struct PairSet {
   // Per (molecule, mask): (first entry in colmap, count); cinv maps a
   // molecule's column to its compact index in the mask's list, or -1.
   std::vector<int2> h_cspace;
+  int* d_shell_cofs = nullptr;    // per segmented shell and mask: first compact column (-1: l not in mask)
   int* d_colmap = nullptr;
   int2* d_cspace = nullptr;
   int64_t* d_cinv_off = nullptr;
@@ -1258,10 +1260,11 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
   const double cutoff = P.cutoff;
   // slab column spaces per (molecule, ket l mask); segmented sets only (the
   // contracted sets flush exchange rows by atomics)
-  std::vector<int> colmap, cinv;
+  std::vector<int> colmap, cinv, shell_cofs;
   std::vector<int64_t> cinv_off((size_t)P.nmol * 8, 0);
   S.h_cspace.assign((size_t)P.nmol * 8, make_int2(0, 0));
-  if (!S.contracted)
+  if (!S.contracted) {
+    shell_cofs.assign(P.seg.size() * 8, -1);
     for (int m = 0; m < P.nmol; ++m)
       for (int mask = 1; mask < 8; ++mask) {
         const int nc = P.mols[m].nc;
@@ -1272,6 +1275,7 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
         for (int g = P.mol_seg[m]; g < P.mol_seg[m + 1]; ++g) {
           const Seg& sg = P.seg[g];
           if (!((mask >> sg.l) & 1)) continue;
+          shell_cofs[(size_t)g * 8 + mask] = cnt;
           for (int c = 0; c < ncart_h(sg.l); ++c) {
             colmap.push_back(sg.ao_cart + c);
             cinv[cinv_off[m * 8 + mask] + sg.ao_cart + c] = cnt++;
@@ -1279,6 +1283,7 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
         }
         S.h_cspace[m * 8 + mask].y = cnt;
       }
+  }
   std::vector<int> keep;
   for (int i = 0; i < (int)cand.size(); ++i) {
     if (cand[i].pr.np == 0) continue;
@@ -1447,7 +1452,16 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
             if (stage)
               for (const Run& r : runs)
                 if (r.cls == cb && r.pat == pb) bra_np_max = std::max(bra_np_max, r.npmax);
-            const size_t per_slot = sizeof(double) * ((size_t)rows * P.nc_max + (size_t)wa * wb * (kJBraCopies + 1) +
+            // exchange rows in LDS span the columns the class's kets reach
+            // (segmented sets; contracted sets keep whole rows)
+            int row_len = P.nc_max;
+            if (!S.contracted) {
+              const int kmask = (1 << pl[ck][0]) | (1 << pl[ck][1]);
+              row_len = 0;
+              for (int m = 0; m < P.nmol; ++m) row_len = std::max(row_len, S.h_cspace[m * 8 + kmask].y);
+            }
+            qc.row_len = row_len;
+            const size_t per_slot = sizeof(double) * ((size_t)rows * row_len + (size_t)wa * wb * (kJBraCopies + 1) +
                                                       (size_t)rec * bra_np_max + 1);
             const int nr = (qc.la + qc.lb + qc.lc + qc.ld) / 2 + 1;
             size_t pack = std::min<size_t>((size_t)tu.lds_pack, P.lds_budget);
@@ -1511,6 +1525,14 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
                 qc.chunks.push_back(c);
                 b = e;
               }
+            }
+            // LDS is laid out for nbra_max slots in every workgroup: size it
+            // for the widest chunk formed (the item limit often closes
+            // chunks first), not for the cap
+            if (qc.nbra_max >= 1 && !qc.chunks.empty()) {
+              int widest = 1;
+              for (const BraChunk& c : qc.chunks) widest = std::max(widest, (int)c.nbra);
+              qc.nbra_max = std::min(qc.nbra_max, widest);
             }
             if (stage)
               for (const BraChunk& c : qc.chunks) {
@@ -1629,6 +1651,7 @@ static void build_pair_set(PlanData& P, PairSet& S, std::vector<PairCand>& cand,
   if (!colmap.empty()) {
     S.d_colmap = dev_upload(colmap, s);
     S.d_cinv = dev_upload(cinv, s);
+    S.d_shell_cofs = dev_upload(shell_cofs, s);
   }
   S.d_cspace = dev_upload(S.h_cspace, s);
   S.d_cinv_off = dev_upload(cinv_off, s);
@@ -1645,7 +1668,7 @@ static void free_pair_set(PairSet& S) {
   }
   f(S.d_coef); f(S.d_blk); f(S.d_pairs); f(S.d_pair_q);
   f(S.d_krp); f(S.d_kent); f(S.d_jrp); f(S.d_jent); f(S.d_row_dst); f(S.d_row_n); f(S.d_row_mol);
-  f(S.d_colmap); f(S.d_cspace); f(S.d_cinv_off); f(S.d_cinv);
+  f(S.d_colmap); f(S.d_cspace); f(S.d_cinv_off); f(S.d_cinv); f(S.d_shell_cofs);
   for (auto& c : S.classes) {
     f(c.d_quartets);
     f(c.d_tasks);
@@ -2339,7 +2362,8 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     b.nchunks = (int)qc.chunks.size();
     b.nbra_max = qc.nbra_max;
     b.mols = P.d_molinfo;
-    b.n = P.nc_max;
+    b.n = qc.row_len ? qc.row_len : P.nc_max;
+    b.shell_cofs = S.d_shell_cofs;
     b.with_j = with_j;
     b.with_k = with_k;
     b.mixed = mixed ? 1 : 0;
@@ -2353,7 +2377,7 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     const int stage_cap = b.mode == 2 ? 0 : qc.stage_cap;
     const int stage_rec = qc.pb ? kStagedPrimCoef : kStagedPrim;
     auto layout = [&](int nbd_l, int nbj_l, size_t cap, size_t& lds, int& nket_max, int& tab_lds) {
-      lds = sizeof(double) * ((size_t)nbd_l * qc.nbra_max * (wa + wb) * P.nc_max +
+      lds = sizeof(double) * ((size_t)nbd_l * qc.nbra_max * (wa + wb) * b.n +
                               (size_t)nbj_l * qc.nbra_max * wa * wb * kJBraCopies);
       nket_max = with_j ? qc.nket_max : 0;
       const size_t jket_bytes = sizeof(double) * (size_t)nbj_l * nket_max * wc * wd;
@@ -2363,7 +2387,9 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
       // (the kernel places the staging area 16-byte aligned after the slabs)
       if (stage_cap) lds = ((lds + 15) & ~(size_t)15) + sizeof(double) * ((size_t)stage_rec * stage_cap + (qc.nbra_max + 2) / 2);
       tab_lds = (b.mode == 2 || det) ? 0 : qc.tab_lds;  // stored blocks need no Rys table
-      if (tab_lds && lds + qc.tab_bytes <= cap) lds += qc.tab_bytes; else tab_lds = 0;
+      // (the kernel places the table 16-byte aligned after the staging area)
+      const size_t tab_at = (lds + 15) & ~(size_t)15;
+      if (tab_lds && tab_at + qc.tab_bytes <= cap) lds = tab_at + qc.tab_bytes; else tab_lds = 0;
     };
     const int nbd_eff = with_k ? nbd : nb;
     size_t lds;

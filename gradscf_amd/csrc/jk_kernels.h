@@ -768,6 +768,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
     }
     }
     const unsigned oc = sC.ao, od = sD.ao;
+    // LDS exchange-row columns of C and D (compact over the class's ket mask)
+    const unsigned kc0 = a.shell_cofs ? (unsigned)a.shell_cofs[kp.sa * 8 + a.cmask] : oc;
+    const unsigned kd0 = a.shell_cofs ? (unsigned)a.shell_cofs[kp.sb * 8 + a.cmask] : od;
     const unsigned ra = (unsigned)oa * n, rb = (unsigned)ob * n, rc = oc * n, rd = od * n;
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
@@ -839,7 +842,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             for (int ib = 0; ib < WB; ++ib)
 #pragma unroll
               for (int id = 0; id < WD; ++id) sk += (double)acc[ia][ib][ic][id] * dbd[ib][id];
-            acc_add<DET>(kr + ia * ns + oc + ic, f * sk);
+            acc_add<DET>(kr + ia * ns + kc0 + ic, f * sk);
           }
 #pragma unroll
           for (int id = 0; id < WD; ++id) {
@@ -848,7 +851,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             for (int ib = 0; ib < WB; ++ib)
 #pragma unroll
               for (int ic = 0; ic < WC; ++ic) sk += (double)acc[ia][ib][ic][id] * dbc[ib][ic];
-            acc_add<DET>(kr + ia * ns + od + id, f * sk);
+            acc_add<DET>(kr + ia * ns + kd0 + id, f * sk);
           }
         }
 #pragma unroll
@@ -860,7 +863,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
               for (int id = 0; id < WD; ++id) sk += (double)acc[ia][ib][ic][id] * dad[ia][id];
-            acc_add<DET>(kr + (WA + ib) * ns + oc + ic, f * sk);
+            acc_add<DET>(kr + (WA + ib) * ns + kc0 + ic, f * sk);
           }
 #pragma unroll
           for (int id = 0; id < WD; ++id) {
@@ -869,7 +872,7 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             for (int ia = 0; ia < WA; ++ia)
 #pragma unroll
               for (int ic = 0; ic < WC; ++ic) sk += (double)acc[ia][ib][ic][id] * dac[ia][ic];
-            acc_add<DET>(kr + (WA + ib) * ns + od + id, f * sk);
+            acc_add<DET>(kr + (WA + ib) * ns + kd0 + id, f * sk);
           }
         }
       }
@@ -936,6 +939,9 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
     }
     }
     const unsigned c = sC.ao + icw, d = sD.ao + idw;
+    // LDS exchange-row columns of this lane's C and D components
+    const unsigned xc = (a.shell_cofs ? (unsigned)a.shell_cofs[kp.sa * 8 + a.cmask] : sC.ao) + icw;
+    const unsigned xd = (a.shell_cofs ? (unsigned)a.shell_cofs[kp.sb * 8 + a.cmask] : sD.ao) + idw;
     const unsigned ra = (unsigned)oa * n, rb = (unsigned)ob * n;
     for (int bt = 0; bt < a.nbd; ++bt) {
       const double* __restrict__ Dm = a.dens + (size_t)bt * a.bstride + mi.base;
@@ -973,8 +979,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             sc += (double)acc[ia][ib] * db_d[ib];
             sd += (double)acc[ia][ib] * db_c[ib];
           }
-          acc_add<DET>(kr + ia * ns + c, f * sc);
-          acc_add<DET>(kr + ia * ns + d, f * sd);
+          acc_add<DET>(kr + ia * ns + xc, f * sc);
+          acc_add<DET>(kr + ia * ns + xd, f * sd);
         }
         double da_d[WA], da_c[WA];
 #pragma unroll
@@ -990,8 +996,8 @@ __device__ __forceinline__ void bra_item(const BraArgs& a, const MolInfo& mi, co
             sc += (double)acc[ia][ib] * da_d[ia];
             sd += (double)acc[ia][ib] * da_c[ia];
           }
-          acc_add<DET>(kr + (WA + ib) * ns + c, f * sc);
-          acc_add<DET>(kr + (WA + ib) * ns + d, f * sd);
+          acc_add<DET>(kr + (WA + ib) * ns + xc, f * sc);
+          acc_add<DET>(kr + (WA + ib) * ns + xd, f * sd);
         }
       }
     }
@@ -1181,8 +1187,9 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
   __syncthreads();
   // the chunk's molecule and the columns its exchange terms can reach (the
   // shells whose l is in the class's ket mask)
-  const int2 cspace = (a.part_mode & 3) ? a.cspace[a.shells[a.pairs[ch.bra0].sa].mol * 8 + a.cmask]
-                                        : make_int2(0, mi.n);
+  const bool compact = a.shell_cofs != nullptr;
+  const int2 cspace = ((a.part_mode & 3) || compact) ? a.cspace[a.shells[a.pairs[ch.bra0].sa].mol * 8 + a.cmask]
+                                                     : make_int2(0, mi.n);
   if (a.part_mode & 1) {
     // exchange rows to the chunk's partial slab with plain stores, over the
     // mask's columns in compact order (zeros included: the reduction reads
@@ -1198,17 +1205,21 @@ __global__ __launch_bounds__(512) void jk_bra_kernel(BraArgs a) {
       const int slot = (rs / ROWS) % ch.nbra;
       const int bt = rs / (ROWS * ch.nbra);
       dst[(size_t)bt * a.part_stride + ((size_t)slot * ROWS + r) * nm + cc] =
-          krow[((bt * a.nbra_max + slot) * ROWS + r) * ns + cmap[cc]];
+          krow[((bt * a.nbra_max + slot) * ROWS + r) * ns + (compact ? cc : cmap[cc])];
     }
   } else
   // flush: exchange rows and bra J blocks of every slot (skip untouched zeros)
   for (int i = threadIdx.x; i < krow_size; i += blockDim.x) {
     if (acc_zero<DET>(krow + i)) continue;
-    const int col = i % ns;
+    int col = i % ns;
     const int r = (i / ns) % ROWS;
     const int slot = (i / (ns * ROWS)) % a.nbra_max;
     const int bt = i / (ns * ROWS * a.nbra_max);
     if (slot >= ch.nbra) continue;
+    if (compact) {
+      if (col >= cspace.y) continue;
+      col = a.colmap[cspace.x + col];
+    }
     const Pair bp = a.pairs[ch.bra0 + slot];
     const MolInfo mi = a.mols[a.shells[bp.sa].mol];
     const int row = r < W::WA ? a.shells[bp.sa].ao + r : a.shells[bp.sb].ao + (r - W::WA);
