@@ -142,30 +142,40 @@ __device__ __forceinline__ void staged_quartet(const double* __restrict__ r, con
   const double PA[3] = {r[1], r[2], r[3]};
   const double PQ[3] = {PA[0] - QA[0], PA[1] - QA[1], PA[2] - QA[2]};
   const double pq = r[0] + Q.p;
-  const double rs = rsqrt_pos(pq);
-  const double ipq = rs * rs;
-  const double T = r[0] * Q.p * ipq * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-  const double pref = r[4] * Q.e * rs;
+  const double d2 = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
   if constexpr (K::NR == 1 && LB == 0 && std::is_same<R, double>::value) {
-    double F0, F1 = 0.0;
-    if (T < BOYS_TMAX) {
-      double F[LA + 1];
-      boys_taylor<LA + 1>(T, F);
-      F0 = F[0];
-      if constexpr (LA == 1) F1 = F[1];
+    // X = p q |PQ|^2 = (p + q) T.  With pref = e_ab e_cd / sqrt(p + q):
+    //   T >= TMAX:  pref F0 = (sqrt(pi)/2) e_ab e_cd / sqrt(X),
+    //               pref (q/(p+q)) F1 = pref F0 q / (2 X)      (F1 = F0 / (2T)),
+    //   T <  TMAX:  T = X / (p + q), F0 and F1 from the grid.
+    // The branch test needs no division (X >= TMAX (p + q)), and both
+    // branches take one reciprocal square root, of X or of p + q.
+    const double X = r[0] * Q.p * d2;
+    const bool asym = X >= BOYS_TMAX * pq;
+    const double y = rsqrt_pos(asym ? X : pq);
+    double af, bf = 0.0;
+    if (asym) {
+      af = 0.88622692545275801365 * r[4] * Q.e * y;  // sqrt(pi) / 2
+      if constexpr (LA == 1) bf = 0.5 * af * Q.p * (y * y);
     } else {
-      const double sq = rsqrt_pos(T);
-      F0 = 0.88622692545275801365 * sq;  // sqrt(pi) / 2 / sqrt(T)
-      F1 = 0.5 * F0 * sq * sq;
+      const double ipq = y * y;
+      double F[LA + 1];
+      boys_taylor<LA + 1>(X * ipq, F);
+      const double pref = r[4] * Q.e * y;
+      af = pref * F[0];
+      if constexpr (LA == 1) bf = pref * Q.p * ipq * F[1];
     }
     if constexpr (LA == 0) {
-      blk[0][0][0][0] += pref * F0;
+      blk[0][0][0][0] += af;
     } else {
-      const double af = pref * F0, bf = pref * Q.p * ipq * F1;
 #pragma unroll
       for (int x = 0; x < 3; ++x) blk[x][0][0][0] += PA[x] * af - PQ[x] * bf;
     }
   } else {
+    const double rs = rsqrt_pos(pq);
+    const double ipq = rs * rs;
+    const double T = r[0] * Q.p * ipq * d2;
+    const double pref = r[4] * Q.e * rs;
     PrimQ<K::NR> q;
     q.ipq = ipq;
     q.ip = r[5];
