@@ -10,10 +10,13 @@ schedule:
     plan: one launch per quartet class for the whole batch), optionally
     replayed from a captured HIP graph;
   * every molecule's matrices sit in one (nmol, N, N) batch padded to the
-    largest AO count N (identity overlap and a lifted diagonal in the
-    padding, masks for occupied and virtual orbitals), so diagonalisation,
-    DIIS extrapolation and convergence tests run as single batched FP64
-    tensor operations over all molecules;
+    largest AO count N (identity overlap in the padding, masks for occupied
+    and virtual orbitals), so DIIS extrapolation and convergence tests run as
+    single batched FP64 tensor operations over all molecules;
+  * the diagonalisations solve each molecule's own block, molecules of one
+    AO count together, the counts concurrently on several streams
+    (linalg.BatchedEigh: rocSOLVER's batched solver, no host
+    synchronisation), and skip molecules that have converged;
   * a molecule that has converged is frozen (its density change is zero from
     then on), so its energy is the one an independent run would stop at.
 
@@ -29,7 +32,7 @@ import numpy as np
 import torch
 
 from .jk import DirectJKPlan
-from .linalg import eigh
+from .linalg import BatchedEigh
 
 DIIS_SPACE = 8
 
@@ -73,20 +76,21 @@ class _Padded:
         packed[self.src] = mats.reshape(-1)[self.dst]
 
 
-def _orthogonalizer(s, eps=1e-10):
-    w, v = eigh(s)
+def _orthogonalizer(s, eig, eps=1e-10):
+    """Loewdin S^{-1/2} of every molecule's block (the identity in the
+    padding: unit eigenvalues there)."""
+    w, v = eig(s, e_out=torch.ones((eig.k, eig.N), dtype=s.dtype, device=s.device))
     return (v * torch.clamp(w, min=eps).rsqrt()[:, None, :]) @ v.transpose(1, 2)
 
 
-def _diagonalize(f, x, pad):
-    """Eigenpairs of x^T f x; pad (k, N) marks padding rows, whose diagonal is
-    lifted above every eigenvalue so that each molecule's own eigenpairs come
-    first (the off-diagonal padding blocks are zero)."""
+def _diagonalize(f, x, eig, c_buf=None, active_h=None):
+    """Eigenvectors of x^T f x per molecule block (each molecule's in its
+    leading columns, the identity in the padding); molecules outside active_h
+    keep the columns c_buf held."""
     fo = x.transpose(1, 2) @ f @ x
     fo = 0.5 * (fo + fo.transpose(1, 2))
-    fo = fo + torch.diag_embed(pad * (fo.abs().sum(dim=-1).amax() + 1.0))
-    e, c = eigh(fo)
-    return e, x @ c
+    _, c = eig(fo, c_out=c_buf, active_h=active_h)
+    return c, x @ c
 
 
 def _diis_solve(fock_hist, gram, count):
@@ -193,8 +197,9 @@ def _rhf_batch(plan, nelectrons, enucs, conv_tol, conv_tol_density, conv_tol_gra
             out = plan.jk(packed[None], symmetric=True, validate=False, mixed_threshold=mixed_threshold)
             return out[0, 0], out[1, 0]
 
-    X = _orthogonalizer(S)
-    _, C0 = _diagonalize(H, X, pad)
+    eig = BatchedEigh(plan.mol_n, dev)
+    X = _orthogonalizer(S, eig)
+    c_buf, C0 = _diagonalize(H, X, eig)
     Co = C0 * occ[:, None, :]
     D = 2.0 * Co @ Co.transpose(1, 2)
     d_packed = torch.zeros(plan.nn_total, dtype=torch.float64, device=dev)
@@ -209,6 +214,7 @@ def _rhf_batch(plan, nelectrons, enucs, conv_tol, conv_tol_density, conv_tol_gra
     J, K = B.gather(j_packed), B.gather(k_packed)
     F = H + J - 0.5 * K
     head = count = 0
+    active_h = None
     iteration_s = []
     n_jk = 1
     torch.cuda.synchronize()
@@ -224,7 +230,7 @@ def _rhf_batch(plan, nelectrons, enucs, conv_tol, conv_tol_density, conv_tol_gra
             gram[:, :, head] = row
         if cycle >= 1 and min(count + 1, DIIS_SPACE) >= 2:
             f_eff = _diis_solve(fh, gram, min(count + 1, DIIS_SPACE))
-        _, C = _diagonalize(f_eff, X, pad)
+        c_buf, C = _diagonalize(f_eff, X, eig, c_buf, active_h)
         Co = C * occ[:, None, :]
         D_new = 2.0 * Co @ Co.transpose(1, 2)
         D_new = torch.where(active[:, None, None], D_new, D)
@@ -249,13 +255,15 @@ def _rhf_batch(plan, nelectrons, enucs, conv_tol, conv_tol_density, conv_tol_gra
         was = active
         active = active & ~done
         # one host synchronisation per iteration: which molecules are still running
-        was_h = was.cpu().numpy()
+        flags = torch.stack((was, active)).cpu().numpy()
+        was_h, active_h = flags[0], flags[1]
         cycles[was_h] = cycle + 1
         iteration_s.append(time.perf_counter() - ti)
         if not bool(active.any()):
             break
     torch.cuda.synchronize()
     t_end = time.perf_counter()
+    eig.check()
     energies = energy.cpu().numpy()
     converged = ~active.cpu().numpy()
     d_final = torch.zeros_like(d_packed)

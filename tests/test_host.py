@@ -432,13 +432,15 @@ def test_basis_gradient_chain_rule_matches_fd():
 
 def test_padded_batch_algebra_matches_per_molecule():
     """The batched SCF's padded layout: gather/scatter round-trip the packed
-    layout, and the Loewdin orthogonaliser and diagonalisation of padded
-    matrices (identity overlap, lifted diagonal in the padding) give every
-    molecule its own eigenvalues and occupied projector."""
+    layout, and the Loewdin orthogonaliser and the per-block diagonalisation
+    (molecules grouped by order, identity in the padding) give every
+    molecule its own eigenvalues and occupied projector, also when only some
+    molecules are solved."""
     import torch
     from gradscf_amd.batch_scf import _Padded, _diagonalize, _orthogonalizer
+    from gradscf_amd.linalg import BatchedEigh
     rng = np.random.default_rng(7)
-    sizes, nocc = [3, 7, 5], [1, 3, 2]
+    sizes, nocc = [3, 7, 5, 7], [1, 3, 2, 2]
     offsets = np.concatenate([[0], np.cumsum(np.square(sizes))[:-1]])
     mats_f, mats_s = [], []
     for n in sizes:
@@ -455,8 +457,17 @@ def test_padded_batch_algebra_matches_per_molecule():
     pad = 1.0 - B.valid
     S = B.gather(packed_s) + torch.diag_embed(pad)
     F = B.gather(packed_f)
-    X = _orthogonalizer(S)
-    e, C = _diagonalize(F, X, pad)
+    eig = BatchedEigh(sizes, "cpu")
+    X = _orthogonalizer(S, eig)
+    c_buf, C = _diagonalize(F, X, eig)
+    fo = X.transpose(1, 2) @ F @ X
+    e, _ = eig(0.5 * (fo + fo.transpose(1, 2)))
+    # molecules outside the active set keep their columns
+    stale = c_buf.clone()
+    active = np.array([True, False, True, False])
+    c2, C2 = _diagonalize(2.0 * F, X, eig, c_buf.clone(), active)
+    assert torch.equal(c2[1], stale[1]) and torch.equal(c2[3], stale[3])
+    assert torch.allclose(C2[0] @ C2[0].T, C[0] @ C[0].T, atol=1e-10)
     for m, (n, no) in enumerate(zip(sizes, nocc)):
         s = torch.as_tensor(mats_s[m])
         w, v = torch.linalg.eigh(s)
