@@ -310,7 +310,8 @@ def batch_section(thr: float, reps: int = 5, autotune: bool = True, retune: bool
                                  search={"rys_lds": (0, 1), "ket_lds": (0, 1), "threads": (256, 512),
                                          ("chunk_items", "target_prim"): tuple(
                                              (c, t) for c in (2048, 4096, 8192) for t in (24, 48, 96)),
-                                         "bras_max": (8, 16, 32)},
+                                         "bras_max": (2, 4, 8, 16, 32),
+                                         "lds_pack": (8192, 16384, 32768, 49152, 65536)},
                                  log=log)
             tuning = {"source": "autotuned in this run", "classes": result}
             if save_dir:
