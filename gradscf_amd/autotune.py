@@ -133,21 +133,28 @@ def _class_times(make_plan, run, reps):
     return times
 
 
-def autotune(make_plan, run, *, reps: int = 3, search=None, passes: int = 2, log=None) -> dict:
+def autotune(make_plan, run, *, reps: int = 3, search=None, passes: int = 2, log=None,
+             confirm_tol: float = 0.15) -> dict:
     """Choose a schedule per class.
 
     make_plan(): builds a DirectJKPlan under the currently installed tuning.
     run(plan): one J/K build of the workload on that plan.
     passes: sweeps of the coordinate descent (the second revisits early
     choices, e.g. the root evaluation, under the later ones).
-    Returns {class name: {"schedule": dict, "ms": best avg ms, "default_ms": ms}}
-    and leaves the chosen schedules installed."""
+    A confirmation pass then times every class under the final table: a
+    class that runs more than `confirm_tol` slower than it did when its
+    schedule was chosen (a choice measured beside other classes' trial
+    schedules) is timed again with each schedule it held earlier, newest
+    first, and keeps the fastest it confirmed.
+    Returns {class name: {"schedule": dict, "ms": confirmed avg ms,
+    "default_ms": ms}} and leaves the chosen schedules installed."""
     search = SEARCH if search is None else search
     reset()
     base = Schedule()
     best_t = _class_times(make_plan, run, reps)
     default_t = dict(best_t)
     best_s = {cls: base for cls in best_t}
+    earlier = {cls: [] for cls in best_t}  # schedules a class held before its current one
     for param, values in [kv for _ in range(passes) for kv in search.items()]:
         names = param if isinstance(param, tuple) else (param,)
         current = {cls: best_s[cls] for cls in best_t}
@@ -162,14 +169,37 @@ def autotune(make_plan, run, *, reps: int = 3, search=None, passes: int = 2, log
             for cls, ms in t.items():
                 if cls in best_t and ms < best_t[cls]:
                     best_t[cls] = ms
-                    best_s[cls] = Schedule(**{**asdict(current[cls]), **upd})
+                    cand = Schedule(**{**asdict(current[cls]), **upd})
+                    if cand != best_s[cls] and best_s[cls] not in earlier[cls]:
+                        earlier[cls].append(best_s[cls])
+                    best_s[cls] = cand
             if log:
                 log(f"autotune {param}={v}: {sum(t.values()):.3f} ms over classes")
         for cls in best_s:
             set_schedule(best_s[cls], cls)
     for cls in best_s:
         set_schedule(best_s[cls], cls)
-    return {class_name(cls): {"schedule": asdict(best_s[cls]), "ms": best_t[cls], "default_ms": default_t[cls]}
+    final_t = _class_times(make_plan, run, reps)
+    for cls in best_s:
+        final_t.setdefault(cls, best_t[cls])
+    suspect = [cls for cls in best_s if final_t[cls] > (1.0 + confirm_tol) * best_t[cls]]
+    for depth in range(max((len(earlier[c]) for c in suspect), default=0)):
+        trial = {cls: earlier[cls][-1 - depth] for cls in suspect if depth < len(earlier[cls])}
+        if not trial:
+            break
+        for cls, sch in trial.items():
+            set_schedule(sch, cls)
+        t = _class_times(make_plan, run, reps)
+        for cls, sch in trial.items():
+            if t.get(cls, float("inf")) < final_t[cls]:
+                final_t[cls] = t[cls]
+                best_s[cls] = sch
+            set_schedule(best_s[cls], cls)
+        if log:
+            log(f"autotune confirmation: {len(trial)} classes retried earlier schedules")
+    for cls in best_s:
+        set_schedule(best_s[cls], cls)
+    return {class_name(cls): {"schedule": asdict(best_s[cls]), "ms": final_t[cls], "default_ms": default_t[cls]}
             for cls in best_s}
 
 

@@ -364,6 +364,34 @@ def test_tuning_table_api_without_gpu():
     assert at.get_schedule((2, 1, 1, 0)) == at.Schedule()
 
 
+def test_autotune_confirmation_pass_reverts_interaction_choices(monkeypatch):
+    """A class whose schedule won only beside other classes' trial schedules
+    (here: class A is fast at 512 threads only while B also runs 512, and B
+    keeps 256) is timed again under the final table and falls back to the
+    schedule it held before; a class whose choice holds keeps it."""
+    if not (ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so").is_file():
+        pytest.skip("engine not built")
+    from gradscf_amd import autotune as at
+    A, B, C = (1, 0, 0, 0), (1, 0, 1, 0), (0, 0, 0, 0)
+
+    def fake_times(make_plan, run, reps):
+        ta, tb, tc = (at.get_schedule(c).threads for c in (A, B, C))
+        a = (0.8 if tb == 512 else 1.5) if ta == 512 else 1.0
+        b = 1.2 if tb == 512 else 1.0
+        c = 0.7 if tc == 512 else 1.0
+        return {A: a, B: b, C: c}
+
+    monkeypatch.setattr(at, "_class_times", fake_times)
+    try:
+        res = at.autotune(None, None, search={"threads": (256, 512)}, passes=1)
+        assert res["psss"]["schedule"]["threads"] == 256 and res["psss"]["ms"] == 1.0
+        assert res["psps"]["schedule"]["threads"] == 256
+        assert res["ssss"]["schedule"]["threads"] == 512 and res["ssss"]["ms"] == 0.7
+        assert at.get_schedule(A).threads == 256 and at.get_schedule(C).threads == 512
+    finally:
+        at.reset()
+
+
 def test_engine_options_api_without_gpu():
     if not (ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so").is_file():
         pytest.skip("engine not built")
