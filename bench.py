@@ -725,9 +725,11 @@ def main() -> int:
             tuning = {"source": os.path.relpath(table_path, ROOT), "table": json.load(f)}
     elif not args.no_autotune:
         t0 = time.perf_counter()
+        # a retune starts from the committed table (it may only improve on it)
+        start = at.start_from_table(table_path) if os.path.isfile(table_path) else None
         result, opts = at.autotune_with_options(lambda: DirectJKPlan(atm, bas, env, cart=False, cutoff=cutoff),
                                                 lambda p: p.jk(dens[None], symmetric=True, validate=False),
-                                                reps=3, log=log)
+                                                reps=3, log=log, start=start)
         tuning = {"source": "autotuned in this run", "classes": result, "options": opts}
         if args.save_tuning:
             at.save_table(os.path.join(args.save_tuning, "water16_ccpvdz_sph.json"), result,

@@ -134,7 +134,7 @@ def _class_times(make_plan, run, reps):
 
 
 def autotune(make_plan, run, *, reps: int = 3, search=None, passes: int = 2, log=None,
-             confirm_tol: float = 0.15) -> dict:
+             confirm_tol: float = 0.15, start=None) -> dict:
     """Choose a schedule per class.
 
     make_plan(): builds a DirectJKPlan under the currently installed tuning.
@@ -146,14 +146,21 @@ def autotune(make_plan, run, *, reps: int = 3, search=None, passes: int = 2, log
     schedule was chosen (a choice measured beside other classes' trial
     schedules) is timed again with each schedule it held earlier, newest
     first, and keeps the fastest it confirmed.
+    start: optional {class: Schedule} (e.g. a committed table, see
+    start_from_table) to descend from instead of the default schedule; the
+    descent only accepts improvements, so a retune from a table never ends
+    slower than the table (coordinate descent from the defaults can settle in
+    a worse local optimum than a table found along another path).
     Returns {class name: {"schedule": dict, "ms": confirmed avg ms,
-    "default_ms": ms}} and leaves the chosen schedules installed."""
+    "default_ms": ms at the start}} and leaves the chosen schedules installed."""
     search = SEARCH if search is None else search
     reset()
     base = Schedule()
+    for cls, sch in (start or {}).items():
+        set_schedule(sch, cls)
     best_t = _class_times(make_plan, run, reps)
     default_t = dict(best_t)
-    best_s = {cls: base for cls in best_t}
+    best_s = {cls: (start or {}).get(cls, base) for cls in best_t}
     earlier = {cls: [] for cls in best_t}  # schedules a class held before its current one
     for param, values in [kv for _ in range(passes) for kv in search.items()]:
         names = param if isinstance(param, tuple) else (param,)
@@ -203,7 +210,17 @@ def autotune(make_plan, run, *, reps: int = 3, search=None, passes: int = 2, log
             for cls in best_s}
 
 
-def autotune_with_options(make_plan, run, *, reps: int = 3, log=None):
+def start_from_table(path) -> dict:
+    """{class: Schedule} of a table written by save_table (the start point
+    of a retune that may only improve on it)."""
+    import json
+    with open(path) as f:
+        table = json.load(f)
+    return {tuple(LETTERS.index(ch) for ch in name): Schedule(**sch)
+            for name, sch in table.items() if not name.startswith("_")}
+
+
+def autotune_with_options(make_plan, run, *, reps: int = 3, log=None, start=None):
     """Tune the per-class schedules once with native general contraction and
     once without, keep whichever gives the faster whole build, and install it.
     Returns (per-class result as from autotune, {"general_contraction": ...,
@@ -211,7 +228,7 @@ def autotune_with_options(make_plan, run, *, reps: int = 3, log=None):
     runs = {}
     for gc in (1, 0):
         set_options(general_contraction=gc)
-        res = autotune(make_plan, run, reps=reps, log=log)
+        res = autotune(make_plan, run, reps=reps, log=log, start=start)
         ms = build_ms(make_plan, run)
         runs[gc] = (ms, res)
         if log:

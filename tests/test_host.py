@@ -392,6 +392,37 @@ def test_autotune_confirmation_pass_reverts_interaction_choices(monkeypatch):
         at.reset()
 
 
+def test_autotune_from_a_table_never_ends_slower(monkeypatch, tmp_path):
+    """Coordinate descent from the defaults settles in a local optimum here
+    (threads 512, bras_max 8: 0.85) worse than a table's schedule (threads
+    256, bras_max 32: 0.5); started from that table (start_from_table) the
+    descent keeps it."""
+    if not (ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so").is_file():
+        pytest.skip("engine not built")
+    from gradscf_amd import autotune as at
+    A = (1, 0, 1, 0)
+    landscape = {(256, 16): 1.0, (512, 16): 0.9, (512, 8): 0.85, (512, 32): 0.95, (256, 8): 0.7,
+                 (256, 32): 0.5}
+
+    def fake_times(make_plan, run, reps):
+        sch = at.get_schedule(A)
+        return {A: landscape[(sch.threads, sch.bras_max)]}
+
+    monkeypatch.setattr(at, "_class_times", fake_times)
+    search = {"threads": (256, 512), "bras_max": (8, 16, 32)}
+    try:
+        free = at.autotune(None, None, search=search, passes=1)
+        assert free["psps"]["ms"] == 0.85
+        at.save_table(tmp_path / "t.json", {"psps": {"schedule": at.asdict(at.Schedule(threads=256, bras_max=32))}})
+        start = at.start_from_table(tmp_path / "t.json")
+        assert start == {A: at.Schedule(threads=256, bras_max=32)}
+        kept = at.autotune(None, None, search=search, passes=1, start=start)
+        assert kept["psps"]["ms"] == 0.5 and kept["psps"]["schedule"]["bras_max"] == 32
+        assert kept["psps"]["default_ms"] == 0.5
+    finally:
+        at.reset()
+
+
 def test_engine_options_api_without_gpu():
     if not (ROOT / "gradscf_amd" / "lib" / "libgradscf_jk.so").is_file():
         pytest.skip("engine not built")
