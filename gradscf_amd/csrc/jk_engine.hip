@@ -2395,10 +2395,23 @@ static void jk_build(PlanData& P, const double* density, int nb, double* out, in
     size_t lds;
     int nket_max, tab_lds;
     layout(nbd_eff, nb, P.lds_budget, lds, nket_max, tab_lds);
-    // every density in one launch within the packing budget; otherwise one
-    // launch per density (up to the device's LDS); otherwise, for segmented
-    // classes only, the global-atomic kernels
-    const bool bra_path = qc.d_chunks && lds <= P.lds_budget;
+    // every density in one launch within the packing budget, else within the
+    // device's LDS; otherwise one launch per density (up to the device's
+    // LDS); otherwise, for segmented classes only, the global-atomic kernels
+    bool bra_path = qc.d_chunks && lds <= P.lds_budget;
+    if (!bra_path && qc.d_chunks && nbd_eff > 1) {
+      // several densities: one launch with up to the device's LDS (every
+      // quartet evaluated once, at lower occupancy) before one per density
+      size_t lds_all;
+      int nket_all, tab_all;
+      layout(nbd_eff, nb, P.lds_max, lds_all, nket_all, tab_all);
+      if (lds_all <= P.lds_max) {
+        bra_path = true;
+        lds = lds_all;
+        nket_max = nket_all;
+        tab_lds = tab_all;
+      }
+    }
     size_t lds1 = 0;
     int nket1 = 0, tab1 = 0;
     if (!bra_path && qc.d_chunks) layout(1, 1, P.lds_max, lds1, nket1, tab1);
