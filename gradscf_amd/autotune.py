@@ -38,9 +38,10 @@ SEARCH = {
     "rys_lds": (0, 1),
     "ket_lds": (0, 1),
     "threads": (256, 512),
-    ("chunk_items", "target_prim"): tuple((c, t) for c in (1024, 2048, 4096, 8192) for t in (12, 24, 48, 96)),
-    "bras_max": (2, 4, 8, 16, 32),
-    "lds_pack": (8192, 16384, 32768, 49152, 65536),
+    ("chunk_items", "target_prim"): tuple((c, t) for c in (512, 1024, 2048, 4096, 8192, 16384)
+                                          for t in (6, 12, 24, 48, 96, 192)),
+    "bras_max": (1, 2, 4, 8, 16, 32, 64),
+    "lds_pack": (4096, 8192, 16384, 32768, 49152, 65536),
 }
 
 
